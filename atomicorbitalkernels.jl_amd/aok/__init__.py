@@ -1,0 +1,27 @@
+"""aok -- MI355X (gfx950) implementation of AtomicOrbitalKernels.jl's two hot paths.
+
+H1  batched atomic-orbital evaluation (values, spatial gradients, (zeta, D, x) pullback);
+H2  batched 2-/3-centre Cartesian-Gaussian overlaps and the 2-centre (zeta, D) pullback.
+
+All compute runs in libaok.so (hand-written HIP kernels); this package is the host
+mirror of the reference's API.  Importing it does not load the library; the first
+call does, and raises ``AOKLibraryError`` if it is not built.
+"""
+from ._lib import (AOKArgumentError, AOKDeviceError, AOKError, AOKLibraryError, AOKLmaxError, AOKSpeciesError,
+                   Context, context)
+from .basis import (ANG2BOHR, AtomicOrbitals, CartesianGTOBasis, GaussianTypeRadials, SlaterTypeRadials, as_fortran,
+                    compile_basis, compile_coef, compile_coef_pb, fortran_from_numpy, fortran_zeros,
+                    gaussian_orbitals, length_factor, slater_orbitals)
+from .orbitals import (NO_TANGENT, NoTangent, PStates, evaluate, evaluate_ed, pullback_ps, pullback_x,
+                       rrule_evaluate)
+from .overlap import (batch_overlap, batch_overlap_, overlap_pb_coef_alpha, rrule_batch_overlap)
+from .overlap import pullback_ps as overlap_pullback_ps
+
+__all__ = [
+    "ANG2BOHR", "AOKArgumentError", "AOKDeviceError", "AOKError", "AOKLibraryError", "AOKLmaxError",
+    "AOKSpeciesError", "AtomicOrbitals", "CartesianGTOBasis", "Context", "GaussianTypeRadials", "NO_TANGENT",
+    "NoTangent", "PStates", "SlaterTypeRadials", "as_fortran", "batch_overlap", "batch_overlap_", "compile_basis",
+    "compile_coef", "compile_coef_pb", "context", "evaluate", "evaluate_ed", "fortran_from_numpy", "fortran_zeros",
+    "gaussian_orbitals", "length_factor", "overlap_pb_coef_alpha", "overlap_pullback_ps", "pullback_ps",
+    "pullback_x", "rrule_batch_overlap", "rrule_evaluate", "slater_orbitals",
+]

@@ -1,0 +1,205 @@
+"""ctypes binding of libaok.so (the C ABI in include/aok.h).
+
+The library is the only compute path: there is no CPU fallback.  If libaok.so is
+missing or cannot be loaded, every entry point raises ``AOKLibraryError``.
+Device memory comes from torch tensors (``data_ptr()``); the library runs on the
+context's own HIP stream and every call is blocking, like the reference's
+wrappers (which end in ``KA.synchronize``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libaok.so")
+
+AOK_OK = 0
+AOK_ERR_ARG = 1
+AOK_ERR_SPECIES = 2
+AOK_ERR_LMAX = 3
+AOK_ERR_DEVICE = 4
+AOK_ERR_NOMEM = 5
+AOK_ERR_UNSUPPORTED = 6
+
+AOK_F32 = 0
+AOK_F64 = 1
+AOK_I64 = 0
+AOK_I32 = 1
+AOK_DECAY_GAUSSIAN = 0
+AOK_DECAY_SLATER = 1
+AOK_FAST = 0
+AOK_PLAIN = 1
+
+AO_LMAX = 4
+OVL_LMAX = 4
+
+
+class AOKError(RuntimeError):
+    """Base class of errors reported by libaok."""
+
+
+class AOKLibraryError(AOKError):
+    """libaok.so is not built or cannot be loaded (no fallback exists)."""
+
+
+class AOKArgumentError(AOKError, ValueError):
+    """Invalid argument or shape mismatch (the reference raises ErrorException /
+    AssertionError before launching)."""
+
+
+class AOKSpeciesError(AOKError, ValueError):
+    """A species index outside 1..NZ (the reference: "species ... not in basis species list")."""
+
+
+class AOKLmaxError(AOKError, ValueError):
+    """Angular momentum beyond what the kernels are built for."""
+
+
+class AOKDeviceError(AOKError):
+    """HIP runtime failure."""
+
+
+_ERRORS = {
+    AOK_ERR_ARG: AOKArgumentError,
+    AOK_ERR_SPECIES: AOKSpeciesError,
+    AOK_ERR_LMAX: AOKLmaxError,
+    AOK_ERR_DEVICE: AOKDeviceError,
+    AOK_ERR_NOMEM: AOKDeviceError,
+    AOK_ERR_UNSUPPORTED: AOKArgumentError,
+}
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_int = ctypes.c_int
+_uint = ctypes.c_uint
+_dbl = ctypes.c_double
+
+# name -> argtypes (restype aok_status = int unless noted)
+_SIGS = {
+    "aok_create": [ctypes.POINTER(_vp), _int, _vp],
+    "aok_destroy": [_vp],
+    "aok_set_timing": [_vp, _int],
+    "aok_ao_eval": [_vp, _int, _vp, _vp, _int, _i64, _vp, _vp, _i64, _i64, _i64, _i64p, _i64p, _i64p, _i64,
+                    _int, _dbl, _int, _vp, _vp],
+    "aok_ao_pullback": [_vp, _int, _vp, _vp, _int, _i64, _vp, _vp, _i64, _i64, _i64, _i64p, _i64p, _i64p, _i64,
+                        _int, _dbl, _int, _vp, _vp, _vp, _vp],
+    "aok_compile_coef": [_vp, _int, _vp, _vp, _i64, _i64, _i64, _i64p, _vp],
+    "aok_compile_coef_pb": [_vp, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64p, _vp, _vp],
+    "aok_ovl2c": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp, _dbl, _vp],
+    "aok_ovl2c_ps": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp, _dbl,
+                     _vp],
+    "aok_ovl2c_pullback": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp,
+                           _dbl, _vp, _vp],
+    "aok_ovl2c_pullback_ps": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp,
+                              _vp, _dbl, _vp, _vp],
+    "aok_ovl3c": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp,
+                  _dbl, _vp],
+}
+
+# Every symbol include/aok.h declares (checked by the CPU test suite).
+EXPORTED = sorted(list(_SIGS) + ["aok_last_error", "aok_stream", "aok_version", "aok_last_kernel_ms",
+                                 "aok_last_kernel_name"])
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libaok.so (once) and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise AOKLibraryError(f"{LIB_PATH} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        try:
+            L = ctypes.CDLL(LIB_PATH)
+        except OSError as e:  # pragma: no cover - depends on the box
+            raise AOKLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _int
+        L.aok_last_error.argtypes = [_vp]
+        L.aok_last_error.restype = ctypes.c_char_p
+        L.aok_stream.argtypes = [_vp]
+        L.aok_stream.restype = _vp
+        L.aok_version.argtypes = []
+        L.aok_version.restype = ctypes.c_char_p
+        L.aok_last_kernel_ms.argtypes = [_vp]
+        L.aok_last_kernel_ms.restype = _dbl
+        L.aok_last_kernel_name.argtypes = [_vp]
+        L.aok_last_kernel_name.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+class Context:
+    """One libaok context: a HIP stream (private, or torch's current stream when
+    ``use_torch_stream``), workspace and error state."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.lib = load()
+        h = _vp()
+        rc = self.lib.aok_create(ctypes.byref(h), int(device), _vp(stream) if stream else None)
+        if rc != AOK_OK:
+            raise AOKDeviceError(f"aok_create failed with status {rc}")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.aok_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int, what: str):
+        if rc == AOK_OK:
+            return
+        msg = self.lib.aok_last_error(self.handle).decode(errors="replace")
+        raise _ERRORS.get(rc, AOKError)(f"{what}: {msg} (status {rc})")
+
+    def call(self, name: str, *args):
+        rc = getattr(self.lib, name)(self.handle, *args)
+        self.check(rc, name)
+
+    @property
+    def stream(self) -> int:
+        return self.lib.aok_stream(self.handle)
+
+    def set_timing(self, on: bool):
+        self.lib.aok_set_timing(self.handle, 1 if on else 0)
+
+    def last_kernel_ms(self) -> float:
+        return float(self.lib.aok_last_kernel_ms(self.handle))
+
+    def last_kernel_name(self) -> str:
+        return self.lib.aok_last_kernel_name(self.handle).decode()
+
+
+_ctx = {}
+
+
+def context(device: int = 0) -> Context:
+    """Process-wide default context per device (created on first use)."""
+    c = _ctx.get(device)
+    if c is None:
+        c = _ctx[device] = Context(device)
+    return c
+
+
+def i64_array(values):
+    """Host int64 array (kept alive by the caller) for the structural arguments."""
+    vals = [int(v) for v in values]
+    arr = (ctypes.c_int64 * max(1, len(vals)))(*vals)
+    return arr

@@ -1,0 +1,188 @@
+"""H2 host layer: batched 2-centre overlaps and their (zeta, D) pullback.
+
+Mirrors src/integrals/orbital/overlap_2c.jl:171-235 and overlap_2c_grad.jl:22-318:
+  batch_overlap(basis::CartesianGTOBasis, XA, XB)            -> out (N, N, B)
+  batch_overlap_(out, basis, XA, XB)                          (the mutating ``!`` form)
+  batch_overlap(orb::AtomicOrbitals, XA, XB[, ps, st])        (compile-on-call / Lux form)
+  pullback_ps(dS, orb, XA, XB, ps, st)                        -> {'Rnl': {'zeta', 'D'}, 'Ylm': {}}
+  rrule_batch_overlap(orb, XA, XB, ps, st)                    -> (S, pullback)
+  overlap_pb_coef_alpha(dS, basis, XA, XB)                    -> (dcoef, dalpha)
+``XA``/``XB`` are ``PStates`` (positions in the basis' length unit + species labels),
+or ``(positions, sidx)`` tuples for the kernel-facing layer.  ``mode`` selects the
+fast kernels (default) or the reference-decomposition ``"plain"`` kernels.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .basis import (AtomicOrbitals, CartesianGTOBasis, _dtype_code, as_fortran, compile_basis, compile_struct,
+                    fortran_zeros)
+from .orbitals import NO_TANGENT, PStates
+
+
+def _flags(mode: str) -> int:
+    if mode == "fast":
+        return _lib.AOK_FAST
+    if mode == "plain":
+        return _lib.AOK_PLAIN
+    raise _lib.AOKArgumentError(f"unknown mode {mode!r}")
+
+
+def _unpack(X, zlist, what):
+    """_cgto_posmat + _cgto_sidx (overlap_2c.jl:139-158)."""
+    if isinstance(X, PStates):
+        lut = {z: i + 1 for i, z in enumerate(zlist)}
+        idx = []
+        for s in X.species:
+            s = s.item() if hasattr(s, "item") else s
+            if s not in lut:
+                raise _lib.AOKSpeciesError(f"species {s} not in basis species list {zlist}")
+            idx.append(lut[s])
+        pos = X.positions.contiguous()
+        return pos, torch.tensor(idx, dtype=torch.int64, device=pos.device)
+    if isinstance(X, tuple) and len(X) == 2:
+        pos, sidx = X
+        return pos.contiguous(), (sidx.contiguous() if sidx is not None else None)
+    raise _lib.AOKArgumentError(f"{what} must be PStates or a (positions, sidx) tuple")
+
+
+def _sidx_ptr(sidx):
+    if sidx is None:
+        return None, _lib.AOK_I64
+    if sidx.dtype == torch.int64:
+        return sidx.data_ptr(), _lib.AOK_I64
+    if sidx.dtype == torch.int32:
+        return sidx.data_ptr(), _lib.AOK_I32
+    raise _lib.AOKArgumentError("species indices must be int64 or int32")
+
+
+def _pair_inputs(XA, XB, zlist):
+    pA, sA = _unpack(XA, zlist, "XA")
+    pB, sB = _unpack(XB, zlist, "XB")
+    if pA.shape[0] != pB.shape[0]:  # overlap_2c.jl:177-178
+        raise _lib.AOKArgumentError(f"XA and XB must have equal length; got {pA.shape[0]} and {pB.shape[0]}")
+    if pA.shape[1:] != (3,) or pB.shape[1:] != (3,):
+        raise _lib.AOKArgumentError("positions must have shape (B, 3)")
+    return pA, sA, pB, sB
+
+
+def batch_overlap_(out, basis: CartesianGTOBasis, XA, XB, *, mode="fast", ctx=None):
+    """batch_overlap!(out, basis, XA, XB) (overlap_2c.jl:171-197); ``out`` is (N, N, B),
+    column-major, and sets the kernel precision."""
+    pA, sA, pB, sB = _pair_inputs(XA, XB, basis.zlist)
+    B = pA.shape[0]
+    N = basis.nbf_total
+    if tuple(out.shape) != (N, N, B):
+        raise _lib.AOKArgumentError(f"out must be {(N, N, B)}, got {tuple(out.shape)}")
+    if not out.permute(2, 1, 0).is_contiguous():
+        raise _lib.AOKArgumentError("out must be column-major contiguous")
+    dt = out.dtype
+    pA, pB = pA.to(dt), pB.to(dt)
+    coef = as_fortran(basis.coef.to(dt))
+    zeta = as_fortran(basis.zeta.to(dt))
+    ctx = ctx or _lib.context(out.device.index or 0)
+    a, it = _sidx_ptr(sA)
+    b, _ = _sidx_ptr(sB)
+    ns, K, NZ = zeta.shape
+    ctx.call("aok_ovl2c", _dtype_code(dt), _flags(mode), pA.data_ptr(), pB.data_ptr(), a, b, it, B, ns, K, NZ,
+             _lib.i64_array(basis.ls), coef.data_ptr(), zeta.data_ptr(), float(basis.lengthscale), out.data_ptr())
+    return out
+
+
+def _out_tensor(N, B, dtype, device):
+    return torch.empty((B, N, N), dtype=dtype, device=device).permute(2, 1, 0)
+
+
+def batch_overlap(basis, XA, XB, ps=None, st=None, *, mode="fast", dtype=None, ctx=None):
+    """batch_overlap(basis::CartesianGTOBasis, XA, XB) or batch_overlap(orb, XA, XB[, ps, st]).
+    The output precision follows the positions (promoted with (zeta, D) in the Lux form)."""
+    if isinstance(basis, CartesianGTOBasis):
+        pA = XA.positions if isinstance(XA, PStates) else XA[0]
+        dt = dtype or pA.dtype
+        N = basis.nbf_total
+        out = _out_tensor(N, pA.shape[0], dt, pA.device)
+        return batch_overlap_(out, basis, XA, XB, mode=mode, ctx=ctx)
+    if not isinstance(basis, AtomicOrbitals):
+        raise _lib.AOKArgumentError("basis must be a CartesianGTOBasis or an AtomicOrbitals")
+    orb = basis
+    if ps is None:  # batch_overlap(orb, XA, XB) = batch_overlap(compile_basis(orb), XA, XB)
+        return batch_overlap(compile_basis(orb), XA, XB, mode=mode, dtype=dtype, ctx=ctx)
+    # Lux form: coef recompiled from ps on the device (overlap_2c_grad.jl:41-50)
+    if orb.Rnl.decay != _lib.AOK_DECAY_GAUSSIAN:
+        raise _lib.AOKArgumentError("the overlap kernels integrate plain Gaussians")
+    ls, nbf, off = compile_struct(orb)
+    pA, sA, pB, sB = _pair_inputs(XA, XB, orb.Rnl.zlist)
+    zeta = ps["Rnl"]["zeta"]
+    D = ps["Rnl"]["D"]
+    dt = dtype or torch.promote_types(pA.dtype, torch.promote_types(zeta.dtype, D.dtype))
+    zeta, D = as_fortran(zeta.to(dt)), as_fortran(D.to(dt))
+    pA, pB = pA.to(dt), pB.to(dt)
+    B = pA.shape[0]
+    out = _out_tensor(off[-1], B, dt, pA.device)
+    ctx = ctx or _lib.context(pA.device.index or 0)
+    a, it = _sidx_ptr(sA)
+    b, _ = _sidx_ptr(sB)
+    ns, K, NZ = zeta.shape
+    ctx.call("aok_ovl2c_ps", _dtype_code(dt), _flags(mode), pA.data_ptr(), pB.data_ptr(), a, b, it, B, ns, K, NZ,
+             _lib.i64_array(ls), zeta.data_ptr(), D.data_ptr(), float(orb.lengthscale), out.data_ptr())
+    return out
+
+
+def overlap_pb_coef_alpha(dS, basis: CartesianGTOBasis, XA, XB, *, mode="fast", ctx=None):
+    """_overlap_pb_coef_alpha (overlap_2c_grad.jl:218-242) -> (dcoef, dalpha), (ns, K, NZ)."""
+    pA, sA, pB, sB = _pair_inputs(XA, XB, basis.zlist)
+    dt = torch.promote_types(dS.dtype, basis.coef.dtype)
+    dS = as_fortran(dS.to(dt))
+    B = pA.shape[0]
+    N = basis.nbf_total
+    if tuple(dS.shape) != (N, N, B):
+        raise _lib.AOKArgumentError(f"dS must be {(N, N, B)}")
+    pA, pB = pA.to(dt), pB.to(dt)
+    coef = as_fortran(basis.coef.to(dt))
+    zeta = as_fortran(basis.zeta.to(dt))
+    ns, K, NZ = zeta.shape
+    dc = fortran_zeros(zeta.shape, dt, dS.device)
+    da = fortran_zeros(zeta.shape, dt, dS.device)
+    ctx = ctx or _lib.context(dS.device.index or 0)
+    a, it = _sidx_ptr(sA)
+    b, _ = _sidx_ptr(sB)
+    ctx.call("aok_ovl2c_pullback", _dtype_code(dt), _flags(mode), dS.data_ptr(), pA.data_ptr(), pB.data_ptr(), a, b,
+             it, B, ns, K, NZ, _lib.i64_array(basis.ls), coef.data_ptr(), zeta.data_ptr(), float(basis.lengthscale),
+             dc.data_ptr(), da.data_ptr())
+    return dc, da
+
+
+def pullback_ps(dS, orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", ctx=None):
+    """pullback_ps(dS, orb, XA, XB, ps, st) (overlap_2c_grad.jl:296-304)."""
+    ls, _, off = compile_struct(orb)
+    pA, sA, pB, sB = _pair_inputs(XA, XB, orb.Rnl.zlist)
+    zeta, D = ps["Rnl"]["zeta"], ps["Rnl"]["D"]
+    dt = torch.promote_types(dS.dtype, torch.promote_types(zeta.dtype, D.dtype))
+    zeta, D = as_fortran(zeta.to(dt)), as_fortran(D.to(dt))
+    dS = as_fortran(dS.to(dt))
+    B = pA.shape[0]
+    if tuple(dS.shape) != (off[-1], off[-1], B):
+        raise _lib.AOKArgumentError(f"dS must be {(off[-1], off[-1], B)}")
+    pA, pB = pA.to(dt), pB.to(dt)
+    ns, K, NZ = zeta.shape
+    dz = fortran_zeros(zeta.shape, dt, dS.device)
+    dD = fortran_zeros(zeta.shape, dt, dS.device)
+    ctx = ctx or _lib.context(dS.device.index or 0)
+    a, it = _sidx_ptr(sA)
+    b, _ = _sidx_ptr(sB)
+    ctx.call("aok_ovl2c_pullback_ps", _dtype_code(dt), _flags(mode), dS.data_ptr(), pA.data_ptr(), pB.data_ptr(), a,
+             b, it, B, ns, K, NZ, _lib.i64_array(ls), zeta.data_ptr(), D.data_ptr(), float(orb.lengthscale),
+             dz.data_ptr(), dD.data_ptr())
+    return {"Rnl": {"zeta": dz, "D": dD}, "Ylm": {}}
+
+
+def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", ctx=None):
+    """rrule(batch_overlap, orb, XA, XB, ps, st) (overlap_2c_grad.jl:309-318)."""
+    S = batch_overlap(orb, XA, XB, ps, st, mode=mode, ctx=ctx)
+
+    def pullback(dS):
+        dps = pullback_ps(dS, orb, XA, XB, ps, st, mode=mode, ctx=ctx)
+        return NO_TANGENT, NO_TANGENT, NO_TANGENT, NO_TANGENT, dps, NO_TANGENT
+
+    return S, pullback

@@ -1,0 +1,95 @@
+// ao_plan.hpp -- host-side planning of the orbital loop for the AO kernels.
+//
+// The reference maps orbital o to radial iR[o] and harmonic iY[o]
+// (src/orbitals/atomicorbitals.jl:56-75).  For the maps AtomicOrbitals builds, the
+// orbitals of radial k are 2l+1 consecutive columns with harmonics l^2 .. l^2+2l, so
+// the kernels walk "runs": (k, l, first column).  Any other map is still accepted:
+// orbitals that do not form a full shell become single-orbital runs (l < 0 encodes
+// harmonic -1-l).  The table travels by value in the kernel arguments.
+#pragma once
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/aok.h"
+
+namespace aok {
+
+constexpr int AO_MAXRUN = 192;
+constexpr int AO_MAXRAD = 256;
+
+struct AoRunTable {
+    int nrun;
+    int16_t k[AO_MAXRUN];     // radial index (0-based)
+    int16_t l[AO_MAXRUN];     // >= 0 full shell; < 0 single orbital with harmonic -1-l
+    int16_t o0[AO_MAXRUN];    // first orbital column
+    int16_t poly[AO_MAXRUN];  // r^poly of radial k
+    int16_t kpoly[AO_MAXRAD]; // r^poly per radial (pullback)
+};
+
+struct AoArgs {
+    const void *X;
+    const void *sidx;
+    int itype;
+    int64_t nX;
+    const void *zeta;
+    const void *D;
+    int nRad, K, NZ;
+    int L;
+    double scale;
+    int decay;
+    void *P;
+    void *dP;
+    const void *dPin;
+};
+
+// Build the run table; returns an empty string on success, else the error message.
+inline std::string build_runs(const int64_t *poly, const int64_t *iR, const int64_t *iY, int64_t nOrb,
+                              int64_t nRad, int L, AoRunTable &t)
+{
+    t.nrun = 0;
+    if (nRad > AO_MAXRAD) return "nRad exceeds " + std::to_string(AO_MAXRAD);
+    for (int64_t k = 0; k < nRad; ++k) {
+        const int64_t p = poly ? poly[k] : 0;
+        if (p < 0 || p > 64) return "poly out of range";
+        t.kpoly[k] = static_cast<int16_t>(p);
+    }
+    const int nY = (L + 1) * (L + 1);
+    int64_t o = 0;
+    while (o < nOrb) {
+        const int64_t k = iR[o] - 1, y = iY[o] - 1;
+        if (k < 0 || k >= nRad) return "iR[" + std::to_string(o) + "] out of range";
+        if (y < 0 || y >= nY) return "iY[" + std::to_string(o) + "] out of range for L";
+        if (t.nrun >= AO_MAXRUN) return "too many orbital runs";
+        // full shell?  y must be l^2 and the next 2l columns must follow
+        int l = 0;
+        while ((l + 1) * (l + 1) <= y) ++l;
+        bool full = (y == l * l) && (o + 2 * l < nOrb);
+        if (full)
+            for (int j = 0; j <= 2 * l; ++j)
+                if (iR[o + j] - 1 != k || iY[o + j] - 1 != y + j) {
+                    full = false;
+                    break;
+                }
+        const int q = t.nrun++;
+        t.k[q] = static_cast<int16_t>(k);
+        t.o0[q] = static_cast<int16_t>(o);
+        t.poly[q] = t.kpoly[k];
+        if (full) {
+            t.l[q] = static_cast<int16_t>(l);
+            o += 2 * l + 1;
+        } else {
+            t.l[q] = static_cast<int16_t>(-1 - y);
+            o += 1;
+        }
+    }
+    if (nOrb > 32767) return "nOrb too large";
+    return std::string();
+}
+
+template <typename T>
+aok_status ao_eval_launch(aok_ctx ctx, const AoArgs &a, const AoRunTable &runs);
+template <typename T>
+aok_status ao_pullback_launch(aok_ctx ctx, const AoArgs &a, const AoRunTable &runs, void *dX, void *dzeta, void *dD);
+
+}  // namespace aok

@@ -1,0 +1,118 @@
+// aok_common.hpp -- context, status plumbing and device helpers shared by the
+// libaok.so translation units (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/aok.h"
+
+struct aok_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+    // Grow-only device workspace (partials, bucket tables, primitive-pair tables).
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    // Device error word (kernels OR bits into it) and its pinned host mirror.
+    unsigned *d_err = nullptr;
+    unsigned *h_err = nullptr;
+    // Optional timing of the dominant kernel of a call.
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    const char *last_kernel = "";
+};
+
+namespace aok {
+
+constexpr unsigned ERR_SPECIES = 1u;
+
+// Status helpers ---------------------------------------------------------------
+inline aok_status fail(aok_ctx ctx, aok_status st, const std::string &msg)
+{
+    if (ctx) ctx->last_error = msg;
+    return st;
+}
+
+#define AOK_HIP(ctx, call)                                                                \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::aok::fail((ctx), AOK_ERR_DEVICE,                                     \
+                               std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// Ensure at least `bytes` of workspace; returns the base pointer or nullptr.
+aok_status ensure_ws(aok_ctx ctx, size_t bytes);
+
+// Clear the device error word (a tiny kernel, not a memset, so the sequence stays
+// capturable into a graph).
+aok_status clear_err(aok_ctx ctx);
+// Copy the error word back, synchronise the stream, translate into a status.
+aok_status finish(aok_ctx ctx, const char *what);
+
+// Event-bracketed launch of the dominant kernel (only when timing is enabled).
+inline void time_begin(aok_ctx ctx, const char *name)
+{
+    if (ctx->timing) {
+        (void)hipEventRecord(ctx->ev0, ctx->stream);
+        ctx->last_kernel = name;
+    }
+}
+inline void time_end(aok_ctx ctx)
+{
+    if (ctx->timing) (void)hipEventRecord(ctx->ev1, ctx->stream);
+}
+
+// Device helpers ----------------------------------------------------------------
+// 1-based species index (int64 or int32) -> 0-based sigma, -1 for NULL input.
+__device__ __forceinline__ int load_sigma(const void *sidx, int itype, int64_t i)
+{
+    if (sidx == nullptr) return 0;
+    if (itype == AOK_I32) return static_cast<const int32_t *>(sidx)[i] - 1;
+    return static_cast<int>(static_cast<const int64_t *>(sidx)[i] - 1);
+}
+
+template <typename T>
+__device__ __forceinline__ void store_nt(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+template <typename T>
+__device__ __forceinline__ T fast_exp_neg(T x);  // exp(-x), see specialisations
+
+// exp(-x) for float via the hardware base-2 exponential (v_exp_f32) on x*log2(e).
+template <>
+__device__ __forceinline__ float fast_exp_neg<float>(float x)
+{
+    return __builtin_amdgcn_exp2f(-x * 1.4426950408889634f);
+}
+template <>
+__device__ __forceinline__ double fast_exp_neg<double>(double x)
+{
+    return exp(-x);
+}
+
+// Wave64 butterfly sum.
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+inline int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
+
+}  // namespace aok

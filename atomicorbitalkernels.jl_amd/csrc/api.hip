@@ -1,0 +1,308 @@
+// api.hip -- extern "C" entry points of libaok.so (declared in include/aok.h).
+#include <cstring>
+#include <new>
+
+#include "aok_common.hpp"
+#include "ao_plan.hpp"
+#include "ovl_plan.hpp"
+
+namespace aok {
+
+__global__ void clear_word(unsigned *w) { *w = 0u; }
+
+aok_status ensure_ws(aok_ctx ctx, size_t bytes)
+{
+    if (bytes <= ctx->ws_bytes) return AOK_OK;
+    size_t want = std::max(bytes, ctx->ws_bytes + ctx->ws_bytes / 2);
+    want = (want + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+    if (ctx->ws) {
+        AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        AOK_HIP(ctx, hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    if (hipMalloc(&ctx->ws, want) != hipSuccess) {
+        ctx->ws = nullptr;
+        return fail(ctx, AOK_ERR_NOMEM, "workspace allocation of " + std::to_string(want) + " bytes failed");
+    }
+    ctx->ws_bytes = want;
+    return AOK_OK;
+}
+
+aok_status clear_err(aok_ctx ctx)
+{
+    hipLaunchKernelGGL(clear_word, dim3(1), dim3(1), 0, ctx->stream, ctx->d_err);
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+aok_status finish(aok_ctx ctx, const char *what)
+{
+    AOK_HIP(ctx, hipMemcpyAsync(ctx->h_err, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->timing) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    }
+    if (*ctx->h_err & ERR_SPECIES)
+        return fail(ctx, AOK_ERR_SPECIES, std::string(what) + ": species index outside 1..NZ");
+    return AOK_OK;
+}
+
+static bool bad_dtype(aok_dtype d) { return d != AOK_F32 && d != AOK_F64; }
+static bool bad_itype(aok_itype t) { return t != AOK_I64 && t != AOK_I32; }
+
+}  // namespace aok
+
+using namespace aok;
+
+#define AOK_CHECK_CTX(ctx) \
+    if (!(ctx)) return AOK_ERR_ARG
+#define AOK_REQUIRE(ctx, cond, msg) \
+    if (!(cond)) return ::aok::fail((ctx), AOK_ERR_ARG, (msg))
+
+extern "C" {
+
+aok_status aok_create(aok_ctx *out, int device, void *stream)
+{
+    if (!out) return AOK_ERR_ARG;
+    *out = nullptr;
+    aok_context *c = new (std::nothrow) aok_context();
+    if (!c) return AOK_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return AOK_ERR_DEVICE;
+    }
+    if (stream) {
+        c->stream = static_cast<hipStream_t>(stream);
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return AOK_ERR_DEVICE;
+        }
+        c->own_stream = true;
+    }
+    if (hipMalloc(&c->d_err, sizeof(unsigned)) != hipSuccess ||
+        hipHostMalloc(&c->h_err, sizeof(unsigned), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        aok_destroy(c);
+        return AOK_ERR_DEVICE;
+    }
+    *out = c;
+    return AOK_OK;
+}
+
+aok_status aok_destroy(aok_ctx ctx)
+{
+    if (!ctx) return AOK_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->h_err) (void)hipHostFree(ctx->h_err);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return AOK_OK;
+}
+
+const char *aok_last_error(aok_ctx ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+void *aok_stream(aok_ctx ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+const char *aok_version(void) { return "aok-mi355x 0.1.0 (gfx950)"; }
+
+double aok_last_kernel_ms(aok_ctx ctx) { return ctx ? ctx->last_ms : 0.0; }
+
+const char *aok_last_kernel_name(aok_ctx ctx) { return ctx ? ctx->last_kernel : ""; }
+
+aok_status aok_set_timing(aok_ctx ctx, int enable)
+{
+    AOK_CHECK_CTX(ctx);
+    ctx->timing = enable != 0;
+    return AOK_OK;
+}
+
+// ---- H1 --------------------------------------------------------------------------------
+static aok_status ao_common(aok_ctx ctx, aok_dtype dtype, const void *X, aok_itype itype, int64_t nX,
+                            const void *zeta, const void *D, int64_t nRad, int64_t K, int64_t NZ,
+                            const int64_t *poly, const int64_t *iR, const int64_t *iY, int64_t nOrb, int L,
+                            aok_decay decay, AoRunTable &runs)
+{
+    AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
+    AOK_REQUIRE(ctx, !bad_itype(itype), "itype must be AOK_I64 or AOK_I32");
+    AOK_REQUIRE(ctx, nX >= 0, "nX must be >= 0");
+    AOK_REQUIRE(ctx, nX == 0 || X != nullptr, "X is NULL");
+    AOK_REQUIRE(ctx, zeta && D, "zeta/D are NULL");
+    AOK_REQUIRE(ctx, nRad >= 1 && K >= 1 && NZ >= 1, "nRad, K, NZ must be >= 1");
+    AOK_REQUIRE(ctx, NZ * nRad * K <= 16384, "parameter table too large");
+    AOK_REQUIRE(ctx, nOrb >= 1 && iR && iY, "orbital maps iR/iY required");
+    AOK_REQUIRE(ctx, decay == AOK_DECAY_GAUSSIAN || decay == AOK_DECAY_SLATER, "unknown decay");
+    if (L < 0 || L > AOK_AO_LMAX)
+        return fail(ctx, AOK_ERR_LMAX, "L = " + std::to_string(L) + " outside 0.." + std::to_string(AOK_AO_LMAX));
+    const std::string msg = build_runs(poly, iR, iY, nOrb, nRad, L, runs);
+    if (!msg.empty()) return fail(ctx, AOK_ERR_ARG, msg);
+    return AOK_OK;
+}
+
+aok_status aok_ao_eval(aok_ctx ctx, aok_dtype dtype, const void *X, const void *sidx, aok_itype itype, int64_t nX,
+                       const void *zeta, const void *D, int64_t nRad, int64_t K, int64_t NZ, const int64_t *poly,
+                       const int64_t *iR, const int64_t *iY, int64_t nOrb, int L, double lengthscale,
+                       aok_decay decay, void *P, void *dP)
+{
+    AOK_CHECK_CTX(ctx);
+    AoRunTable runs;
+    aok_status st = ao_common(ctx, dtype, X, itype, nX, zeta, D, nRad, K, NZ, poly, iR, iY, nOrb, L, decay, runs);
+    if (st != AOK_OK) return st;
+    AOK_REQUIRE(ctx, nX == 0 || P != nullptr, "P is NULL");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    if ((st = clear_err(ctx)) != AOK_OK) return st;
+    AoArgs a{X, sidx, itype, nX, zeta, D, (int)nRad, (int)K, (int)NZ, L, lengthscale, decay, P, dP, nullptr};
+    st = dtype == AOK_F32 ? ao_eval_launch<float>(ctx, a, runs) : ao_eval_launch<double>(ctx, a, runs);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_ao_eval");
+}
+
+aok_status aok_ao_pullback(aok_ctx ctx, aok_dtype dtype, const void *X, const void *sidx, aok_itype itype,
+                           int64_t nX, const void *zeta, const void *D, int64_t nRad, int64_t K, int64_t NZ,
+                           const int64_t *poly, const int64_t *iR, const int64_t *iY, int64_t nOrb, int L,
+                           double lengthscale, aok_decay decay, const void *dPin, void *dX, void *dzeta, void *dD)
+{
+    AOK_CHECK_CTX(ctx);
+    AoRunTable runs;
+    aok_status st = ao_common(ctx, dtype, X, itype, nX, zeta, D, nRad, K, NZ, poly, iR, iY, nOrb, L, decay, runs);
+    if (st != AOK_OK) return st;
+    AOK_REQUIRE(ctx, nX == 0 || dPin != nullptr, "dPin is NULL");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    if ((st = clear_err(ctx)) != AOK_OK) return st;
+    AoArgs a{X, sidx, itype, nX, zeta, D, (int)nRad, (int)K, (int)NZ, L, lengthscale, decay, nullptr, nullptr, dPin};
+    st = dtype == AOK_F32 ? ao_pullback_launch<float>(ctx, a, runs, dX, dzeta, dD)
+                          : ao_pullback_launch<double>(ctx, a, runs, dX, dzeta, dD);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_ao_pullback");
+}
+
+// ---- H2 normalisation -------------------------------------------------------------------
+aok_status aok_compile_coef(aok_ctx ctx, aok_dtype dtype, const void *zeta, const void *D, int64_t ns, int64_t K,
+                            int64_t NZ, const int64_t *ls, void *coef)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
+    AOK_REQUIRE(ctx, zeta && D && coef && ls, "NULL argument");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    st = dtype == AOK_F32 ? compile_coef_run<float>(ctx, zeta, D, ns, K, NZ, ls, coef)
+                          : compile_coef_run<double>(ctx, zeta, D, ns, K, NZ, ls, coef);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_compile_coef");
+}
+
+aok_status aok_compile_coef_pb(aok_ctx ctx, aok_dtype dtype, const void *dcoef, const void *zeta, const void *D,
+                               int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, void *dD, void *dzeta_norm)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
+    AOK_REQUIRE(ctx, dcoef && zeta && D && ls && dD && dzeta_norm, "NULL argument");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    st = dtype == AOK_F32 ? compile_coef_pb_run<float>(ctx, dcoef, zeta, D, ns, K, NZ, ls, dD, dzeta_norm)
+                          : compile_coef_pb_run<double>(ctx, dcoef, zeta, D, ns, K, NZ, ls, dD, dzeta_norm);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_compile_coef_pb");
+}
+
+// ---- H2 2-centre ------------------------------------------------------------------------
+static aok_status ovl2c_entry(aok_ctx ctx, aok_dtype dtype, Ovl2cCall &c, bool pullback, const char *what)
+{
+    AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
+    AOK_REQUIRE(ctx, !bad_itype(static_cast<aok_itype>(c.itype)), "itype must be AOK_I64 or AOK_I32");
+    AOK_REQUIRE(ctx, (c.flags & ~AOK_PLAIN) == 0u, "unknown flags");
+    AOK_REQUIRE(ctx, c.ls != nullptr, "ls is NULL");
+    AOK_REQUIRE(ctx, c.B >= 0, "B must be >= 0");
+    AOK_REQUIRE(ctx, c.B == 0 || (c.posA && c.posB), "positions are NULL");
+    AOK_REQUIRE(ctx, c.alpha != nullptr, "exponents are NULL");
+    AOK_REQUIRE(ctx, c.mode == 0 ? c.coef != nullptr : c.D != nullptr, "coefficients are NULL");
+    if (!pullback) AOK_REQUIRE(ctx, c.B == 0 || c.out != nullptr, "out is NULL");
+    if (pullback) AOK_REQUIRE(ctx, c.B == 0 || c.dS != nullptr, "dS is NULL");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    st = dtype == AOK_F32 ? ovl2c_run<float>(ctx, c, pullback) : ovl2c_run<double>(ctx, c, pullback);
+    if (st != AOK_OK) return st;
+    return finish(ctx, what);
+}
+
+aok_status aok_ovl2c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *posA, const void *posB,
+                     const void *sidxA, const void *sidxB, aok_itype itype, int64_t B, int64_t ns, int64_t K,
+                     int64_t NZ, const int64_t *ls, const void *coef, const void *alpha, double lengthscale,
+                     void *out)
+{
+    AOK_CHECK_CTX(ctx);
+    Ovl2cCall c{};
+    c.flags = flags;
+    c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
+    c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.coef = coef, c.alpha = alpha, c.mode = 0, c.scale = lengthscale;
+    c.out = out;
+    return ovl2c_entry(ctx, dtype, c, false, "aok_ovl2c");
+}
+
+aok_status aok_ovl2c_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *posA, const void *posB,
+                        const void *sidxA, const void *sidxB, aok_itype itype, int64_t B, int64_t ns, int64_t K,
+                        int64_t NZ, const int64_t *ls, const void *zeta, const void *D, double lengthscale,
+                        void *out)
+{
+    AOK_CHECK_CTX(ctx);
+    Ovl2cCall c{};
+    c.flags = flags;
+    c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
+    c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.alpha = zeta, c.D = D, c.mode = 1, c.scale = lengthscale;
+    c.out = out;
+    return ovl2c_entry(ctx, dtype, c, false, "aok_ovl2c_ps");
+}
+
+aok_status aok_ovl2c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dS, const void *posA,
+                              const void *posB, const void *sidxA, const void *sidxB, aok_itype itype, int64_t B,
+                              int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, const void *coef,
+                              const void *alpha, double lengthscale, void *dcoef, void *dalpha)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, dcoef && dalpha, "dcoef/dalpha are NULL");
+    Ovl2cCall c{};
+    c.flags = flags;
+    c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
+    c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.coef = coef, c.alpha = alpha, c.mode = 0, c.scale = lengthscale;
+    c.dS = dS, c.o1 = dcoef, c.o2 = dalpha, c.vjp = false;
+    return ovl2c_entry(ctx, dtype, c, true, "aok_ovl2c_pullback");
+}
+
+aok_status aok_ovl2c_pullback_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dS, const void *posA,
+                                 const void *posB, const void *sidxA, const void *sidxB, aok_itype itype, int64_t B,
+                                 int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, const void *zeta,
+                                 const void *D, double lengthscale, void *dzeta, void *dD)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, dzeta && dD, "dzeta/dD are NULL");
+    Ovl2cCall c{};
+    c.flags = flags;
+    c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
+    c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.alpha = zeta, c.D = D, c.mode = 1, c.scale = lengthscale;
+    c.dS = dS, c.o1 = dzeta, c.o2 = dD, c.vjp = true;
+    return ovl2c_entry(ctx, dtype, c, true, "aok_ovl2c_pullback_ps");
+}
+
+aok_status aok_ovl3c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *posA, const void *posB,
+                     const void *posC, const void *sidxA, const void *sidxB, const void *sidxC, aok_itype itype,
+                     int64_t B, int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, const void *coef,
+                     const void *alpha, double lengthscale, void *out)
+{
+    AOK_CHECK_CTX(ctx);
+    (void)dtype, (void)flags, (void)posA, (void)posB, (void)posC, (void)sidxA, (void)sidxB, (void)sidxC;
+    (void)itype, (void)B, (void)ns, (void)K, (void)NZ, (void)ls, (void)coef, (void)alpha, (void)lengthscale;
+    (void)out;
+    return fail(ctx, AOK_ERR_UNSUPPORTED, "aok_ovl3c: not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,1185 @@
+// ovl2c_kernels.hip -- H2: batched 2-centre Cartesian-Gaussian overlaps and their
+// (coef, alpha) pullback on gfx950.
+//
+// Semantics follow batch_S_orbital_kernel! (src/integrals/orbital/overlap_2c.jl:165-281)
+// and batch_S_orbital_pb_kernel! (src/integrals/orbital/overlap_2c_grad.jl:71-214):
+// Cartesian components ordered (x,y,z) = (ll, l-ll-n, n) for ll = l..0, n = 0..l-ll;
+// contracted values accumulated at lin = index1 + (2 l_a + 1) index2 and read back
+// as out[off_a + i, off_b + j] = blk[i + nbf_a j] (the reference's l >= 2 aliasing);
+// forward skips primitive pairs with a zero coefficient, the pullback does not.
+//
+// Fast path (AOK_FAST): see ovl_plan.hpp.  Per primitive pair and lane the work is
+//   e = exp(-mu |AB|^2) with the exponent argument formed and range-reduced in FP64
+//       (y = mu log2(e) |AB|^2, y = n + f, e = 2^-n * v_exp_f32(-f)), so the FP32 result
+//       carries no error proportional to the argument;
+//   1-D Obara-Saika overlaps s_ij (s_00 = 1) per axis in registers, from
+//       PA = -(beta/p) AB and PB = (alpha/p) AB;
+//   contraction G s_x s_y s_z into compile-time-indexed accumulators.
+// This is the same quantity as the MD E^{ij}_0 product (E^{ij}_0 = exp(-mu X_AB^2) s_ij).
+//
+// Plain path (AOK_PLAIN): the reference's own decomposition -- one work item per
+// (b, s_a, s_b), runtime l loops, private E scratch, three exps per primitive pair,
+// float atomics in the pullback -- kept as the baseline the fast path is measured against.
+#include "aok_common.hpp"
+#include "ovl_plan.hpp"
+
+namespace aok {
+
+// ---------------------------------------------------------------------------------
+// FP64 helpers (device)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double cart_cl(int l)
+{
+    double df = 1.0;
+    for (int k = 2 * l - 1; k > 1; k -= 2) df *= k;
+    return 5.568327996831707845 * df / static_cast<double>(1 << l);  // pi^{3/2} (2l-1)!! / 2^l
+}
+
+// normsq of one (shell, species) contraction (cartesian_gto_basis.jl:74-83)
+__device__ double cart_normsq(const double *Dv, const double *zv, int K, int l)
+{
+    const double e = l + 1.5;
+    double s = 0.0;
+    for (int i = 0; i < K; ++i)
+        for (int j = 0; j < K; ++j) s += Dv[i] * Dv[j] / pow(zv[i] + zv[j], e);
+    return cart_cl(l) * s;
+}
+
+// ---------------------------------------------------------------------------------
+// Prep 1: species-pair keys, stable in-block ranks, per-block histograms.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ovl_hist_kernel(const void *__restrict__ sidxA, const void *__restrict__ sidxB,
+                                                       int itype, int64_t B, int NZ, int *__restrict__ rank,
+                                                       int *__restrict__ blk_hist, unsigned *__restrict__ err)
+{
+    __shared__ int keys[256];
+    __shared__ int hist[OVL_MAXNZ * OVL_MAXNZ];
+    const int NK = NZ * NZ;
+    for (int q = threadIdx.x; q < NK; q += 256) hist[q] = 0;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    int key = -1;
+    if (i < B) {
+        int sa = load_sigma(sidxA, itype, i), sb = load_sigma(sidxB, itype, i);
+        if (sa < 0 || sa >= NZ || sb < 0 || sb >= NZ) {
+            atomicOr(err, ERR_SPECIES);
+            sa = sa < 0 || sa >= NZ ? 0 : sa;
+            sb = sb < 0 || sb >= NZ ? 0 : sb;
+        }
+        key = sa * NZ + sb;
+    }
+    keys[threadIdx.x] = key;
+    __syncthreads();
+    if (i < B) {
+        int r = 0;
+        for (int j = 0; j < static_cast<int>(threadIdx.x); ++j) r += (keys[j] == key);
+        rank[i] = r;
+        atomicAdd(&hist[key], 1);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NK; q += 256) blk_hist[static_cast<int64_t>(blockIdx.x) * NK + q] = hist[q];
+}
+
+// ---------------------------------------------------------------------------------
+// Prep 2 (one block): primitive lists per (species, shell) and bucket offsets.
+//   mode 0: coef given (cast to FP64); mode 1: coef = D / sqrt(normsq) from (zeta, D).
+// List of a (species, shell): its real primitives (coef != 0) in slot order, then one
+// representative per distinct exponent among its padded (coef == 0) slots; pcls maps
+// every slot to its list entry.  Padded slots with the same exponent have identical
+// cotangents, so the pullback computes each once.
+// ---------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void ovl_plan_kernel(const OvlShape sh, int mode, const T *__restrict__ coefT,
+                                                       const T *__restrict__ zetaT, const T *__restrict__ DT,
+                                                       double *__restrict__ coef64, double *__restrict__ alpha64,
+                                                       int *__restrict__ nreal, int *__restrict__ nlist,
+                                                       int *__restrict__ lidx, int *__restrict__ pcls, int nblk_h,
+                                                       const int *__restrict__ blk_hist, int *__restrict__ blk_off,
+                                                       int *__restrict__ bcnt, int *__restrict__ boff,
+                                                       int *__restrict__ toff)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, NK = NZ * NZ;
+    for (int idx = threadIdx.x; idx < NZ * ns; idx += blockDim.x) {
+        const int sg = idx / ns, s = idx - sg * ns;
+        double zv[OVL_MAXK], dv[OVL_MAXK], cv[OVL_MAXK];
+        for (int k = 0; k < K; ++k) {
+            const int64_t q = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+            zv[k] = static_cast<double>(zetaT[q]);
+            if (mode == 0) {
+                cv[k] = static_cast<double>(coefT[q]);
+            } else {
+                dv[k] = static_cast<double>(DT[q]);
+            }
+        }
+        if (mode == 1) {
+            const double nsq = cart_normsq(dv, zv, K, sh.ls[s]);
+            const double inv = nsq > 0.0 ? 1.0 / sqrt(nsq) : 0.0;
+            for (int k = 0; k < K; ++k) cv[k] = nsq > 0.0 ? dv[k] * inv : 0.0;
+        }
+        int nr = 0;
+        int *li = lidx + static_cast<int64_t>(idx) * K;
+        int *pc = pcls + static_cast<int64_t>(idx) * K;
+        for (int k = 0; k < K; ++k) {
+            const int64_t q = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+            coef64[q] = cv[k];
+            alpha64[q] = zv[k];
+            // the forward's skip test is made in the kernel precision (overlap_2c.jl:202)
+            if (static_cast<T>(cv[k]) != T(0)) {
+                pc[k] = nr;
+                li[nr++] = k;
+            }
+        }
+        int nl = nr;
+        for (int k = 0; k < K; ++k) {
+            if (static_cast<T>(cv[k]) != T(0)) continue;
+            int rep = -1;
+            for (int e = nr; e < nl; ++e)
+                if (zv[li[e]] == zv[k]) {
+                    rep = e;
+                    break;
+                }
+            if (rep < 0) {
+                rep = nl;
+                li[nl++] = k;
+            }
+            pc[k] = rep;
+        }
+        nreal[idx] = nr;
+        nlist[idx] = nl;
+    }
+    for (int key = threadIdx.x; key < NK; key += blockDim.x) {
+        int run = 0;
+        for (int b = 0; b < nblk_h; ++b) {
+            blk_off[static_cast<int64_t>(b) * NK + key] = run;
+            run += blk_hist[static_cast<int64_t>(b) * NK + key];
+        }
+        bcnt[key] = run;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int o = 0, t = 0;
+        for (int key = 0; key < NK; ++key) {
+            boff[key] = o;
+            toff[key] = t;
+            o += bcnt[key];
+            t += (bcnt[key] + OVL_TILE - 1) / OVL_TILE;
+        }
+        toff[NK] = t;
+    }
+    __syncthreads();
+    for (int key = threadIdx.x; key < NK; key += blockDim.x)
+        for (int b = 0; b < nblk_h; ++b) blk_off[static_cast<int64_t>(b) * NK + key] += boff[key];
+}
+
+// ---------------------------------------------------------------------------------
+// Prep 3 (grid, three roles): stable scatter of b into bucket order, tile -> bucket
+// keys, and the FP64 primitive-pair records of every (species pair, shell pair).
+// ---------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void make_record(double a, double b, double ca, double cb, typename RecOf<T>::type &r)
+{
+    const double p = a + b;
+    const double ip = 1.0 / p;
+    const double mu = a * b * ip;
+    const double pref = 5.568327996831707845 * ip * sqrt(ip);  // (pi/p)^{3/2}
+    if constexpr (std::is_same<T, float>::value) {
+        r.mu2 = mu * 1.4426950408889634074;
+    } else {
+        r.mu2 = mu;
+    }
+    r.aop = static_cast<T>(a * ip);
+    r.bop = static_cast<T>(b * ip);
+    r.i2p = static_cast<T>(0.5 * ip);
+    r.Kab = static_cast<T>(ca * cb * pref);
+    r.w1 = static_cast<T>(cb * pref);
+    r.w2 = static_cast<T>(ca * pref);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void ovl_fill_kernel(const OvlShape sh, const void *__restrict__ sidxA,
+                                                       const void *__restrict__ sidxB, int itype, int64_t B,
+                                                       int nblk_perm, int nblk_tile, const int *__restrict__ rank,
+                                                       const int *__restrict__ blk_off, const int *__restrict__ toff,
+                                                       int *__restrict__ perm, int *__restrict__ tile_key,
+                                                       int maxtiles, const double *__restrict__ coef64,
+                                                       const double *__restrict__ alpha64,
+                                                       const int *__restrict__ nlist, const int *__restrict__ lidx,
+                                                       typename RecOf<T>::type *__restrict__ rec)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, NK = NZ * NZ;
+    const int bid = blockIdx.x;
+    if (bid < nblk_perm) {
+        const int64_t i = static_cast<int64_t>(bid) * 256 + threadIdx.x;
+        if (i >= B) return;
+        int sa = load_sigma(sidxA, itype, i), sb = load_sigma(sidxB, itype, i);
+        sa = (sa < 0 || sa >= NZ) ? 0 : sa;
+        sb = (sb < 0 || sb >= NZ) ? 0 : sb;
+        const int key = sa * NZ + sb;
+        perm[blk_off[static_cast<int64_t>(i / 256) * NK + key] + rank[i]] = static_cast<int>(i);
+        return;
+    }
+    if (bid < nblk_perm + nblk_tile) {
+        const int t = (bid - nblk_perm) * 256 + threadIdx.x;
+        if (t >= maxtiles) return;
+        int key = -1;
+        if (t < toff[NK]) {
+            key = 0;
+            while (key + 1 < NK && toff[key + 1] <= t) ++key;
+        }
+        tile_key[t] = key;
+        return;
+    }
+    const int64_t gid = static_cast<int64_t>(bid - nblk_perm - nblk_tile) * 256 + threadIdx.x;
+    const int64_t total = static_cast<int64_t>(NK) * ns * ns * K * K;
+    if (gid >= total) return;
+    const int j = static_cast<int>(gid % K);
+    const int i = static_cast<int>((gid / K) % K);
+    const int64_t g = gid / (static_cast<int64_t>(K) * K);
+    const int sa = static_cast<int>(g % ns);
+    const int sb = static_cast<int>((g / ns) % ns);
+    const int key = static_cast<int>(g / (static_cast<int64_t>(ns) * ns));
+    const int sgA = key / NZ, sgB = key - sgA * NZ;
+    const int la_idx = sgA * ns + sa, lb_idx = sgB * ns + sb;
+    if (i >= nlist[la_idx] || j >= nlist[lb_idx]) return;
+    const int ka = lidx[static_cast<int64_t>(la_idx) * K + i];
+    const int kb = lidx[static_cast<int64_t>(lb_idx) * K + j];
+    const int64_t qa = sa + static_cast<int64_t>(ns) * (ka + static_cast<int64_t>(K) * sgA);
+    const int64_t qb = sb + static_cast<int64_t>(ns) * (kb + static_cast<int64_t>(K) * sgB);
+    typename RecOf<T>::type r;
+    make_record<T>(alpha64[qa], alpha64[qb], coef64[qa], coef64[qb], r);
+    rec[gid] = r;
+}
+
+// ---------------------------------------------------------------------------------
+// Per-lane primitive-pair math
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ float pair_exp(const PairRecF &r, double R2)
+{
+    const double y = r.mu2 * R2;  // mu |AB|^2 log2(e), in FP64
+    const double n = rint(y);
+    const float f = static_cast<float>(y - n);
+    return ldexpf(__builtin_amdgcn_exp2f(-f), -static_cast<int>(n));
+}
+__device__ __forceinline__ double pair_exp(const PairRecD &r, double R2) { return exp(-r.mu2 * R2); }
+
+// 1-D Obara-Saika overlaps s[i][j], i <= IA, j <= IB, normalised to s[0][0] = 1.
+template <typename T, int IA, int IB>
+__device__ __forceinline__ void os1d(T PA, T PB, T i2p, T (&s)[IA + 1][IB + 1])
+{
+    s[0][0] = T(1);
+#pragma unroll
+    for (int i = 1; i <= IA; ++i) s[i][0] = PA * s[i - 1][0] + (i > 1 ? T(i - 1) * i2p * s[i > 1 ? i - 2 : 0][0] : T(0));
+#pragma unroll
+    for (int j = 1; j <= IB; ++j)
+#pragma unroll
+        for (int i = 0; i <= IA; ++i) {
+            T v = PB * s[i][j - 1];
+            if (i > 0) v += T(i) * i2p * s[i > 0 ? i - 1 : 0][j - 1];
+            if (j > 1) v += T(j - 1) * i2p * s[i][j > 1 ? j - 2 : 0];
+            s[i][j] = v;
+        }
+}
+
+template <int L>
+struct ShellC {
+    static constexpr int nbf = (L + 1) * (L + 2) / 2;
+};
+
+// Forward of one shell pair for the lane's b: accumulate over the pair list and write
+// the aliased nbf_a x nbf_b block into the LDS stripe (column stride nrows).
+template <typename T, int LA, int LB>
+__device__ __forceinline__ void pair_fwd(const typename RecOf<T>::type *__restrict__ recs, int nra, int nrb, int K,
+                                         double R2, const T (&ab)[3], T *dst, int nrows)
+{
+    constexpr int NBA = ShellC<LA>::nbf, NBB = ShellC<LB>::nbf, N1 = 2 * LA + 1;
+    constexpr int NLIN = (NBA - 1) + N1 * (NBB - 1) + 1;
+    T acc[NLIN];
+#pragma unroll
+    for (int q = 0; q < NLIN; ++q) acc[q] = T(0);
+    for (int i = 0; i < nra; ++i) {
+        const typename RecOf<T>::type *row = recs + i * K;
+        for (int j = 0; j < nrb; ++j) {
+            const typename RecOf<T>::type r = row[j];
+            const T G = r.Kab * pair_exp(r, R2);
+            if constexpr (LA + LB == 0) {
+                acc[0] += G;
+            } else {
+                T sx[LA + 1][LB + 1], sy[LA + 1][LB + 1], sz[LA + 1][LB + 1];
+                os1d<T, LA, LB>(-r.bop * ab[0], r.aop * ab[0], r.i2p, sx);
+                os1d<T, LA, LB>(-r.bop * ab[1], r.aop * ab[1], r.i2p, sy);
+                os1d<T, LA, LB>(-r.bop * ab[2], r.aop * ab[2], r.i2p, sz);
+                int index1 = 0;
+#pragma unroll
+                for (int ll1 = LA; ll1 >= 0; --ll1)
+#pragma unroll
+                    for (int n1 = 0; n1 <= LA - ll1; ++n1) {
+                        const int m1 = LA - ll1 - n1;
+                        int index2 = 0;
+#pragma unroll
+                        for (int ll2 = LB; ll2 >= 0; --ll2)
+#pragma unroll
+                            for (int n2 = 0; n2 <= LB - ll2; ++n2) {
+                                const int m2 = LB - ll2 - n2;
+                                acc[index1 + N1 * index2] += G * sx[ll1][ll2] * sy[m1][m2] * sz[n1][n2];
+                                ++index2;
+                            }
+                        ++index1;
+                    }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NBB; ++j)
+#pragma unroll
+        for (int i = 0; i < NBA; ++i) {
+            const int q = i + NBA * j;
+            dst[i + nrows * j] = q < NLIN ? acc[q < NLIN ? q : 0] : T(0);
+        }
+}
+
+// Contraction of one primitive pair against the aliased cotangent block g (LDS stripe
+// slice of the lane, column stride nrows).  H = sum g s_x s_y s_z; with RAISED also the
+// bra- and ket-raised sums Ha, Hb (d/dalpha x^l e^{-alpha x^2} = -x^{l+2} e^{-alpha x^2}).
+template <typename T, int LA, int LB, bool RAISED>
+__device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, const T (&ab)[3], const T *g,
+                                              int nrows, T &H, T &Ha, T &Hb)
+{
+    constexpr int NBA = ShellC<LA>::nbf, N1 = 2 * LA + 1;
+    constexpr int RA = RAISED ? LA + 2 : LA, RB = RAISED ? LB + 2 : LB;
+    T sx[RA + 1][RB + 1], sy[RA + 1][RB + 1], sz[RA + 1][RB + 1];
+    os1d<T, RA, RB>(-r.bop * ab[0], r.aop * ab[0], r.i2p, sx);
+    os1d<T, RA, RB>(-r.bop * ab[1], r.aop * ab[1], r.i2p, sy);
+    os1d<T, RA, RB>(-r.bop * ab[2], r.aop * ab[2], r.i2p, sz);
+    H = T(0);
+    Ha = T(0);
+    Hb = T(0);
+    int index1 = 0;
+#pragma unroll
+    for (int ll1 = LA; ll1 >= 0; --ll1)
+#pragma unroll
+        for (int n1 = 0; n1 <= LA - ll1; ++n1) {
+            const int m1 = LA - ll1 - n1;
+            int index2 = 0;
+#pragma unroll
+            for (int ll2 = LB; ll2 >= 0; --ll2)
+#pragma unroll
+                for (int n2 = 0; n2 <= LB - ll2; ++n2) {
+                    const int m2 = LB - ll2 - n2;
+                    const int lin = index1 + N1 * index2;
+                    const int i = lin % NBA, j = lin / NBA;
+                    const T gv = g[i + nrows * j];
+                    const T x0 = sx[ll1][ll2], y0 = sy[m1][m2], z0 = sz[n1][n2];
+                    H += gv * (x0 * y0 * z0);
+                    if constexpr (RAISED) {
+                        Ha += gv * (sx[ll1 + 2][ll2] * y0 * z0 + x0 * sy[m1 + 2][m2] * z0 + x0 * y0 * sz[n1 + 2][n2]);
+                        Hb += gv * (sx[ll1][ll2 + 2] * y0 * z0 + x0 * sy[m1][m2 + 2] * z0 + x0 * y0 * sz[n1][n2 + 2]);
+                    }
+                    ++index2;
+                }
+            ++index1;
+        }
+}
+
+// Pullback of one shell pair: per-pair contributions reduced over the wave (FP64) and
+// added to the wave's FP64 LDS accumulators of the bra side (accA, list index i) and
+// the ket side (accB, list index j); [2*li + 0] coefficient, [2*li + 1] exponent.
+template <typename T, int LA, int LB>
+__device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restrict__ recs, int nra, int nla, int nrb,
+                                         int nlb, int K, double R2, const T (&ab)[3], const T *g, int nrows,
+                                         double *accA, double *accB, int lane)
+{
+    using R = typename RecOf<T>::type;
+    // real x real: all four cotangents
+    for (int i = 0; i < nra; ++i) {
+        T cA = T(0), aA = T(0);
+        for (int j = 0; j < nrb; ++j) {
+            const R r = recs[i * K + j];
+            const T e = pair_exp(r, R2);
+            T H, Ha, Hb;
+            pair_contract<T, LA, LB, true>(r, ab, g, nrows, H, Ha, Hb);
+            const T eH = e * H;
+            cA += r.w1 * eH;
+            aA -= r.Kab * e * Ha;
+            const double cB = wave_sum(static_cast<double>(r.w2 * eH));
+            const double aB = wave_sum(static_cast<double>(-r.Kab * e * Hb));
+            if (lane == 0) {
+                accB[2 * j] += cB;
+                accB[2 * j + 1] += aB;
+            }
+        }
+        const double sc = wave_sum(static_cast<double>(cA));
+        const double sa = wave_sum(static_cast<double>(aA));
+        if (lane == 0) {
+            accA[2 * i] += sc;
+            accA[2 * i + 1] += sa;
+        }
+    }
+    // padded bra representative x real ket: coefficient cotangent of the bra slot only
+    for (int i = nra; i < nla; ++i) {
+        T cA = T(0);
+        for (int j = 0; j < nrb; ++j) {
+            const R r = recs[i * K + j];
+            T H, Ha, Hb;
+            pair_contract<T, LA, LB, false>(r, ab, g, nrows, H, Ha, Hb);
+            cA += r.w1 * pair_exp(r, R2) * H;
+        }
+        const double sc = wave_sum(static_cast<double>(cA));
+        if (lane == 0) accA[2 * i] += sc;
+    }
+    // real bra x padded ket representative: coefficient cotangent of the ket slot only
+    for (int j = nrb; j < nlb; ++j) {
+        T cB = T(0);
+        for (int i = 0; i < nra; ++i) {
+            const R r = recs[i * K + j];
+            T H, Ha, Hb;
+            pair_contract<T, LA, LB, false>(r, ab, g, nrows, H, Ha, Hb);
+            cB += r.w2 * pair_exp(r, R2) * H;
+        }
+        const double sc = wave_sum(static_cast<double>(cB));
+        if (lane == 0) accB[2 * j] += sc;
+    }
+}
+
+// (la, lb) dispatch over the compile-time cases allowed by LMAXB
+#define AOK_LL_CASES(X)                                                                                       \
+    X(0, 0) X(0, 1) X(0, 2) X(0, 3) X(0, 4) X(1, 0) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(2, 0) X(2, 1) X(2, 2) \
+        X(2, 3) X(2, 4) X(3, 0) X(3, 1) X(3, 2) X(3, 3) X(3, 4) X(4, 0) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
+
+// ---------------------------------------------------------------------------------
+// Tile bookkeeping shared by the fast kernels
+// ---------------------------------------------------------------------------------
+struct TileInfo {
+    int key, first, nact;
+};
+
+__device__ __forceinline__ bool tile_info(int t, int NK, const int *__restrict__ tile_key,
+                                          const int *__restrict__ boff, const int *__restrict__ bcnt,
+                                          const int *__restrict__ toff, TileInfo &ti)
+{
+    if (t >= toff[NK]) return false;
+    ti.key = tile_key[t];
+    ti.first = boff[ti.key] + (t - toff[ti.key]) * OVL_TILE;
+    ti.nact = min(OVL_TILE, boff[ti.key] + bcnt[ti.key] - ti.first);
+    return true;
+}
+
+// ---------------------------------------------------------------------------------
+// Fast forward
+// ---------------------------------------------------------------------------------
+template <typename T, int LMAXB>
+__global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T *__restrict__ posA,
+                                                     const T *__restrict__ posB, T scale,
+                                                     const int *__restrict__ perm, const int *__restrict__ tile_key,
+                                                     const int *__restrict__ boff, const int *__restrict__ bcnt,
+                                                     const int *__restrict__ toff,
+                                                     const typename RecOf<T>::type *__restrict__ rec,
+                                                     const int *__restrict__ nreal, T *__restrict__ out)
+{
+    extern __shared__ unsigned char smem_raw[];
+    int *bl_b = reinterpret_cast<int *>(smem_raw);
+    T *stripe = reinterpret_cast<T *>(smem_raw + 256);
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    TileInfo ti;
+    if (!tile_info(blockIdx.x, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
+    const int lane = threadIdx.x;
+    const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
+    const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
+    bl_b[lane] = b;
+    const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
+    const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
+    const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);
+    const double dy = static_cast<double>(Ay) - static_cast<double>(By);
+    const double dz = static_cast<double>(Az) - static_cast<double>(Bz);
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    const T ab[3] = {static_cast<T>(dx), static_cast<T>(dy), static_cast<T>(dz)};
+    T *mine = stripe + lane * sh.str;
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    for (int c = 0; c < sh.nchunk; ++c) {
+        const OvlChunk ch = sh.ch[c];
+        const int sb = ch.sb, lb = sh.ls[sb];
+        const int nrb = nreal[sgB * ns + sb];
+        for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
+            const int la = sh.ls[sa];
+            const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
+            const typename RecOf<T>::type *recs = rec + g * K * K;
+            const int nra = nreal[sgA * ns + sa];
+            T *dst = mine + (sh.off[sa] - ch.row0);
+#define AOK_FWD_CASE(A, B_)                                                          \
+    if constexpr (A <= LMAXB && B_ <= LMAXB)                                         \
+        if (la == A && lb == B_) {                                                   \
+            pair_fwd<T, A, B_>(recs, nra, nrb, K, R2, ab, dst, ch.nrows);            \
+            continue;                                                                \
+        }
+            AOK_LL_CASES(AOK_FWD_CASE)
+#undef AOK_FWD_CASE
+        }
+        __syncthreads();
+        const int nbb = nbf_of(lb), nrows = ch.nrows;
+        const int len = nrows * nbb, total = ti.nact * len;
+        const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
+        const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
+        for (int e = lane; e < total; e += OVL_TILE) {
+            const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
+            const int q = e - bl * len;
+            const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
+            const int i = q - j * nrows;
+            out[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i] =
+                stripe[bl * sh.str + q];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Fast pullback
+// ---------------------------------------------------------------------------------
+template <typename T, int LMAXB>
+__global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
+                                                     const T *__restrict__ posA, const T *__restrict__ posB, T scale,
+                                                     const int *__restrict__ perm, const int *__restrict__ tile_key,
+                                                     const int *__restrict__ boff, const int *__restrict__ bcnt,
+                                                     const int *__restrict__ toff,
+                                                     const typename RecOf<T>::type *__restrict__ rec,
+                                                     const int *__restrict__ nreal, const int *__restrict__ nlist,
+                                                     double *__restrict__ partial)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    int *bl_b = reinterpret_cast<int *>(smem_raw);
+    double *accA = reinterpret_cast<double *>(smem_raw + 256);
+    double *accB = accA + 2 * ns * K;
+    T *stripe = reinterpret_cast<T *>(accB + 2 * ns * K);
+    TileInfo ti;
+    if (!tile_info(blockIdx.x, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
+    const int lane = threadIdx.x;
+    for (int q = lane; q < 4 * ns * K; q += OVL_TILE) accA[q] = 0.0;
+    const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
+    const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
+    bl_b[lane] = b;
+    const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
+    const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
+    const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);
+    const double dy = static_cast<double>(Ay) - static_cast<double>(By);
+    const double dz = static_cast<double>(Az) - static_cast<double>(Bz);
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    const T ab[3] = {static_cast<T>(dx), static_cast<T>(dy), static_cast<T>(dz)};
+    T *mine = stripe + lane * sh.str;
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    __syncthreads();
+    for (int c = 0; c < sh.nchunk; ++c) {
+        const OvlChunk ch = sh.ch[c];
+        const int sb = ch.sb, lb = sh.ls[sb];
+        const int nbb = nbf_of(lb), nrows = ch.nrows;
+        const int len = nrows * nbb, total = OVL_TILE * len;
+        const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
+        const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
+        // stage the cotangent stripe (zeros for inactive lanes)
+        for (int e = lane; e < total; e += OVL_TILE) {
+            const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
+            const int q = e - bl * len;
+            const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
+            const int i = q - j * nrows;
+            stripe[bl * sh.str + q] =
+                bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i]
+                             : T(0);
+        }
+        __syncthreads();
+        const int nrb = nreal[sgB * ns + sb], nlb = nlist[sgB * ns + sb];
+        double *aB = accB + 2 * sb * K;
+        for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
+            const int la = sh.ls[sa];
+            const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
+            const typename RecOf<T>::type *recs = rec + g * K * K;
+            const int nra = nreal[sgA * ns + sa], nla = nlist[sgA * ns + sa];
+            const T *gsl = mine + (sh.off[sa] - ch.row0);
+            double *aA = accA + 2 * sa * K;
+#define AOK_BWD_CASE(A, B_)                                                                      \
+    if constexpr (A <= LMAXB && B_ <= LMAXB)                                                     \
+        if (la == A && lb == B_) {                                                               \
+            pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gsl, nrows, aA, aB, lane);   \
+            continue;                                                                            \
+        }
+            AOK_LL_CASES(AOK_BWD_CASE)
+#undef AOK_BWD_CASE
+        }
+        __syncthreads();
+    }
+    double *pt = partial + static_cast<int64_t>(blockIdx.x) * 4 * ns * K;
+    for (int q = lane; q < 4 * ns * K; q += OVL_TILE) pt[q] = accA[q];
+}
+
+// Fixed-order sum of the per-tile partials into every (species, shell, slot); padded
+// slots read their representative's entry.  Output in the Julia [ns x K x NZ] layout.
+__global__ void ovl2c_bwd_finalize(const OvlShape sh, const int *__restrict__ tile_key,
+                                   const int *__restrict__ toff, const int *__restrict__ pcls,
+                                   const double *__restrict__ partial, double *__restrict__ dc64,
+                                   double *__restrict__ da64)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= NZ * ns * K) return;
+    const int sg = q / (ns * K), rem = q - sg * ns * K, s = rem / K, k = rem - s * K;
+    const int li = pcls[(sg * ns + s) * K + k];
+    const int ntiles = toff[NZ * NZ];
+    double c = 0.0, a = 0.0;
+    for (int t = 0; t < ntiles; ++t) {
+        const int key = tile_key[t];
+        const double *pt = partial + static_cast<int64_t>(t) * 4 * ns * K;
+        if (key / NZ == sg) {
+            c += pt[2 * (s * K + li)];
+            a += pt[2 * (s * K + li) + 1];
+        }
+        if (key % NZ == sg) {
+            c += pt[2 * ns * K + 2 * (s * K + li)];
+            a += pt[2 * ns * K + 2 * (s * K + li) + 1];
+        }
+    }
+    const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+    dc64[dst] = c;
+    da64[dst] = a;
+}
+
+// ---------------------------------------------------------------------------------
+// Output stages: cast, or normalisation VJP (overlap_2c_grad.jl:251-304) in FP64.
+// ---------------------------------------------------------------------------------
+template <typename T>
+__global__ void ovl2c_cast_out(int n, const double *__restrict__ a, const double *__restrict__ b, T *__restrict__ oa,
+                               T *__restrict__ ob)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (oa) oa[q] = static_cast<T>(a[q]);
+    if (ob) ob[q] = static_cast<T>(b[q]);
+}
+
+// dD, dzeta from (dcoef, dalpha): one thread per (species, shell).
+template <typename T>
+__global__ void ovl2c_vjp_out(const OvlShape sh, const double *__restrict__ dc64, const double *__restrict__ da64,
+                              const T *__restrict__ zetaT, const T *__restrict__ DT, T *__restrict__ dzeta,
+                              T *__restrict__ dD)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= NZ * ns) return;
+    const int sg = idx / ns, s = idx - sg * ns, l = sh.ls[s];
+    double zv[OVL_MAXK], dv[OVL_MAXK];
+    int64_t q[OVL_MAXK];
+    for (int k = 0; k < K; ++k) {
+        q[k] = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+        zv[k] = static_cast<double>(zetaT[q[k]]);
+        dv[k] = static_cast<double>(DT[q[k]]);
+    }
+    const double nsq = cart_normsq(dv, zv, K, l);
+    if (!(nsq > 0.0)) {
+        for (int k = 0; k < K; ++k) {
+            if (dD) dD[q[k]] = T(0);
+            if (dzeta) dzeta[q[k]] = static_cast<T>(da64[q[k]]);
+        }
+        return;
+    }
+    const double n = sqrt(nsq), c = cart_cl(l), e = l + 1.5;
+    double dot = 0.0;
+    for (int k = 0; k < K; ++k) dot += dc64[q[k]] * dv[k];
+    const double s2n3 = dot / (2.0 * n * n * n);
+    for (int m = 0; m < K; ++m) {
+        double gm = 0.0, hm = 0.0;
+        for (int j = 0; j < K; ++j) {
+            const double den = zv[m] + zv[j];
+            const double pe = pow(den, -e);
+            gm += dv[j] * pe;
+            hm += dv[j] * pe / den;
+        }
+        gm *= 2.0 * c;
+        hm *= -2.0 * c * e * dv[m];
+        if (dD) dD[q[m]] = static_cast<T>(dc64[q[m]] / n - gm * s2n3);
+        if (dzeta) dzeta[q[m]] = static_cast<T>(da64[q[m]] - hm * s2n3);
+    }
+}
+
+// coef = D / sqrt(normsq) per (species, shell) in FP64 (cartesian_gto_basis.jl:110-120)
+template <typename T>
+__global__ void compile_coef_kernel(const OvlShape sh, const T *__restrict__ zetaT, const T *__restrict__ DT,
+                                    T *__restrict__ coef)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= NZ * ns) return;
+    const int sg = idx / ns, s = idx - sg * ns;
+    double zv[OVL_MAXK], dv[OVL_MAXK];
+    int64_t q[OVL_MAXK];
+    for (int k = 0; k < K; ++k) {
+        q[k] = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+        zv[k] = static_cast<double>(zetaT[q[k]]);
+        dv[k] = static_cast<double>(DT[q[k]]);
+    }
+    const double nsq = cart_normsq(dv, zv, K, sh.ls[s]);
+    for (int k = 0; k < K; ++k) coef[q[k]] = nsq > 0.0 ? static_cast<T>(dv[k] / sqrt(nsq)) : T(0);
+}
+
+// ---------------------------------------------------------------------------------
+// Plain path: the reference decomposition (one item per (b, s_a, s_b)).
+// ---------------------------------------------------------------------------------
+template <typename T, int LMAXP, bool RAISE>
+__device__ void plain_etab(T (&E)[2 * (LMAXP + (RAISE ? 2 : 0)) + 2][LMAXP + (RAISE ? 2 : 0) + 1]
+                                  [LMAXP + (RAISE ? 2 : 0) + 1][3],
+                           int la1, int lb1, T PA, T PB, T oo2p, T E00, int ax)
+{
+    // MD recursion of overlap_2c.jl:222-253 (0-based: t, i, j)
+    constexpr int LR = LMAXP + (RAISE ? 2 : 0);
+    constexpr int Tmax = 2 * LR + 2;
+    for (int jb = 0; jb < lb1; ++jb)
+        for (int ia = 0; ia < la1; ++ia) {
+            const int ab = ia + jb + 1;  // first index beyond t = ia + jb
+            if (ab < Tmax) E[ab][ia][jb][ax] = T(0);
+            if (ab + 1 < Tmax) E[ab + 1][ia][jb][ax] = T(0);
+        }
+    E[0][0][0][ax] = E00;
+    for (int i = 1; i < la1; ++i) {
+        E[0][i][0][ax] = PA * E[0][i - 1][0][ax] + E[1][i - 1][0][ax];
+        for (int t = 1; t <= i; ++t)
+            E[t][i][0][ax] = PA * E[t][i - 1][0][ax] + T(t + 1) * E[t + 1][i - 1][0][ax] + oo2p * E[t - 1][i - 1][0][ax];
+    }
+    for (int j = 1; j < lb1; ++j) {
+        E[0][0][j][ax] = PB * E[0][0][j - 1][ax] + E[1][0][j - 1][ax];
+        for (int t = 1; t <= j; ++t)
+            E[t][0][j][ax] = PB * E[t][0][j - 1][ax] + T(t + 1) * E[t + 1][0][j - 1][ax] + oo2p * E[t - 1][0][j - 1][ax];
+        for (int i = 1; i < la1; ++i) {
+            E[0][i][j][ax] = PA * E[0][i - 1][j][ax] + E[1][i - 1][j][ax];
+            for (int t = 1; t <= i + j; ++t)
+                E[t][i][j][ax] =
+                    PA * E[t][i - 1][j][ax] + T(t + 1) * E[t + 1][i - 1][j][ax] + oo2p * E[t - 1][i - 1][j][ax];
+        }
+    }
+}
+
+template <typename T, int LMAXP>
+__global__ __launch_bounds__(256) void ovl2c_fwd_plain(const OvlShape sh, int64_t B, const T *__restrict__ coef,
+                                                       const T *__restrict__ alpha, const void *__restrict__ sidxA,
+                                                       const void *__restrict__ sidxB, int itype,
+                                                       const T *__restrict__ posA, const T *__restrict__ posB,
+                                                       T scale, T *__restrict__ out, unsigned *__restrict__ err)
+{
+    constexpr int NBF = (LMAXP + 1) * (LMAXP + 2) / 2;
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    const int64_t item = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= B * ns * ns) return;
+    const int64_t b = item % B;
+    const int sa = static_cast<int>((item / B) % ns), sb = static_cast<int>(item / (B * ns));
+    int sgA = load_sigma(sidxA, itype, b), sgB = load_sigma(sidxB, itype, b);
+    if (sgA < 0 || sgA >= NZ || sgB < 0 || sgB >= NZ) {
+        atomicOr(err, ERR_SPECIES);
+        return;
+    }
+    const int la = sh.ls[sa], lb = sh.ls[sb];
+    const int nba = nbf_of(la), nbb = nbf_of(lb), N1 = 2 * la + 1;
+    T E[2 * LMAXP + 2][LMAXP + 1][LMAXP + 1][3];
+    T blk[NBF * NBF];
+    for (int k = 0; k < nba * nbb; ++k) blk[k] = T(0);
+    const T A[3] = {scale * posA[3 * b], scale * posA[3 * b + 1], scale * posA[3 * b + 2]};
+    const T Bv[3] = {scale * posB[3 * b], scale * posB[3 * b + 1], scale * posB[3 * b + 2]};
+    const T pi32 = T(5.568327996831707845);
+    for (int ip = 0; ip < K; ++ip)
+        for (int jp = 0; jp < K; ++jp) {
+            const int64_t qa = sa + static_cast<int64_t>(ns) * (ip + static_cast<int64_t>(K) * sgA);
+            const int64_t qb = sb + static_cast<int64_t>(ns) * (jp + static_cast<int64_t>(K) * sgB);
+            const T ca = coef[qa], cb = coef[qb];
+            if (ca == T(0) || cb == T(0)) continue;
+            const T aa = alpha[qa], ab = alpha[qb];
+            const T p = aa + ab, mu = aa * ab / p, oo2p = T(0.5) / p;
+            const T prefac = ca * cb * pi32 / (p * sqrt(p));
+            for (int ax = 0; ax < 3; ++ax) {
+                const T P = (aa * A[ax] + ab * Bv[ax]) / p;
+                const T AB = A[ax] - Bv[ax];
+                plain_etab<T, LMAXP, false>(E, la + 1, lb + 1, P - A[ax], P - Bv[ax], oo2p, exp(-mu * AB * AB), ax);
+            }
+            int index1 = 0;
+            for (int ll1 = la; ll1 >= 0; --ll1)
+                for (int n1 = 0; n1 <= la - ll1; ++n1) {
+                    const int m1 = la - ll1 - n1;
+                    int index2 = 0;
+                    for (int ll2 = lb; ll2 >= 0; --ll2)
+                        for (int n2 = 0; n2 <= lb - ll2; ++n2) {
+                            const int m2 = lb - ll2 - n2;
+                            blk[index1 + N1 * index2] += E[0][ll1][ll2][0] * E[0][m1][m2][1] * E[0][n1][n2][2] * prefac;
+                            ++index2;
+                        }
+                    ++index1;
+                }
+        }
+    const int64_t base = static_cast<int64_t>(N) * N * b;
+    for (int j = 0; j < nbb; ++j)
+        for (int i = 0; i < nba; ++i)
+            out[base + (sh.off[sa] + i) + static_cast<int64_t>(N) * (sh.off[sb] + j)] = blk[i + nba * j];
+}
+
+template <typename T, int LMAXP>
+__global__ __launch_bounds__(256) void ovl2c_bwd_plain(const OvlShape sh, int64_t B, const T *__restrict__ dS,
+                                                       const T *__restrict__ coef, const T *__restrict__ alpha,
+                                                       const void *__restrict__ sidxA, const void *__restrict__ sidxB,
+                                                       int itype, const T *__restrict__ posA,
+                                                       const T *__restrict__ posB, T scale, T *__restrict__ dcoef,
+                                                       T *__restrict__ dalpha, unsigned *__restrict__ err)
+{
+    constexpr int LR = LMAXP + 2;
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    const int64_t item = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (item >= B * ns * ns) return;
+    const int64_t b = item % B;
+    const int sa = static_cast<int>((item / B) % ns), sb = static_cast<int>(item / (B * ns));
+    int sgA = load_sigma(sidxA, itype, b), sgB = load_sigma(sidxB, itype, b);
+    if (sgA < 0 || sgA >= NZ || sgB < 0 || sgB >= NZ) {
+        atomicOr(err, ERR_SPECIES);
+        return;
+    }
+    const int la = sh.ls[sa], lb = sh.ls[sb];
+    const int nba = nbf_of(la), N1 = 2 * la + 1;
+    T E[2 * LR + 2][LR + 1][LR + 1][3];
+    const T A[3] = {scale * posA[3 * b], scale * posA[3 * b + 1], scale * posA[3 * b + 2]};
+    const T Bv[3] = {scale * posB[3 * b], scale * posB[3 * b + 1], scale * posB[3 * b + 2]};
+    const T pi32 = T(5.568327996831707845);
+    const int64_t base = static_cast<int64_t>(N) * N * b;
+    for (int ip = 0; ip < K; ++ip)
+        for (int jp = 0; jp < K; ++jp) {
+            const int64_t qa = sa + static_cast<int64_t>(ns) * (ip + static_cast<int64_t>(K) * sgA);
+            const int64_t qb = sb + static_cast<int64_t>(ns) * (jp + static_cast<int64_t>(K) * sgB);
+            const T ca = coef[qa], cb = coef[qb];
+            const T aa = alpha[qa], ab = alpha[qb];
+            const T p = aa + ab, mu = aa * ab / p, oo2p = T(0.5) / p;
+            const T pref0 = pi32 / (p * sqrt(p));
+            for (int ax = 0; ax < 3; ++ax) {
+                const T P = (aa * A[ax] + ab * Bv[ax]) / p;
+                const T AB = A[ax] - Bv[ax];
+                plain_etab<T, LMAXP, true>(E, la + 3, lb + 3, P - A[ax], P - Bv[ax], oo2p, exp(-mu * AB * AB), ax);
+            }
+            T Bsum = T(0), Aa = T(0), Ab = T(0);
+            int index1 = 0;
+            for (int ll1 = la; ll1 >= 0; --ll1)
+                for (int n1 = 0; n1 <= la - ll1; ++n1) {
+                    const int m1 = la - ll1 - n1;
+                    int index2 = 0;
+                    for (int ll2 = lb; ll2 >= 0; --ll2)
+                        for (int n2 = 0; n2 <= lb - ll2; ++n2) {
+                            const int m2 = lb - ll2 - n2;
+                            const int lin = index1 + N1 * index2;
+                            const int i = lin % nba, j = lin / nba;
+                            const T g = dS[base + (sh.off[sa] + i) + static_cast<int64_t>(N) * (sh.off[sb] + j)];
+                            const T Ex = E[0][ll1][ll2][0], Ey = E[0][m1][m2][1], Ez = E[0][n1][n2][2];
+                            const T dGa = -(E[0][ll1 + 2][ll2][0] * Ey * Ez + Ex * E[0][m1 + 2][m2][1] * Ez +
+                                            Ex * Ey * E[0][n1 + 2][n2][2]);
+                            const T dGb = -(E[0][ll1][ll2 + 2][0] * Ey * Ez + Ex * E[0][m1][m2 + 2][1] * Ez +
+                                            Ex * Ey * E[0][n1][n2 + 2][2]);
+                            Bsum += g * (Ex * Ey * Ez);
+                            Aa += g * dGa;
+                            Ab += g * dGb;
+                            ++index2;
+                        }
+                    ++index1;
+                }
+            atomicAdd(&dcoef[qa], cb * pref0 * Bsum);
+            atomicAdd(&dcoef[qb], ca * pref0 * Bsum);
+            atomicAdd(&dalpha[qa], ca * cb * pref0 * Aa);
+            atomicAdd(&dalpha[qb], ca * cb * pref0 * Ab);
+        }
+}
+
+template <typename T>
+__global__ void widen_kernel(int64_t n, const T *__restrict__ a, const T *__restrict__ b, double *__restrict__ oa,
+                             double *__restrict__ ob)
+{
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    oa[q] = static_cast<double>(a[q]);
+    ob[q] = static_cast<double>(b[q]);
+}
+
+template <typename T>
+__global__ void fill_zero(int64_t n, T *__restrict__ a)
+{
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = T(0);
+}
+
+// ---------------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------------
+
+
+static int lmax_of(const int64_t *ls, int64_t ns)
+{
+    int m = 0;
+    for (int64_t q = 0; q < ns; ++q) m = std::max<int>(m, static_cast<int>(ls[q]));
+    return m;
+}
+
+constexpr int OVL_STRIPE_CAP = 96;
+
+template <typename T>
+static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w)
+{
+    char *ws = static_cast<char *>(ctx->ws);
+    using R = typename RecOf<T>::type;
+    const int nblk_h = w.nblk_h;
+    hipLaunchKernelGGL(ovl_hist_kernel, dim3(nblk_h), dim3(256), 0, ctx->stream, c.sidxA, c.sidxB, c.itype, c.B,
+                       sh.NZ, reinterpret_cast<int *>(ws + w.rank), reinterpret_cast<int *>(ws + w.blk_hist),
+                       ctx->d_err);
+    AOK_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL((ovl_plan_kernel<T>), dim3(1), dim3(256), 0, ctx->stream, sh, c.mode,
+                       static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha), static_cast<const T *>(c.D),
+                       reinterpret_cast<double *>(ws + w.coef64), reinterpret_cast<double *>(ws + w.alpha64),
+                       reinterpret_cast<int *>(ws + w.nreal), reinterpret_cast<int *>(ws + w.nlist),
+                       reinterpret_cast<int *>(ws + w.lidx), reinterpret_cast<int *>(ws + w.pcls), nblk_h,
+                       reinterpret_cast<const int *>(ws + w.blk_hist), reinterpret_cast<int *>(ws + w.blk_off),
+                       reinterpret_cast<int *>(ws + w.bcnt), reinterpret_cast<int *>(ws + w.boff),
+                       reinterpret_cast<int *>(ws + w.toff));
+    AOK_HIP(ctx, hipGetLastError());
+    const int nblk_tile = (w.maxtiles + 255) / 256;
+    const int64_t nrec = static_cast<int64_t>(w.NK) * sh.ns * sh.ns * sh.K * sh.K;
+    const int nblk_rec = static_cast<int>((nrec + 255) / 256);
+    hipLaunchKernelGGL((ovl_fill_kernel<T>), dim3(nblk_h + nblk_tile + nblk_rec), dim3(256), 0, ctx->stream, sh,
+                       c.sidxA, c.sidxB, c.itype, c.B, nblk_h, nblk_tile, reinterpret_cast<const int *>(ws + w.rank),
+                       reinterpret_cast<const int *>(ws + w.blk_off), reinterpret_cast<const int *>(ws + w.toff),
+                       reinterpret_cast<int *>(ws + w.perm), reinterpret_cast<int *>(ws + w.tile_key), w.maxtiles,
+                       reinterpret_cast<const double *>(ws + w.coef64), reinterpret_cast<const double *>(ws + w.alpha64),
+                       reinterpret_cast<const int *>(ws + w.nlist), reinterpret_cast<const int *>(ws + w.lidx),
+                       reinterpret_cast<R *>(ws + w.rec));
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T, int LMAXB>
+static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w)
+{
+    char *ws = static_cast<char *>(ctx->ws);
+    using R = typename RecOf<T>::type;
+    const size_t shm = 256 + sizeof(T) * OVL_TILE * sh.str;
+    auto kern = &ovl2c_fwd_fast<T, LMAXB>;
+    if (shm > 64 * 1024)
+        AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
+    time_begin(ctx, "ovl2c_fwd_fast");
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles), dim3(OVL_TILE), shm, ctx->stream, sh, static_cast<const T *>(c.posA),
+                       static_cast<const T *>(c.posB), static_cast<T>(c.scale),
+                       reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
+                       reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
+                       reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
+                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out));
+    time_end(ctx);
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T, int LMAXB>
+static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w)
+{
+    char *ws = static_cast<char *>(ctx->ws);
+    using R = typename RecOf<T>::type;
+    const size_t shm = 256 + sizeof(double) * 4 * sh.ns * sh.K + sizeof(T) * OVL_TILE * sh.str;
+    auto kern = &ovl2c_bwd_fast<T, LMAXB>;
+    if (shm > 64 * 1024)
+        AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
+    time_begin(ctx, "ovl2c_bwd_fast");
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles), dim3(OVL_TILE), shm, ctx->stream, sh, static_cast<const T *>(c.dS),
+                       static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
+                       reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
+                       reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
+                       reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
+                       reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.nlist),
+                       reinterpret_cast<double *>(ws + w.partial));
+    time_end(ctx);
+    AOK_HIP(ctx, hipGetLastError());
+    const int nslot = sh.NZ * sh.ns * sh.K;
+    hipLaunchKernelGGL(ovl2c_bwd_finalize, dim3((nslot + 127) / 128), dim3(128), 0, ctx->stream, sh,
+                       reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.toff),
+                       reinterpret_cast<const int *>(ws + w.pcls), reinterpret_cast<const double *>(ws + w.partial),
+                       reinterpret_cast<double *>(ws + w.dc64), reinterpret_cast<double *>(ws + w.da64));
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T>
+static aok_status out_stage(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const double *dc64,
+                            const double *da64)
+{
+    const int npar = sh.NZ * sh.ns * sh.K;
+    if (c.vjp) {
+        const int nidx = sh.NZ * sh.ns;
+        hipLaunchKernelGGL((ovl2c_vjp_out<T>), dim3((nidx + 63) / 64), dim3(64), 0, ctx->stream, sh, dc64, da64,
+                           static_cast<const T *>(c.alpha), static_cast<const T *>(c.D), static_cast<T *>(c.o1),
+                           static_cast<T *>(c.o2));
+    } else {
+        hipLaunchKernelGGL((ovl2c_cast_out<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dc64,
+                           da64, static_cast<T *>(c.o1), static_cast<T *>(c.o2));
+    }
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T, int LMAXP>
+static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, bool pullback)
+{
+    const int64_t items = c.B * sh.ns * sh.ns;
+    const int grid = static_cast<int>((items + 255) / 256);
+    const T *coef = static_cast<const T *>(c.coef);
+    if (c.mode == 1) {  // compile coef in FP64 into the workspace first
+        const size_t npar = static_cast<size_t>(sh.NZ) * sh.ns * sh.K;
+        aok_status st = ensure_ws(ctx, sizeof(T) * npar + 2 * sizeof(double) * npar + 1024);
+        if (st != AOK_OK) return st;
+        T *cw = static_cast<T *>(ctx->ws);
+        hipLaunchKernelGGL((compile_coef_kernel<T>), dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh,
+                           static_cast<const T *>(c.alpha), static_cast<const T *>(c.D), cw);
+        AOK_HIP(ctx, hipGetLastError());
+        coef = cw;
+    }
+    if (!pullback) {
+        if (grid == 0) return AOK_OK;
+        time_begin(ctx, "ovl2c_fwd_plain");
+        hipLaunchKernelGGL((ovl2c_fwd_plain<T, LMAXP>), dim3(grid), dim3(256), 0, ctx->stream, sh, c.B, coef,
+                           static_cast<const T *>(c.alpha), c.sidxA, c.sidxB, c.itype, static_cast<const T *>(c.posA),
+                           static_cast<const T *>(c.posB), static_cast<T>(c.scale), static_cast<T *>(c.out),
+                           ctx->d_err);
+        time_end(ctx);
+        AOK_HIP(ctx, hipGetLastError());
+        return AOK_OK;
+    }
+    // pullback: T-precision atomics like the reference, into (dcoef, dalpha) buffers
+    const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
+    T *dcoef, *dalpha;
+    double *dc64 = nullptr, *da64 = nullptr;
+    if (c.vjp) {
+        const size_t off0 = align256(sizeof(T) * npar);  // compiled coef (mode 1) lives at ws + 0
+        aok_status st = ensure_ws(ctx, off0 + 2 * align256(sizeof(T) * npar) + 2 * align256(sizeof(double) * npar));
+        if (st != AOK_OK) return st;
+        char *ws = static_cast<char *>(ctx->ws);
+        dcoef = reinterpret_cast<T *>(ws + off0);
+        dalpha = reinterpret_cast<T *>(ws + off0 + align256(sizeof(T) * npar));
+        dc64 = reinterpret_cast<double *>(ws + off0 + 2 * align256(sizeof(T) * npar));
+        da64 = reinterpret_cast<double *>(ws + off0 + 2 * align256(sizeof(T) * npar) + align256(sizeof(double) * npar));
+    } else {
+        dcoef = static_cast<T *>(c.o1);
+        dalpha = static_cast<T *>(c.o2);
+    }
+    hipLaunchKernelGGL((fill_zero<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dcoef);
+    hipLaunchKernelGGL((fill_zero<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dalpha);
+    if (grid > 0) {
+        time_begin(ctx, "ovl2c_bwd_plain");
+        hipLaunchKernelGGL((ovl2c_bwd_plain<T, LMAXP>), dim3(grid), dim3(256), 0, ctx->stream, sh, c.B,
+                           static_cast<const T *>(c.dS), coef, static_cast<const T *>(c.alpha), c.sidxA, c.sidxB,
+                           c.itype, static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
+                           static_cast<T>(c.scale), dcoef, dalpha, ctx->d_err);
+        time_end(ctx);
+    }
+    AOK_HIP(ctx, hipGetLastError());
+    if (c.vjp) {
+        // widen to FP64, then the FP64 normalisation VJP
+        hipLaunchKernelGGL((widen_kernel<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dcoef,
+                           dalpha, dc64, da64);
+        AOK_HIP(ctx, hipGetLastError());
+        return out_stage<T>(ctx, c, sh, dc64, da64);
+    }
+    return AOK_OK;
+}
+
+// Entry used by the C ABI: validation, workspace, dispatch on the basis' largest l.
+template <typename T>
+aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback)
+{
+    OvlShape sh;
+    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, OVL_STRIPE_CAP, sh);
+    if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
+    if (c.B < 0) return fail(ctx, AOK_ERR_ARG, "B must be >= 0");
+    if (c.B > (int64_t(1) << 30)) return fail(ctx, AOK_ERR_ARG, "B too large");
+    const int lmax = lmax_of(c.ls, c.ns);
+    if (c.flags & AOK_PLAIN) {
+        switch (lmax) {
+            case 0: return plain_launch<T, 0>(ctx, c, sh, pullback);
+            case 1: return plain_launch<T, 1>(ctx, c, sh, pullback);
+            case 2: return plain_launch<T, 2>(ctx, c, sh, pullback);
+            case 3: return plain_launch<T, 3>(ctx, c, sh, pullback);
+            default: return plain_launch<T, 4>(ctx, c, sh, pullback);
+        }
+    }
+    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sizeof(typename RecOf<T>::type), pullback);
+    aok_status st = ensure_ws(ctx, w.total);
+    if (st != AOK_OK) return st;
+    if (c.B == 0) {
+        if (!pullback) return AOK_OK;
+        const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
+        char *ws = static_cast<char *>(ctx->ws);
+        hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
+                           reinterpret_cast<double *>(ws + w.dc64));
+        hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
+                           reinterpret_cast<double *>(ws + w.da64));
+        AOK_HIP(ctx, hipGetLastError());
+        return out_stage<T>(ctx, c, sh, reinterpret_cast<double *>(ws + w.dc64),
+                            reinterpret_cast<double *>(ws + w.da64));
+    }
+    st = fast_prep<T>(ctx, c, sh, w);
+    if (st != AOK_OK) return st;
+    if (!pullback) {
+        switch (lmax) {
+            case 0: return fast_fwd_launch<T, 0>(ctx, c, sh, w);
+            case 1: return fast_fwd_launch<T, 1>(ctx, c, sh, w);
+            case 2: return fast_fwd_launch<T, 2>(ctx, c, sh, w);
+            case 3: return fast_fwd_launch<T, 3>(ctx, c, sh, w);
+            default: return fast_fwd_launch<T, 4>(ctx, c, sh, w);
+        }
+    }
+    switch (lmax) {
+        case 0: st = fast_bwd_launch<T, 0>(ctx, c, sh, w); break;
+        case 1: st = fast_bwd_launch<T, 1>(ctx, c, sh, w); break;
+        case 2: st = fast_bwd_launch<T, 2>(ctx, c, sh, w); break;
+        case 3: st = fast_bwd_launch<T, 3>(ctx, c, sh, w); break;
+        default: st = fast_bwd_launch<T, 4>(ctx, c, sh, w); break;
+    }
+    if (st != AOK_OK) return st;
+    char *ws = static_cast<char *>(ctx->ws);
+    return out_stage<T>(ctx, c, sh, reinterpret_cast<double *>(ws + w.dc64), reinterpret_cast<double *>(ws + w.da64));
+}
+
+template <typename T>
+aok_status compile_coef_run(aok_ctx ctx, const void *zeta, const void *D, int64_t ns, int64_t K, int64_t NZ,
+                            const int64_t *ls, void *coef)
+{
+    OvlShape sh;
+    const std::string msg = build_shape(ns, K, NZ, ls, 64, 1 << 30, sh);
+    if (!msg.empty()) return fail(ctx, AOK_ERR_ARG, msg);
+    hipLaunchKernelGGL((compile_coef_kernel<T>), dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh,
+                       static_cast<const T *>(zeta), static_cast<const T *>(D), static_cast<T *>(coef));
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T>
+aok_status compile_coef_pb_run(aok_ctx ctx, const void *dcoef, const void *zeta, const void *D, int64_t ns,
+                               int64_t K, int64_t NZ, const int64_t *ls, void *dD, void *dzeta_norm)
+{
+    OvlShape sh;
+    const std::string msg = build_shape(ns, K, NZ, ls, 64, 1 << 30, sh);
+    if (!msg.empty()) return fail(ctx, AOK_ERR_ARG, msg);
+    const int64_t npar = static_cast<int64_t>(NZ) * ns * K;
+    aok_status st = ensure_ws(ctx, 2 * align256(sizeof(double) * npar));
+    if (st != AOK_OK) return st;
+    double *dc64 = static_cast<double *>(ctx->ws);
+    double *zero64 = reinterpret_cast<double *>(static_cast<char *>(ctx->ws) + align256(sizeof(double) * npar));
+    hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, zero64);
+    hipLaunchKernelGGL((widen_kernel<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
+                       static_cast<const T *>(dcoef), static_cast<const T *>(dcoef), dc64, dc64);
+    hipLaunchKernelGGL((ovl2c_vjp_out<T>), dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh, dc64,
+                       zero64, static_cast<const T *>(zeta), static_cast<const T *>(D), static_cast<T *>(dzeta_norm),
+                       static_cast<T *>(dD));
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template aok_status ovl2c_run<float>(aok_ctx, const Ovl2cCall &, bool);
+template aok_status ovl2c_run<double>(aok_ctx, const Ovl2cCall &, bool);
+template aok_status compile_coef_run<float>(aok_ctx, const void *, const void *, int64_t, int64_t, int64_t,
+                                            const int64_t *, void *);
+template aok_status compile_coef_run<double>(aok_ctx, const void *, const void *, int64_t, int64_t, int64_t,
+                                             const int64_t *, void *);
+template aok_status compile_coef_pb_run<float>(aok_ctx, const void *, const void *, const void *, int64_t, int64_t,
+                                               int64_t, const int64_t *, void *, void *);
+template aok_status compile_coef_pb_run<double>(aok_ctx, const void *, const void *, const void *, int64_t, int64_t,
+                                                int64_t, const int64_t *, void *, void *);
+
+}  // namespace aok

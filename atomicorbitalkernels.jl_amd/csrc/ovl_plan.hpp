@@ -1,0 +1,188 @@
+// ovl_plan.hpp -- shared layout of the overlap kernels' launch plans and workspace.
+//
+// Work decomposition of the fast 2-centre kernels:
+//  * batch elements b are stably bucketed by species pair (sigma_A, sigma_B) on the
+//    device, and every wave takes 64 b's of one bucket, so primitive data, shell
+//    structure and loop trip counts are wave-uniform;
+//  * per (species pair, shell pair) the primitive-pair constants are precomputed once
+//    per call in FP64 ("records"); the kernels read them as uniform data;
+//  * the N x N block of each b is produced shell-column by shell-column ("stripes":
+//    all rows of shells sa in [sa0, sa1) times the nbf(sb) columns of shell sb), staged
+//    in LDS and written (or, for the cotangent, read) with coalesced accesses, because
+//    out[:, :, b] is contiguous in the Julia layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "../../include/aok.h"
+
+namespace aok {
+
+constexpr int OVL_MAXSH = 64;    // shells per species
+constexpr int OVL_MAXCH = 160;   // stripe chunks per call
+constexpr int OVL_MAXK = 32;     // primitive slots per shell
+constexpr int OVL_MAXNZ = 16;    // species
+constexpr int OVL_TILE = 64;     // batch elements per wave
+
+struct OvlChunk {
+    int16_t sb, sa0, sa1, row0, nrows;
+};
+
+struct OvlShape {
+    int ns, K, NZ, N;
+    int nchunk;
+    int str;  // LDS floats per lane (odd)
+    int8_t ls[OVL_MAXSH];
+    int16_t off[OVL_MAXSH + 1];
+    OvlChunk ch[OVL_MAXCH];
+};
+
+// One primitive pair (record) of the fast kernels.  mu2 = alpha*beta/p (times log2(e)
+// in the FP32 variant); aop = alpha/p, bop = beta/p, i2p = 1/(2p); Kab = ca cb (pi/p)^{3/2};
+// w1 = cb (pi/p)^{3/2}, w2 = ca (pi/p)^{3/2} (pullback weights).
+struct alignas(32) PairRecF {
+    double mu2;
+    float aop, bop, i2p, Kab, w1, w2;
+};
+struct alignas(64) PairRecD {
+    double mu2, aop, bop, i2p, Kab, w1, w2, pad;
+};
+template <typename T>
+struct RecOf;
+template <>
+struct RecOf<float> {
+    using type = PairRecF;
+};
+template <>
+struct RecOf<double> {
+    using type = PairRecD;
+};
+
+__host__ __device__ inline int nbf_of(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Stripe plan: for each ket shell sb, consecutive bra shells grouped while
+// rows * nbf(sb) fits the per-lane LDS capacity `cap`.
+inline std::string build_shape(int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, int lmax_allowed, int cap,
+                               OvlShape &s)
+{
+    if (ns < 1 || ns > OVL_MAXSH) return "number of shells must be in 1.." + std::to_string(OVL_MAXSH);
+    if (K < 1 || K > OVL_MAXK) return "primitive slots K must be in 1.." + std::to_string(OVL_MAXK);
+    if (NZ < 1 || NZ > OVL_MAXNZ) return "number of species must be in 1.." + std::to_string(OVL_MAXNZ);
+    s.ns = static_cast<int>(ns);
+    s.K = static_cast<int>(K);
+    s.NZ = static_cast<int>(NZ);
+    s.off[0] = 0;
+    for (int64_t q = 0; q < ns; ++q) {
+        if (ls[q] < 0 || ls[q] > lmax_allowed) return "shell l out of range (max " + std::to_string(lmax_allowed) + ")";
+        s.ls[q] = static_cast<int8_t>(ls[q]);
+        s.off[q + 1] = static_cast<int16_t>(s.off[q] + nbf_of(static_cast<int>(ls[q])));
+    }
+    s.N = s.off[ns];
+    int maxlen = 0;
+    s.nchunk = 0;
+    for (int sb = 0; sb < ns; ++sb) {
+        const int nb = nbf_of(s.ls[sb]);
+        int sa = 0;
+        while (sa < ns) {
+            int sa1 = sa, rows = 0;
+            while (sa1 < ns && (rows + nbf_of(s.ls[sa1])) * nb <= cap) rows += nbf_of(s.ls[sa1++]);
+            if (sa1 == sa) {  // a single shell pair larger than cap still gets its own chunk
+                rows = nbf_of(s.ls[sa]);
+                sa1 = sa + 1;
+            }
+            if (s.nchunk >= OVL_MAXCH) return "too many stripe chunks";
+            OvlChunk &c = s.ch[s.nchunk++];
+            c.sb = static_cast<int16_t>(sb);
+            c.sa0 = static_cast<int16_t>(sa);
+            c.sa1 = static_cast<int16_t>(sa1);
+            c.row0 = s.off[sa];
+            c.nrows = static_cast<int16_t>(rows);
+            maxlen = std::max(maxlen, rows * nb);
+            sa = sa1;
+        }
+    }
+    s.str = maxlen | 1;
+    return std::string();
+}
+
+// Workspace map of one fast 2c call (byte offsets, 256-aligned).
+struct Ovl2cWs {
+    size_t coef64, alpha64, nreal, nlist, lidx, pcls, rec;
+    size_t blk_hist, rank, blk_off, bcnt, boff, toff, perm, tile_key;
+    size_t partial, dc64, da64;
+    size_t total;
+    int nblk_h, NK, maxtiles;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, size_t rec_bytes, bool pullback)
+{
+    Ovl2cWs w{};
+    const size_t npar = static_cast<size_t>(ns) * K * NZ;
+    w.NK = NZ * NZ;
+    w.nblk_h = static_cast<int>((B + 255) / 256);
+    w.maxtiles = static_cast<int>((B + OVL_TILE - 1) / OVL_TILE) + w.NK;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    w.coef64 = take(sizeof(double) * npar);
+    w.alpha64 = take(sizeof(double) * npar);
+    w.nreal = take(sizeof(int) * NZ * ns);
+    w.nlist = take(sizeof(int) * NZ * ns);
+    w.lidx = take(sizeof(int) * npar);
+    w.pcls = take(sizeof(int) * npar);
+    w.rec = take(rec_bytes * static_cast<size_t>(w.NK) * ns * ns * K * K);
+    w.blk_hist = take(sizeof(int) * static_cast<size_t>(w.nblk_h) * w.NK);
+    w.rank = take(sizeof(int) * static_cast<size_t>(B > 0 ? B : 1));
+    w.blk_off = take(sizeof(int) * static_cast<size_t>(w.nblk_h) * w.NK);
+    w.bcnt = take(sizeof(int) * w.NK);
+    w.boff = take(sizeof(int) * w.NK);
+    w.toff = take(sizeof(int) * (w.NK + 1));
+    w.perm = take(sizeof(int) * static_cast<size_t>(B > 0 ? B : 1));
+    w.tile_key = take(sizeof(int) * static_cast<size_t>(w.maxtiles));
+    if (pullback) {
+        w.partial = take(sizeof(double) * static_cast<size_t>(w.maxtiles) * 2 * ns * K * 2);
+        w.dc64 = take(sizeof(double) * npar);
+        w.da64 = take(sizeof(double) * npar);
+    }
+    w.total = o;
+    return w;
+}
+
+// One 2-centre call as seen by the kernels' host side.
+struct Ovl2cCall {
+    unsigned flags;
+    const void *posA, *posB, *sidxA, *sidxB;
+    int itype;
+    int64_t B;
+    const int64_t *ls;
+    int64_t ns, K, NZ;
+    const void *coef;   // mode 0
+    const void *alpha;  // exponents (zeta)
+    const void *D;      // mode 1
+    int mode;
+    double scale;
+    void *out;
+    const void *dS;
+    void *o1, *o2;  // pullback outputs: (dcoef, dalpha) or (dzeta, dD)
+    bool vjp;
+};
+
+template <typename T>
+aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback);
+template <typename T>
+aok_status compile_coef_run(aok_ctx ctx, const void *zeta, const void *D, int64_t ns, int64_t K, int64_t NZ,
+                            const int64_t *ls, void *coef);
+template <typename T>
+aok_status compile_coef_pb_run(aok_ctx ctx, const void *dcoef, const void *zeta, const void *D, int64_t ns,
+                               int64_t K, int64_t NZ, const int64_t *ls, void *dD, void *dzeta_norm);
+
+}  // namespace aok

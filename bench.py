@@ -1,0 +1,262 @@
+"""Benchmark of the hot path on one MI355X.
+
+Workload (the configuration the metric is quoted on): 2-centre Cartesian-Gaussian
+overlaps of B = 65536 atom pairs over 3 synthetic double-zeta-shaped species
+(9s4p1d -> 3s2p1d, N = 15 Cartesian functions, K = 9 primitive slots, exponents
+log-uniform in [0.1, 7000], mixed-sign general contractions), FP32.  One step = the
+forward overlap S (batch_overlap with ps) + the (zeta, D) pullback of a fixed
+cotangent dS (pullback_ps), both through libaok.so; inputs resident in HBM.
+
+Also reported (extra keys): forward-only and pullback-only pairs/s, the plain-FP64
+GPU form (the reference decomposition) and the speedup over it, the FP32 error
+against FP64, and atomic-orbital evaluation at nX = 2^24 (30 orbitals, 3 species)
+in GB/s against the 8 TB/s HBM peak.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "atomicorbitalkernels.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+METRIC = "2-ctr overlap pairs/s + (ζ,D)-grad pairs/s, FP32@1e-6 vs plain FP64; AO GB/s"
+
+
+def e2(la, lb):
+    return sum(i + j + 1 for i in range(la + 1) for j in range(lb + 1)) - 1
+
+
+def nbf(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def f2(la, lb):
+    return 30 + 3 * (5 * e2(la, lb) + 3) + 4 * nbf(la) * nbf(lb)
+
+
+def g2(la, lb):
+    return 40 + 3 * (5 * e2(la + 2, lb + 2) + 3) + 25 * nbf(la) * nbf(lb)
+
+
+def algorithmic_flops(basis, sidxA, sidxB):
+    """Counting model: forward F2 per non-skipped primitive pair; pullback G2 per
+    real x real pair plus F2 per (distinct padded exponent x real) pair."""
+    import numpy as np
+
+    NZ = basis.NZ
+    nreal = (basis.D != 0).sum(axis=1)  # (ns, NZ)
+    npad = np.zeros_like(nreal)
+    for s in range(basis.ns):
+        for g in range(NZ):
+            pad = basis.zeta[s, basis.D[s, :, g] == 0, g]
+            npad[s, g] = len(set(pad.tolist()))
+    fwd = np.zeros((NZ, NZ))
+    bwd = np.zeros((NZ, NZ))
+    for a in range(NZ):
+        for b in range(NZ):
+            for sa in range(basis.ns):
+                for sb in range(basis.ns):
+                    la, lb = basis.ls[sa], basis.ls[sb]
+                    rr = nreal[sa, a] * nreal[sb, b]
+                    fwd[a, b] += rr * f2(la, lb)
+                    bwd[a, b] += rr * g2(la, lb) + (npad[sa, a] * nreal[sb, b] + nreal[sa, a] * npad[sb, b]) * f2(la, lb)
+    cnt = np.zeros((NZ, NZ))
+    np.add.at(cnt, (np.asarray(sidxA) - 1, np.asarray(sidxB) - 1), 1)
+    return float((fwd * cnt).sum()), float((bwd * cnt).sum())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the kernels run on a single MI355X); --gpus {args.gpus} is not "
+              f"supported. Multi-GPU sharding is described in DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import aok
+    from aok import configs
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    B = 65536
+    d = configs.c1(1, B=B)
+    cb = d["basis"]
+    spec, _, _ = cb.ao_maps()
+    zlist = (1, 2, 3)
+
+    def orbital(dtype):
+        R = aok.GaussianTypeRadials(aok.fortran_from_numpy(cb.zeta, dtype, dev),
+                                    aok.fortran_from_numpy(cb.D, dtype, dev), spec, None, zlist)
+        return aok.AtomicOrbitals(R, 2, length_unit="bohr")
+
+    def inputs(dtype):
+        XA = (torch.as_tensor(d["posA"], dtype=dtype, device=dev).contiguous(),
+              torch.as_tensor(d["sidxA"], dtype=torch.int32, device=dev))
+        XB = (torch.as_tensor(d["posB"], dtype=dtype, device=dev).contiguous(),
+              torch.as_tensor(d["sidxB"], dtype=torch.int32, device=dev))
+        return XA, XB
+
+    ctx = aok.context(0)
+    ctx.set_timing(True)
+    orb32 = orbital(torch.float32)
+    ps32 = orb32.static_params()
+    XA32, XB32 = inputs(torch.float32)
+    g = torch.Generator(device=dev).manual_seed(7)
+    dS32 = torch.randn((B, cb.N, cb.N), generator=g, device=dev, dtype=torch.float32).permute(2, 1, 0)
+
+    kern = {"ovl2c_fwd_fast": [], "ovl2c_bwd_fast": []}
+
+    def step(record):
+        S = aok.batch_overlap(orb32, XA32, XB32, ps32, ctx=ctx)
+        if record:
+            kern[ctx.last_kernel_name()].append(ctx.last_kernel_ms())
+        dps = aok.overlap_pullback_ps(dS32, orb32, XA32, XB32, ps32, ctx=ctx)
+        if record:
+            kern[ctx.last_kernel_name()].append(ctx.last_kernel_ms())
+        return S, dps
+
+    log(f"C1 warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    log(f"C1 timed {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = B / (ms_step * 1e-3)
+    fwd_ms = float(np.mean(kern["ovl2c_fwd_fast"]))
+    bwd_ms = float(np.mean(kern["ovl2c_bwd_fast"]))
+    flops_fwd, flops_bwd = algorithmic_flops(cb, d["sidxA"], d["sidxB"])
+    dom = "ovl2c_bwd_fast" if bwd_ms >= fwd_ms else "ovl2c_fwd_fast"
+    dom_ms = max(fwd_ms, bwd_ms)
+    dom_flops = flops_bwd if dom == "ovl2c_bwd_fast" else flops_fwd
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2c overlap forward + (zeta,D) pullback, B=65536 pairs, 3 species 9s4p1d->3s2p1d "
+                               "(N=15, K=9), zeta log-uniform [0.1,7000], FP32", "global_batch": B,
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": dom,
+                     "kernel_ms": dom_ms, "algorithmic_flop_per_launch": dom_flops,
+                     "note": "VALU FP32 kernel priced against the published FP32 peak"},
+        "fwd_kernel_ms": fwd_ms, "bwd_kernel_ms": bwd_ms,
+        "fwd_kernel_pairs_per_s": B / (fwd_ms * 1e-3), "grad_kernel_pairs_per_s": B / (bwd_ms * 1e-3),
+    }
+
+    if not args.no_extra:
+        # FP32 vs FP64 (fast FP64 kernels, themselves pinned to the oracle by the tests)
+        log("C1 FP64 reference run for the FP32 error")
+        orb64 = orbital(torch.float64)
+        XA64, XB64 = inputs(torch.float64)
+        z32 = torch.as_tensor(cb.zeta.astype(np.float32), dtype=torch.float64)
+        # feed FP64 the FP32-rounded inputs
+        ps64 = {"Rnl": {"zeta": aok.as_fortran(orb32.Rnl.zeta.double()), "D": aok.as_fortran(orb32.Rnl.D.double())},
+                "Ylm": {}}
+        XA64r = (XA32[0].double().contiguous(), XA32[1])
+        XB64r = (XB32[0].double().contiguous(), XB32[1])
+        S32, _ = step(False)
+        S64 = aok.batch_overlap(orb64, XA64r, XB64r, ps64, ctx=ctx)
+        num = (S32.double() - S64).abs().amax(dim=(0, 1))
+        den = S64.abs().amax(dim=(0, 1)).clamp_min(1e-300)
+        out["fp32_blockwise_max_rel_err"] = float((num / den).max())
+        big = S64.abs() >= 1e-3 * S64.abs().amax(dim=(0, 1), keepdim=True)
+        rel = ((S32.double() - S64).abs() / S64.abs().clamp_min(1e-300))[big]
+        out["fp32_elementwise_max_rel_err_tau1e-3"] = float(rel.max())
+        del z32
+        # plain FP64 GPU form (reference decomposition) -- the >= 4x denominator
+        log("C1 plain FP64 forward/pullback")
+        dS64 = dS32.double()
+        dS64 = aok.as_fortran(dS64)
+        tp = {}
+        for name, fn in (("fwd", lambda: aok.batch_overlap(orb64, XA64, XB64, orb64.static_params(), mode="plain",
+                                                           ctx=ctx)),
+                         ("bwd", lambda: aok.overlap_pullback_ps(dS64, orb64, XA64, XB64, orb64.static_params(),
+                                                                 mode="plain", ctx=ctx))):
+            fn()
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            ks = []
+            for _ in range(3):
+                fn()
+                ks.append(ctx.last_kernel_ms())
+            torch.cuda.synchronize()
+            tp[name] = ((time.perf_counter() - ts) * 1e3 / 3, float(np.mean(ks)))
+        out["plain_f64_fwd_ms"], out["plain_f64_fwd_kernel_ms"] = tp["fwd"]
+        out["plain_f64_bwd_ms"], out["plain_f64_bwd_kernel_ms"] = tp["bwd"]
+        out["plain_f64_step_pairs_per_s"] = B / ((tp["fwd"][0] + tp["bwd"][0]) * 1e-3)
+        out["speedup_vs_plain_f64"] = {
+            "fwd": tp["fwd"][1] / fwd_ms, "bwd": tp["bwd"][1] / bwd_ms,
+            "fwd+bwd": (tp["fwd"][1] + tp["bwd"][1]) / (fwd_ms + bwd_ms),
+            "step_wall": (tp["fwd"][0] + tp["bwd"][0]) / ms_step}
+        # AO evaluation at nX = 2^24, 30 orbitals, 3 species, FP32 (HBM-bound)
+        log("AO nX=2^24 evaluation")
+        c4 = configs.c4b_basis()
+        spec4, _, _ = c4.ao_maps()
+        R4 = aok.GaussianTypeRadials(aok.fortran_from_numpy(c4.zeta, torch.float32, dev),
+                                     aok.fortran_from_numpy(c4.D, torch.float32, dev), spec4, None, (1, 2, 3))
+        orb4 = aok.AtomicOrbitals(R4, 3, length_unit="bohr")
+        nX = 2 ** 24
+        gx = torch.Generator(device=dev).manual_seed(4)
+        X = 3.0 * torch.randn((nX, 3), generator=gx, device=dev, dtype=torch.float32)
+        sidx = torch.randint(1, 4, (nX,), generator=gx, device=dev, dtype=torch.int32)
+        for _ in range(3):
+            P = aok.evaluate(orb4, X, sidx=sidx, ctx=ctx)
+        ks = []
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(10):
+            P = aok.evaluate(orb4, X, sidx=sidx, ctx=ctx)
+            ks.append(ctx.last_kernel_ms())
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - ts) * 1e3 / 10
+        bytes_alg = nX * (12 + 4 + 4 * len(orb4))
+        kms = float(np.mean(ks))
+        out["ao_c4b"] = {"nX": nX, "nOrb": len(orb4), "kernel_ms": kms, "wall_ms": wall,
+                         "GB_per_s": bytes_alg / (kms * 1e-3) / 1e9,
+                         "frac_of_hbm_peak": bytes_alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes": bytes_alg}
+        del P, X
+
+    if not args.no_cpu:
+        log("CPU baseline (C oracle, FP64, 1 thread)")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        nb = 512
+        coef = O.compile_coef(cb.zeta, cb.D, cb.ls)
+        dSn = np.random.default_rng(0).standard_normal((cb.N, cb.N, nb))
+        ts = time.perf_counter()
+        O.ovl2c(d["posA"][:nb], d["posB"][:nb], d["sidxA"][:nb], d["sidxB"][:nb], cb.ls, coef, cb.zeta)
+        O.pullback_ps_2c(dSn, d["posA"][:nb], d["posB"][:nb], d["sidxA"][:nb], d["sidxB"][:nb], cb.ls, cb.zeta,
+                         cb.D)
+        tcpu = time.perf_counter() - ts
+        out["cpu_baseline"] = {"value": nb / tcpu, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"first {nb} of the {B} C1 pairs: forward + (zeta,D) pullback, FP64 C "
+                                         f"restatement of the reference kernels ({tcpu:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

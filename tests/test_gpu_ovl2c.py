@@ -1,0 +1,208 @@
+"""GPU parity tests of the 2-centre overlap and its (zeta, D) pullback through the C ABI.
+
+Tolerances (written per assertion):
+  FP64 kernels (fast and plain) vs the FP64 oracle: 1e-12 relative to the block max
+  (values), 1e-10 (gradients, which sum many terms);
+  FP32 fast kernel vs the FP64 oracle on the same FP32-rounded inputs: blockwise
+  max_b max_ij |dS| / max_ij |S64_b| <= 1e-6 (the target metric), gradients 1e-5;
+  FP32 plain kernel (the reference's own FP32 algorithm): 1e-4 like
+  test/integrals/test_compile_orbitals.jl:138."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import aok
+import oracle as O
+from aok import configs
+from helpers import blockwise_rel, dev_f, dev_rows, elementwise_rel, host, log, rel_max, rounded
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.json")))
+
+
+def _cart_basis(ls, zeta, coef, dtype, device, zlist=None, lengthscale=1.0):
+    nbf = [(l + 1) * (l + 2) // 2 for l in ls]
+    off = [0]
+    for n in nbf:
+        off.append(off[-1] + n)
+    NZ = zeta.shape[2]
+    return aok.CartesianGTOBasis(max(ls), len(ls), zeta.shape[1], list(ls), nbf, off, off[-1],
+                                 dev_f(zeta, dtype, device), dev_f(coef, dtype, device),
+                                 tuple(zlist or range(1, NZ + 1)), lengthscale)
+
+
+def _orb(cb, dtype, device):
+    spec, _, _ = cb.ao_maps()
+    R = aok.GaussianTypeRadials(dev_f(cb.zeta, dtype, device), dev_f(cb.D, dtype, device), spec, None,
+                                tuple(range(1, cb.NZ + 1)))
+    return aok.AtomicOrbitals(R, max(cb.ls), length_unit="bohr")
+
+
+@pytest.mark.parametrize("mode", ["fast", "plain"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_golden_blocks(dev, dtype, mode):
+    """Quadrature golden vectors (shell pairs up to (4, 4)), via the kernel-facing layer."""
+    for blk in GOLD["blocks_2c"]:
+        la, lb = blk["la"], blk["lb"]
+        ka, kb = len(blk["za"]), len(blk["zb"])
+        K = max(ka, kb)
+        zeta = np.ones((2, K, 1))
+        coef = np.zeros((2, K, 1))
+        zeta[0, :ka, 0], coef[0, :ka, 0] = blk["za"], blk["ca"]
+        zeta[1, :kb, 0], coef[1, :kb, 0] = blk["zb"], blk["cb"]
+        log(f"2c golden l=({la},{lb}) dtype={dtype} mode={mode}")
+        basis = _cart_basis([la, lb], zeta, coef, dtype, dev)
+        A = dev_rows(np.array([blk["A"]]), dtype, dev)
+        B = dev_rows(np.array([blk["B"]]), dtype, dev)
+        out = aok.batch_overlap(basis, (A, None), (B, None), mode=mode)
+        nba, nbb = (la + 1) * (la + 2) // 2, (lb + 1) * (lb + 2) // 2
+        got = host(out)[:nba, nba:nba + nbb, 0]
+        ref = np.array(blk["out"]).reshape((nba, nbb), order="F")
+        # FP32: the golden inputs are FP64, so the tolerance also covers input rounding
+        t = 1e-12 if dtype == torch.float64 else (2e-5 if mode == "fast" else 1e-4)
+        assert rel_max(got, ref) < t, (la, lb)
+
+
+def _c1_small(B, seed=5):
+    d = configs.c1(1, B=B)
+    return d
+
+
+@pytest.mark.parametrize("mode,dtype", [("fast", torch.float64), ("plain", torch.float64), ("fast", torch.float32),
+                                        ("plain", torch.float32)])
+def test_c1_forward_vs_oracle(dev, mode, dtype):
+    B = 1500  # not a multiple of the 64-wide tiles
+    d = _c1_small(B)
+    cb = d["basis"]
+    log(f"2c C1 forward B={B} mode={mode} dtype={dtype}")
+    orb = _orb(cb, dtype, dev)
+    ps = orb.static_params()
+    pA, pB = dev_rows(d["posA"], dtype, dev), dev_rows(d["posB"], dtype, dev)
+    sA = torch.as_tensor(d["sidxA"], device=dev)
+    sB = torch.as_tensor(d["sidxB"], device=dev)
+    S = aok.batch_overlap(orb, (pA, sA), (pB, sB), ps, None, mode=mode)
+    coef = O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls)
+    Sr = O.ovl2c(rounded(d["posA"], dtype), rounded(d["posB"], dtype), d["sidxA"], d["sidxB"], cb.ls, coef,
+                 rounded(cb.zeta, dtype))
+    err = blockwise_rel(host(S), Sr)
+    if dtype == torch.float64:
+        assert err < 1e-12
+    elif mode == "fast":
+        assert err < 1e-6, err
+        el, excl = elementwise_rel(host(S), Sr, tau=1e-3)
+        print(f"fp32 fast: blockwise {err:.3e} elementwise(tau=1e-3) {el:.3e} excluded {excl}")
+    else:
+        assert err < 1e-4
+    # the compile-once form (coef given) agrees
+    basis = aok.compile_basis(orb)
+    S2 = aok.batch_overlap(basis, (pA, sA), (pB, sB), mode=mode)
+    assert blockwise_rel(host(S2), host(S)) < (1e-13 if dtype == torch.float64 else 1e-6)
+
+
+@pytest.mark.parametrize("mode,dtype", [("fast", torch.float64), ("plain", torch.float64), ("fast", torch.float32)])
+def test_c1_pullback_vs_oracle(dev, mode, dtype):
+    B = 200
+    d = _c1_small(B)
+    cb = d["basis"]
+    rng = np.random.default_rng(17)
+    dS = rng.standard_normal((cb.N, cb.N, B))
+    log(f"2c C1 pullback B={B} mode={mode} dtype={dtype}")
+    orb = _orb(cb, dtype, dev)
+    ps = orb.static_params()
+    pA, pB = dev_rows(d["posA"], dtype, dev), dev_rows(d["posB"], dtype, dev)
+    sA = torch.as_tensor(d["sidxA"], device=dev)
+    sB = torch.as_tensor(d["sidxB"], device=dev)
+    S, pb = aok.rrule_batch_overlap(orb, (pA, sA), (pB, sB), ps, None, mode=mode)
+    _, _, _, _, dps, _ = pb(dev_f(dS, dtype, dev))
+    dzr, dDr = O.pullback_ps_2c(rounded(dS, dtype), rounded(d["posA"], dtype), rounded(d["posB"], dtype),
+                                d["sidxA"], d["sidxB"], cb.ls, rounded(cb.zeta, dtype), rounded(cb.D, dtype))
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    assert rel_max(host(dps["Rnl"]["zeta"]), dzr) < tol
+    assert rel_max(host(dps["Rnl"]["D"]), dDr) < tol
+    # kernel-level (dcoef, dalpha) against the oracle's batch_S_orbital_pb_kernel!
+    basis = aok.compile_basis(orb)
+    dc, da = aok.overlap_pb_coef_alpha(dev_f(dS, dtype, dev), basis, (pA, sA), (pB, sB), mode=mode)
+    coef = O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls)
+    dcr, dar = O.ovl2c_pb(rounded(dS, dtype), rounded(d["posA"], dtype), rounded(d["posB"], dtype), d["sidxA"],
+                          d["sidxB"], cb.ls, coef, rounded(cb.zeta, dtype))
+    assert rel_max(host(dc), dcr) < (1e-10 if dtype == torch.float64 else 1e-5)
+    assert rel_max(host(da), dar) < (1e-10 if dtype == torch.float64 else 1e-5)
+    if mode == "fast":  # fixed-order FP64 reductions: bitwise reproducible
+        _, _, _, _, dps2, _ = pb(dev_f(dS, dtype, dev))
+        assert torch.equal(dps["Rnl"]["zeta"], dps2["Rnl"]["zeta"])
+        assert torch.equal(dps["Rnl"]["D"], dps2["Rnl"]["D"])
+
+
+def test_pstates_species_labels_and_errors(dev):
+    d = _c1_small(130)
+    cb = d["basis"]
+    log("2c pstates/errors")
+    zlist = (6, 7, 8)
+    spec, _, _ = cb.ao_maps()
+    R = aok.GaussianTypeRadials(dev_f(cb.zeta, torch.float64, dev), dev_f(cb.D, torch.float64, dev), spec, None,
+                                zlist)
+    orb = aok.AtomicOrbitals(R, 2, length_unit="bohr")
+    pA = dev_rows(d["posA"], torch.float64, dev)
+    pB = dev_rows(d["posB"], torch.float64, dev)
+    XA = aok.PStates(pA, [zlist[s - 1] for s in d["sidxA"]])
+    XB = aok.PStates(pB, [zlist[s - 1] for s in d["sidxB"]])
+    S1 = aok.batch_overlap(orb, XA, XB)
+    S2 = aok.batch_overlap(orb, (pA, torch.as_tensor(d["sidxA"], device=dev)),
+                           (pB, torch.as_tensor(d["sidxB"], device=dev)))
+    assert torch.equal(S1, S2)
+    with pytest.raises(aok.AOKSpeciesError):
+        aok.batch_overlap(orb, aok.PStates(pA[:1], [99]), aok.PStates(pB[:1], [6]))
+    with pytest.raises(aok.AOKArgumentError):  # XA and XB must have equal length
+        aok.batch_overlap(orb, XA, aok.PStates(pB[:3], [6, 6, 6]))
+    with pytest.raises(aok.AOKSpeciesError):  # out-of-range index caught on the device
+        aok.batch_overlap(orb, (pA[:2], torch.tensor([1, 4], device=dev)), (pB[:2], torch.tensor([1, 1], device=dev)))
+    # empty batch
+    S0 = aok.batch_overlap(orb, (pA[:0], None), (pB[:0], None))
+    assert S0.shape == (cb.N, cb.N, 0)
+
+
+def test_length_unit_overlap(dev):
+    """test/orbitals/test_length_unit.jl:241-247: the overlap inherits the unit."""
+    cb = configs.c0_basis(0)[0]
+    spec, _, _ = cb.ao_maps()
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, size=(10, 3))
+    Y = rng.normal(0, 1, size=(10, 3))
+    log("2c length unit")
+    outs = []
+    for unit, scale in (("angstrom", 1.0), ("bohr", aok.ANG2BOHR)):
+        R = aok.GaussianTypeRadials(dev_f(cb.zeta, torch.float64, dev), dev_f(cb.D, torch.float64, dev), spec)
+        orb = aok.AtomicOrbitals(R, 2, length_unit=unit)
+        outs.append(host(aok.batch_overlap(aok.compile_basis(orb), (dev_rows(scale * X, torch.float64, dev), None),
+                                           (dev_rows(scale * Y, torch.float64, dev), None))))
+    assert rel_max(outs[0], outs[1]) < 1e-12
+
+
+def test_c1_full_batch_fp32_vs_fp64(dev):
+    """Full-size C1 (B = 65536): FP32 fast vs FP64 fast on the GPU (size-independent check;
+    the FP64 fast kernel is itself pinned to the oracle above), plus an oracle subset."""
+    d = configs.c1(1, B=65536)
+    cb = d["basis"]
+    log("2c C1 full batch fp32 vs fp64")
+    res = {}
+    for dt in (torch.float64, torch.float32):
+        orb = _orb(cb, dt, dev)
+        pA = dev_rows(rounded(d["posA"], torch.float32), dt, dev)
+        pB = dev_rows(rounded(d["posB"], torch.float32), dt, dev)
+        sA = torch.as_tensor(d["sidxA"], device=dev)
+        sB = torch.as_tensor(d["sidxB"], device=dev)
+        zeta = dev_f(rounded(cb.zeta, torch.float32), dt, dev)
+        D = dev_f(rounded(cb.D, torch.float32), dt, dev)
+        res[dt] = host(aok.batch_overlap(orb, (pA, sA), (pB, sB), {"Rnl": {"zeta": zeta, "D": D}, "Ylm": {}}))
+    err = blockwise_rel(res[torch.float32], res[torch.float64])
+    el, excl = elementwise_rel(res[torch.float32], res[torch.float64], tau=1e-3)
+    print(f"C1 B=65536 fp32 blockwise max rel {err:.3e}; elementwise(tau=1e-3) {el:.3e}, excluded {excl}")
+    assert err < 1e-6
+    sub = slice(0, 512)
+    coef = O.compile_coef(rounded(cb.zeta, torch.float32), rounded(cb.D, torch.float32), cb.ls)
+    Sr = O.ovl2c(rounded(d["posA"], torch.float32)[sub], rounded(d["posB"], torch.float32)[sub], d["sidxA"][sub],
+                 d["sidxB"][sub], cb.ls, coef, rounded(cb.zeta, torch.float32))
+    assert blockwise_rel(res[torch.float64][:, :, sub], Sr) < 1e-12
