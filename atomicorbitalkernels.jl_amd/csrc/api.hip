@@ -136,7 +136,7 @@ static aok_status ao_common(aok_ctx ctx, aok_dtype dtype, const void *X, aok_ity
     AOK_REQUIRE(ctx, nX == 0 || X != nullptr, "X is NULL");
     AOK_REQUIRE(ctx, zeta && D, "zeta/D are NULL");
     AOK_REQUIRE(ctx, nRad >= 1 && K >= 1 && NZ >= 1, "nRad, K, NZ must be >= 1");
-    AOK_REQUIRE(ctx, NZ * nRad * K <= 16384, "parameter table too large");
+    AOK_REQUIRE(ctx, NZ * nRad * K <= 4096, "parameter table too large (NZ*nRad*K > 4096)");
     AOK_REQUIRE(ctx, nOrb >= 1 && iR && iY, "orbital maps iR/iY required");
     AOK_REQUIRE(ctx, decay == AOK_DECAY_GAUSSIAN || decay == AOK_DECAY_SLATER, "unknown decay");
     if (L < 0 || L > AOK_AO_LMAX)
@@ -299,10 +299,19 @@ aok_status aok_ovl3c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *p
                      const void *alpha, double lengthscale, void *out)
 {
     AOK_CHECK_CTX(ctx);
-    (void)dtype, (void)flags, (void)posA, (void)posB, (void)posC, (void)sidxA, (void)sidxB, (void)sidxC;
-    (void)itype, (void)B, (void)ns, (void)K, (void)NZ, (void)ls, (void)coef, (void)alpha, (void)lengthscale;
-    (void)out;
-    return fail(ctx, AOK_ERR_UNSUPPORTED, "aok_ovl3c: not built yet");
+    AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
+    AOK_REQUIRE(ctx, !bad_itype(itype), "itype must be AOK_I64 or AOK_I32");
+    AOK_REQUIRE(ctx, (flags & ~AOK_PLAIN) == 0u, "unknown flags");
+    AOK_REQUIRE(ctx, ls && coef && alpha, "NULL basis argument");
+    AOK_REQUIRE(ctx, B >= 0, "B must be >= 0");
+    AOK_REQUIRE(ctx, B == 0 || (posA && posB && posC && out), "positions/out are NULL");
+    Ovl3cCall c{flags, posA, posB, posC, sidxA, sidxB, sidxC, itype, B, ls, ns, K, NZ, coef, alpha, lengthscale, out};
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    st = dtype == AOK_F32 ? ovl3c_run<float>(ctx, c) : ovl3c_run<double>(ctx, c);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_ovl3c");
 }
 
 }  // extern "C"

@@ -1019,12 +1019,15 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
 {
     const int64_t items = c.B * sh.ns * sh.ns;
     const int grid = static_cast<int>((items + 255) / 256);
+    const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
+    // workspace: [compiled coef (T)] [dcoef, dalpha (T)] [dc64, da64 (double)], sized once
+    const size_t sT = align256(sizeof(T) * npar), sD = align256(sizeof(double) * npar);
+    aok_status st = ensure_ws(ctx, sT + 2 * sT + 2 * sD);
+    if (st != AOK_OK) return st;
+    char *ws = static_cast<char *>(ctx->ws);
     const T *coef = static_cast<const T *>(c.coef);
     if (c.mode == 1) {  // compile coef in FP64 into the workspace first
-        const size_t npar = static_cast<size_t>(sh.NZ) * sh.ns * sh.K;
-        aok_status st = ensure_ws(ctx, sizeof(T) * npar + 2 * sizeof(double) * npar + 1024);
-        if (st != AOK_OK) return st;
-        T *cw = static_cast<T *>(ctx->ws);
+        T *cw = reinterpret_cast<T *>(ws);
         hipLaunchKernelGGL((compile_coef_kernel<T>), dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh,
                            static_cast<const T *>(c.alpha), static_cast<const T *>(c.D), cw);
         AOK_HIP(ctx, hipGetLastError());
@@ -1042,22 +1045,10 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
         return AOK_OK;
     }
     // pullback: T-precision atomics like the reference, into (dcoef, dalpha) buffers
-    const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
-    T *dcoef, *dalpha;
-    double *dc64 = nullptr, *da64 = nullptr;
-    if (c.vjp) {
-        const size_t off0 = align256(sizeof(T) * npar);  // compiled coef (mode 1) lives at ws + 0
-        aok_status st = ensure_ws(ctx, off0 + 2 * align256(sizeof(T) * npar) + 2 * align256(sizeof(double) * npar));
-        if (st != AOK_OK) return st;
-        char *ws = static_cast<char *>(ctx->ws);
-        dcoef = reinterpret_cast<T *>(ws + off0);
-        dalpha = reinterpret_cast<T *>(ws + off0 + align256(sizeof(T) * npar));
-        dc64 = reinterpret_cast<double *>(ws + off0 + 2 * align256(sizeof(T) * npar));
-        da64 = reinterpret_cast<double *>(ws + off0 + 2 * align256(sizeof(T) * npar) + align256(sizeof(double) * npar));
-    } else {
-        dcoef = static_cast<T *>(c.o1);
-        dalpha = static_cast<T *>(c.o2);
-    }
+    T *dcoef = c.vjp ? reinterpret_cast<T *>(ws + sT) : static_cast<T *>(c.o1);
+    T *dalpha = c.vjp ? reinterpret_cast<T *>(ws + 2 * sT) : static_cast<T *>(c.o2);
+    double *dc64 = reinterpret_cast<double *>(ws + 3 * sT);
+    double *da64 = reinterpret_cast<double *>(ws + 3 * sT + sD);
     hipLaunchKernelGGL((fill_zero<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dcoef);
     hipLaunchKernelGGL((fill_zero<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dalpha);
     if (grid > 0) {

@@ -1,0 +1,653 @@
+// ovl3c_kernels.hip -- H2: batched 3-centre Cartesian-Gaussian overlaps on gfx950.
+//
+// V[mu, nu, lambda](b) = int phi_mu(r - A_b) phi_nu(r - B_b) phi_lambda(r - C_b) dr with the
+// semantics of batch_V_kernel! (src/integrals/kernels_3c.jl:12-162): contracted values
+// accumulated at lin = index1 + N1 index2 + N1 N2 index3 (N1 = 2l_a+1, N2 = 2l_b+1) and
+// read back as out[off_a+i, off_b+j, off_c+k] = blk[i + nbf_a (j + nbf_b k)]; generalised to
+// a species per centre with the orbital path's padded [ns x K x NZ] primitive layout.
+//
+// Fast path: batch elements are bucketed by species triple; a wave takes 64 b's of one
+// bucket.  Per primitive triple the constants are FP64 records built on the device:
+//   exponent  (ab |AB|^2 + ag |AC|^2 + bg |BC|^2) / q   (times log2 e), evaluated per lane
+//             in FP64 from the pairwise distances and range-reduced for v_exp_f32;
+//   geometry  PA = -(beta/p) AB, PB = (alpha/p) AB, PC = PA + AC,
+//             QA = PA - (gamma/q) PC, QB = PB - (gamma/q) PC, QC = (p/q) PC;
+//   1-D Obara-Saika three-centre overlaps s_ijk (s_000 = 1) per axis,
+//   s_{i+1,j,k} = QA s_ijk + (i s_{i-1,j,k} + j s_{i,j-1,k} + k s_{i,j,k-1}) / (2q).
+// Plain path: one work item per (b, s_a, s_b, s_c), runtime l loops, private E3 scratch
+// (the reference decomposition; the E3 table is zero-filled beyond t = i+j+k).
+#include "aok_common.hpp"
+#include "ovl_plan.hpp"
+
+namespace aok {
+
+constexpr int OVL3_MAXNZ = 8;
+constexpr int OVL3_FAST_LMAX = 2;
+
+struct alignas(16) TripRecF {
+    double cab, cac, cbc;  // (pairwise exponent products / q) * log2(e)
+    float aop, bop, gq, pq, i2q, Kabc;
+};
+struct alignas(16) TripRecD {
+    double cab, cac, cbc, aop, bop, gq, pq, i2q, Kabc, pad;
+};
+template <typename T>
+struct Rec3Of;
+template <>
+struct Rec3Of<float> {
+    using type = TripRecF;
+};
+template <>
+struct Rec3Of<double> {
+    using type = TripRecD;
+};
+
+struct Ovl3Ws {
+    size_t coef64, alpha64, nreal, lidx, grp_off, rec;
+    size_t blk_hist, rank, blk_off, bcnt, boff, toff, perm, tile_key;
+    size_t total;
+    int nblk_h, NK, maxtiles;
+    int64_t ngrp, maxrec;
+};
+
+static Ovl3Ws ovl3_ws_map(int64_t B, int ns, int K, int NZ, size_t rec_bytes)
+{
+    Ovl3Ws w{};
+    const size_t npar = static_cast<size_t>(ns) * K * NZ;
+    w.NK = NZ * NZ * NZ;
+    w.nblk_h = static_cast<int>((B + 255) / 256);
+    w.maxtiles = static_cast<int>((B + OVL_TILE - 1) / OVL_TILE) + w.NK;
+    w.ngrp = static_cast<int64_t>(w.NK) * ns * ns * ns;
+    w.maxrec = w.ngrp * K * K * K;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    w.coef64 = take(sizeof(double) * npar);
+    w.alpha64 = take(sizeof(double) * npar);
+    w.nreal = take(sizeof(int) * NZ * ns);
+    w.lidx = take(sizeof(int) * npar);
+    w.grp_off = take(sizeof(int64_t) * (w.ngrp + 1));
+    w.rec = take(rec_bytes * static_cast<size_t>(w.maxrec));
+    w.blk_hist = take(sizeof(int) * static_cast<size_t>(w.nblk_h) * w.NK);
+    w.rank = take(sizeof(int) * static_cast<size_t>(B > 0 ? B : 1));
+    w.blk_off = take(sizeof(int) * static_cast<size_t>(w.nblk_h) * w.NK);
+    w.bcnt = take(sizeof(int) * w.NK);
+    w.boff = take(sizeof(int) * w.NK);
+    w.toff = take(sizeof(int) * (w.NK + 1));
+    w.perm = take(sizeof(int) * static_cast<size_t>(B > 0 ? B : 1));
+    w.tile_key = take(sizeof(int) * static_cast<size_t>(w.maxtiles));
+    w.total = o;
+    return w;
+}
+
+__device__ __forceinline__ int key3(const void *sA, const void *sB, const void *sC, int itype, int64_t i, int NZ,
+                                    unsigned *err)
+{
+    int a = load_sigma(sA, itype, i), b = load_sigma(sB, itype, i), c = load_sigma(sC, itype, i);
+    const bool bad = a < 0 || a >= NZ || b < 0 || b >= NZ || c < 0 || c >= NZ;
+    if (bad && err) atomicOr(err, ERR_SPECIES);
+    a = (a < 0 || a >= NZ) ? 0 : a;
+    b = (b < 0 || b >= NZ) ? 0 : b;
+    c = (c < 0 || c >= NZ) ? 0 : c;
+    return (a * NZ + b) * NZ + c;
+}
+
+__global__ __launch_bounds__(256) void ovl3_hist_kernel(const void *__restrict__ sA, const void *__restrict__ sB,
+                                                        const void *__restrict__ sC, int itype, int64_t B, int NZ,
+                                                        int *__restrict__ rank, int *__restrict__ blk_hist,
+                                                        unsigned *__restrict__ err)
+{
+    __shared__ int keys[256];
+    __shared__ int hist[OVL3_MAXNZ * OVL3_MAXNZ * OVL3_MAXNZ];
+    const int NK = NZ * NZ * NZ;
+    for (int q = threadIdx.x; q < NK; q += 256) hist[q] = 0;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int key = i < B ? key3(sA, sB, sC, itype, i, NZ, err) : -1;
+    keys[threadIdx.x] = key;
+    __syncthreads();
+    if (i < B) {
+        int r = 0;
+        for (int j = 0; j < static_cast<int>(threadIdx.x); ++j) r += (keys[j] == key);
+        rank[i] = r;
+        atomicAdd(&hist[key], 1);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NK; q += 256) blk_hist[static_cast<int64_t>(blockIdx.x) * NK + q] = hist[q];
+}
+
+// One block: real-primitive lists, bucket offsets, and the exclusive scan of the
+// per-(species triple, shell triple) record counts.
+template <typename T>
+__global__ __launch_bounds__(256) void ovl3_plan_kernel(const OvlShape sh, const T *__restrict__ coefT,
+                                                        const T *__restrict__ alphaT, double *__restrict__ coef64,
+                                                        double *__restrict__ alpha64, int *__restrict__ nreal,
+                                                        int *__restrict__ lidx, int64_t *__restrict__ grp_off,
+                                                        int nblk_h, const int *__restrict__ blk_hist,
+                                                        int *__restrict__ blk_off, int *__restrict__ bcnt,
+                                                        int *__restrict__ boff, int *__restrict__ toff)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, NK = NZ * NZ * NZ;
+    for (int idx = threadIdx.x; idx < NZ * ns; idx += blockDim.x) {
+        const int sg = idx / ns, s = idx - sg * ns;
+        int nr = 0;
+        for (int k = 0; k < K; ++k) {
+            const int64_t q = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+            const double c = static_cast<double>(coefT[q]);
+            coef64[q] = c;
+            alpha64[q] = static_cast<double>(alphaT[q]);
+            if (coefT[q] != T(0)) lidx[static_cast<int64_t>(idx) * K + nr++] = k;
+        }
+        nreal[idx] = nr;
+    }
+    for (int key = threadIdx.x; key < NK; key += blockDim.x) {
+        int run = 0;
+        for (int b = 0; b < nblk_h; ++b) {
+            blk_off[static_cast<int64_t>(b) * NK + key] = run;
+            run += blk_hist[static_cast<int64_t>(b) * NK + key];
+        }
+        bcnt[key] = run;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int o = 0, t = 0;
+        for (int key = 0; key < NK; ++key) {
+            boff[key] = o;
+            toff[key] = t;
+            o += bcnt[key];
+            t += (bcnt[key] + OVL_TILE - 1) / OVL_TILE;
+        }
+        toff[NK] = t;
+    }
+    __syncthreads();
+    for (int key = threadIdx.x; key < NK; key += blockDim.x)
+        for (int b = 0; b < nblk_h; ++b) blk_off[static_cast<int64_t>(b) * NK + key] += boff[key];
+    // record offsets: contiguous ranges of groups per thread, then a scan of the partials
+    __shared__ int64_t part[256];
+    const int64_t ngrp = static_cast<int64_t>(NK) * ns * ns * ns;
+    const int64_t per = (ngrp + blockDim.x - 1) / blockDim.x;
+    const int64_t g0 = per * threadIdx.x, g1 = min(ngrp, g0 + per);
+    auto count = [&](int64_t g) -> int64_t {
+        const int sa = static_cast<int>(g % ns), sb = static_cast<int>((g / ns) % ns);
+        const int sc = static_cast<int>((g / (static_cast<int64_t>(ns) * ns)) % ns);
+        const int key = static_cast<int>(g / (static_cast<int64_t>(ns) * ns * ns));
+        const int a = key / (NZ * NZ), b = (key / NZ) % NZ, c = key % NZ;
+        return static_cast<int64_t>(nreal[a * ns + sa]) * nreal[b * ns + sb] * nreal[c * ns + sc];
+    };
+    int64_t s = 0;
+    for (int64_t g = g0; g < g1; ++g) s += count(g);
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int q = 0; q < static_cast<int>(blockDim.x); ++q) {
+            const int64_t v = part[q];
+            part[q] = run;
+            run += v;
+        }
+        grp_off[ngrp] = run;
+    }
+    __syncthreads();
+    s = part[threadIdx.x];
+    for (int64_t g = g0; g < g1; ++g) {
+        grp_off[g] = s;
+        s += count(g);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void ovl3_fill_kernel(const OvlShape sh, const void *__restrict__ sA,
+                                                        const void *__restrict__ sB, const void *__restrict__ sC,
+                                                        int itype, int64_t B, int nblk_perm, int nblk_tile,
+                                                        const int *__restrict__ rank, const int *__restrict__ blk_off,
+                                                        const int *__restrict__ toff, int *__restrict__ perm,
+                                                        int *__restrict__ tile_key, int maxtiles,
+                                                        const double *__restrict__ coef64,
+                                                        const double *__restrict__ alpha64,
+                                                        const int *__restrict__ nreal, const int *__restrict__ lidx,
+                                                        const int64_t *__restrict__ grp_off,
+                                                        typename Rec3Of<T>::type *__restrict__ rec)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, NK = NZ * NZ * NZ;
+    const int bid = blockIdx.x;
+    if (bid < nblk_perm) {
+        const int64_t i = static_cast<int64_t>(bid) * 256 + threadIdx.x;
+        if (i >= B) return;
+        const int key = key3(sA, sB, sC, itype, i, NZ, nullptr);
+        perm[blk_off[static_cast<int64_t>(i / 256) * NK + key] + rank[i]] = static_cast<int>(i);
+        return;
+    }
+    if (bid < nblk_perm + nblk_tile) {
+        const int t = (bid - nblk_perm) * 256 + threadIdx.x;
+        if (t >= maxtiles) return;
+        int key = -1;
+        if (t < toff[NK]) {
+            key = 0;
+            while (key + 1 < NK && toff[key + 1] <= t) ++key;
+        }
+        tile_key[t] = key;
+        return;
+    }
+    const int64_t gid = static_cast<int64_t>(bid - nblk_perm - nblk_tile) * 256 + threadIdx.x;
+    const int64_t K3 = static_cast<int64_t>(K) * K * K;
+    const int64_t ngrp = static_cast<int64_t>(NK) * ns * ns * ns;
+    if (gid >= ngrp * K3) return;
+    const int64_t g = gid / K3;
+    const int rem = static_cast<int>(gid - g * K3);
+    const int i = rem / (K * K), j = (rem / K) % K, k = rem % K;
+    const int sa = static_cast<int>(g % ns), sb = static_cast<int>((g / ns) % ns);
+    const int sc = static_cast<int>((g / (static_cast<int64_t>(ns) * ns)) % ns);
+    const int key = static_cast<int>(g / (static_cast<int64_t>(ns) * ns * ns));
+    const int a = key / (NZ * NZ), b = (key / NZ) % NZ, c = key % NZ;
+    const int na = nreal[a * ns + sa], nb = nreal[b * ns + sb], nc = nreal[c * ns + sc];
+    if (i >= na || j >= nb || k >= nc) return;
+    const int ka = lidx[(a * ns + sa) * K + i], kb = lidx[(b * ns + sb) * K + j], kc = lidx[(c * ns + sc) * K + k];
+    const int64_t qa = sa + static_cast<int64_t>(ns) * (ka + static_cast<int64_t>(K) * a);
+    const int64_t qb = sb + static_cast<int64_t>(ns) * (kb + static_cast<int64_t>(K) * b);
+    const int64_t qc = sc + static_cast<int64_t>(ns) * (kc + static_cast<int64_t>(K) * c);
+    const double al = alpha64[qa], be = alpha64[qb], ga = alpha64[qc];
+    const double p = al + be, q = p + ga, iq = 1.0 / q, ip = 1.0 / p;
+    const double l2e = std::is_same<T, float>::value ? 1.4426950408889634074 : 1.0;
+    typename Rec3Of<T>::type r;
+    r.cab = al * be * iq * l2e;
+    r.cac = al * ga * iq * l2e;
+    r.cbc = be * ga * iq * l2e;
+    r.aop = static_cast<T>(al * ip);
+    r.bop = static_cast<T>(be * ip);
+    r.gq = static_cast<T>(ga * iq);
+    r.pq = static_cast<T>(p * iq);
+    r.i2q = static_cast<T>(0.5 * iq);
+    r.Kabc = static_cast<T>(coef64[qa] * coef64[qb] * coef64[qc] * 5.568327996831707845 * iq * sqrt(iq));
+    rec[grp_off[g] + (static_cast<int64_t>(i) * nb + j) * nc + k] = r;
+}
+
+template <int L>
+struct Sh3 {
+    static constexpr int nbf = (L + 1) * (L + 2) / 2;
+};
+
+// 1-D three-centre Obara-Saika table, s[0][0][0] = 1.
+template <typename T, int IA, int IB, int IC>
+__device__ __forceinline__ void os3(T QA, T QB, T QC, T i2q, T (&s)[IA + 1][IB + 1][IC + 1])
+{
+    s[0][0][0] = T(1);
+#pragma unroll
+    for (int i = 1; i <= IA; ++i)
+        s[i][0][0] = QA * s[i - 1][0][0] + (i > 1 ? T(i - 1) * i2q * s[i > 1 ? i - 2 : 0][0][0] : T(0));
+#pragma unroll
+    for (int j = 1; j <= IB; ++j)
+#pragma unroll
+        for (int i = 0; i <= IA; ++i) {
+            T v = QB * s[i][j - 1][0];
+            if (i > 0) v += T(i) * i2q * s[i > 0 ? i - 1 : 0][j - 1][0];
+            if (j > 1) v += T(j - 1) * i2q * s[i][j > 1 ? j - 2 : 0][0];
+            s[i][j][0] = v;
+        }
+#pragma unroll
+    for (int k = 1; k <= IC; ++k)
+#pragma unroll
+        for (int j = 0; j <= IB; ++j)
+#pragma unroll
+            for (int i = 0; i <= IA; ++i) {
+                T v = QC * s[i][j][k - 1];
+                if (i > 0) v += T(i) * i2q * s[i > 0 ? i - 1 : 0][j][k - 1];
+                if (j > 0) v += T(j) * i2q * s[i][j > 0 ? j - 1 : 0][k - 1];
+                if (k > 1) v += T(k - 1) * i2q * s[i][j][k > 1 ? k - 2 : 0];
+                s[i][j][k] = v;
+            }
+}
+
+__device__ __forceinline__ float trip_exp(const TripRecF &r, double rab, double rac, double rbc)
+{
+    const double y = r.cab * rab + r.cac * rac + r.cbc * rbc;
+    const double n = rint(y);
+    return ldexpf(__builtin_amdgcn_exp2f(-static_cast<float>(y - n)), -static_cast<int>(n));
+}
+__device__ __forceinline__ double trip_exp(const TripRecD &r, double rab, double rac, double rbc)
+{
+    return exp(-(r.cab * rab + r.cac * rac + r.cbc * rbc));
+}
+
+struct Geo3 {
+    double rab, rac, rbc;
+};
+
+template <typename T, int LA, int LB, int LC>
+__device__ __forceinline__ void triple_fwd(const typename Rec3Of<T>::type *__restrict__ recs, int na, int nb, int nc,
+                                           const Geo3 &geo, const T (&ab)[3], const T (&ac)[3], T *__restrict__ out,
+                                           int N, int offa, int offb, int offc, bool active)
+{
+    constexpr int NBA = Sh3<LA>::nbf, NBB = Sh3<LB>::nbf, NBC = Sh3<LC>::nbf;
+    constexpr int N1 = 2 * LA + 1, N2 = 2 * LB + 1;
+    constexpr int NLIN = (NBA - 1) + N1 * (NBB - 1) + N1 * N2 * (NBC - 1) + 1;
+    T acc[NLIN];
+#pragma unroll
+    for (int q = 0; q < NLIN; ++q) acc[q] = T(0);
+    const int ntrip = na * nb * nc;
+    for (int t = 0; t < ntrip; ++t) {
+        const typename Rec3Of<T>::type r = recs[t];
+        const T G = r.Kabc * trip_exp(r, geo.rab, geo.rac, geo.rbc);
+        if constexpr (LA + LB + LC == 0) {
+            acc[0] += G;
+        } else {
+            T sx[LA + 1][LB + 1][LC + 1], sy[LA + 1][LB + 1][LC + 1], sz[LA + 1][LB + 1][LC + 1];
+            {
+                const T PA = -r.bop * ab[0], PB = r.aop * ab[0], PC = PA + ac[0];
+                os3<T, LA, LB, LC>(PA - r.gq * PC, PB - r.gq * PC, r.pq * PC, r.i2q, sx);
+            }
+            {
+                const T PA = -r.bop * ab[1], PB = r.aop * ab[1], PC = PA + ac[1];
+                os3<T, LA, LB, LC>(PA - r.gq * PC, PB - r.gq * PC, r.pq * PC, r.i2q, sy);
+            }
+            {
+                const T PA = -r.bop * ab[2], PB = r.aop * ab[2], PC = PA + ac[2];
+                os3<T, LA, LB, LC>(PA - r.gq * PC, PB - r.gq * PC, r.pq * PC, r.i2q, sz);
+            }
+            int index1 = 0;
+#pragma unroll
+            for (int ll1 = LA; ll1 >= 0; --ll1)
+#pragma unroll
+                for (int n1 = 0; n1 <= LA - ll1; ++n1) {
+                    const int m1 = LA - ll1 - n1;
+                    int index2 = 0;
+#pragma unroll
+                    for (int ll2 = LB; ll2 >= 0; --ll2)
+#pragma unroll
+                        for (int n2 = 0; n2 <= LB - ll2; ++n2) {
+                            const int m2 = LB - ll2 - n2;
+                            int index3 = 0;
+#pragma unroll
+                            for (int ll3 = LC; ll3 >= 0; --ll3)
+#pragma unroll
+                                for (int n3 = 0; n3 <= LC - ll3; ++n3) {
+                                    const int m3 = LC - ll3 - n3;
+                                    acc[index1 + N1 * index2 + N1 * N2 * index3] +=
+                                        G * sx[ll1][ll2][ll3] * sy[m1][m2][m3] * sz[n1][n2][n3];
+                                    ++index3;
+                                }
+                            ++index2;
+                        }
+                    ++index1;
+                }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < NBC; ++k)
+#pragma unroll
+        for (int j = 0; j < NBB; ++j)
+#pragma unroll
+            for (int i = 0; i < NBA; ++i) {
+                const int q = i + NBA * (j + NBB * k);
+                out[(offa + i) + static_cast<int64_t>(N) * ((offb + j) + static_cast<int64_t>(N) * (offc + k))] =
+                    q < NLIN ? acc[q < NLIN ? q : 0] : T(0);
+            }
+}
+
+#define AOK_L3_CASES(X)                                                                                         \
+    X(0, 0, 0) X(0, 0, 1) X(0, 0, 2) X(0, 1, 0) X(0, 1, 1) X(0, 1, 2) X(0, 2, 0) X(0, 2, 1) X(0, 2, 2) X(1, 0, 0) \
+        X(1, 0, 1) X(1, 0, 2) X(1, 1, 0) X(1, 1, 1) X(1, 1, 2) X(1, 2, 0) X(1, 2, 1) X(1, 2, 2) X(2, 0, 0)       \
+            X(2, 0, 1) X(2, 0, 2) X(2, 1, 0) X(2, 1, 1) X(2, 1, 2) X(2, 2, 0) X(2, 2, 1) X(2, 2, 2)
+
+template <typename T, int LMAXB>
+__global__ __launch_bounds__(64) void ovl3c_fwd_fast(const OvlShape sh, const T *__restrict__ posA,
+                                                     const T *__restrict__ posB, const T *__restrict__ posC, T scale,
+                                                     const int *__restrict__ perm, const int *__restrict__ tile_key,
+                                                     const int *__restrict__ boff, const int *__restrict__ bcnt,
+                                                     const int *__restrict__ toff,
+                                                     const typename Rec3Of<T>::type *__restrict__ rec,
+                                                     const int64_t *__restrict__ grp_off,
+                                                     const int *__restrict__ nreal, T *__restrict__ out)
+{
+    const int ns = sh.ns, NZ = sh.NZ, N = sh.N, NK = NZ * NZ * NZ;
+    const int t = blockIdx.x;
+    if (t >= toff[NK]) return;
+    const int key = tile_key[t];
+    const int first = boff[key] + (t - toff[key]) * OVL_TILE;
+    const int nact = min(OVL_TILE, boff[key] + bcnt[key] - first);
+    const int lane = threadIdx.x;
+    const bool active = lane < nact;
+    const int b = perm[first + (active ? lane : 0)];
+    const int sgA = key / (NZ * NZ), sgB = (key / NZ) % NZ, sgC = key % NZ;
+    T A[3], Bv[3], C[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        A[ax] = scale * posA[3 * b + ax];
+        Bv[ax] = scale * posB[3 * b + ax];
+        C[ax] = scale * posC[3 * b + ax];
+    }
+    Geo3 geo{0.0, 0.0, 0.0};
+    T ab[3], ac[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const double dab = static_cast<double>(A[ax]) - static_cast<double>(Bv[ax]);
+        const double dac = static_cast<double>(A[ax]) - static_cast<double>(C[ax]);
+        const double dbc = static_cast<double>(Bv[ax]) - static_cast<double>(C[ax]);
+        geo.rab += dab * dab;
+        geo.rac += dac * dac;
+        geo.rbc += dbc * dbc;
+        ab[ax] = static_cast<T>(dab);
+        ac[ax] = static_cast<T>(dac);
+    }
+    T *ob = out + static_cast<int64_t>(N) * N * N * b;
+    for (int sc = 0; sc < ns; ++sc) {
+        const int lc = sh.ls[sc], nc = nreal[sgC * ns + sc];
+        for (int sb = 0; sb < ns; ++sb) {
+            const int lb = sh.ls[sb], nb = nreal[sgB * ns + sb];
+            for (int sa = 0; sa < ns; ++sa) {
+                const int la = sh.ls[sa], na = nreal[sgA * ns + sa];
+                const int64_t g = ((static_cast<int64_t>(key) * ns + sc) * ns + sb) * ns + sa;
+                const typename Rec3Of<T>::type *recs = rec + grp_off[g];
+#define AOK_F3_CASE(A_, B_, C_)                                                                                 \
+    if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB)                                                    \
+        if (la == A_ && lb == B_ && lc == C_) {                                                                 \
+            triple_fwd<T, A_, B_, C_>(recs, na, nb, nc, geo, ab, ac, ob, N, sh.off[sa], sh.off[sb], sh.off[sc], \
+                                      active);                                                                  \
+            continue;                                                                                           \
+        }
+                AOK_L3_CASES(AOK_F3_CASE)
+#undef AOK_F3_CASE
+            }
+        }
+    }
+}
+
+// ---- plain path: one item per (b, s_a, s_b, s_c) --------------------------------------------
+template <typename T, int LMAXP>
+__global__ __launch_bounds__(128) void ovl3c_fwd_plain(const OvlShape sh, int64_t B, const T *__restrict__ coef,
+                                                       const T *__restrict__ alpha, const void *__restrict__ sA,
+                                                       const void *__restrict__ sB, const void *__restrict__ sC,
+                                                       int itype, const T *__restrict__ posA,
+                                                       const T *__restrict__ posB, const T *__restrict__ posC,
+                                                       T scale, T *__restrict__ out, unsigned *__restrict__ err)
+{
+    constexpr int NBF = (LMAXP + 1) * (LMAXP + 2) / 2;
+    constexpr int TT = 3 * LMAXP + 4;
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    const int64_t item = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nsh3 = static_cast<int64_t>(ns) * ns * ns;
+    if (item >= B * nsh3) return;
+    const int64_t b = item % B;
+    const int64_t s3 = item / B;
+    const int sa = static_cast<int>(s3 % ns), sb = static_cast<int>((s3 / ns) % ns), sc = static_cast<int>(s3 / (ns * ns));
+    int ga = load_sigma(sA, itype, b), gb = load_sigma(sB, itype, b), gc = load_sigma(sC, itype, b);
+    if (ga < 0 || ga >= NZ || gb < 0 || gb >= NZ || gc < 0 || gc >= NZ) {
+        atomicOr(err, ERR_SPECIES);
+        return;
+    }
+    const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
+    const int na = nbf_of(la), nb = nbf_of(lb), nc = nbf_of(lc);
+    const int N1 = 2 * la + 1, N2 = 2 * lb + 1;
+    T E3[TT][LMAXP + 1][LMAXP + 1][LMAXP + 1][3];
+    T blk[NBF * NBF * NBF];
+    for (int q = 0; q < na * nb * nc; ++q) blk[q] = T(0);
+    T A[3], Bv[3], C[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        A[ax] = scale * posA[3 * b + ax];
+        Bv[ax] = scale * posB[3 * b + ax];
+        C[ax] = scale * posC[3 * b + ax];
+    }
+    const T pi32 = T(5.568327996831707845);
+    for (int ip = 0; ip < K; ++ip)
+        for (int jp = 0; jp < K; ++jp)
+            for (int kp = 0; kp < K; ++kp) {
+                const int64_t qa = sa + static_cast<int64_t>(ns) * (ip + static_cast<int64_t>(K) * ga);
+                const int64_t qb = sb + static_cast<int64_t>(ns) * (jp + static_cast<int64_t>(K) * gb);
+                const int64_t qc = sc + static_cast<int64_t>(ns) * (kp + static_cast<int64_t>(K) * gc);
+                const T ca = coef[qa], cb = coef[qb], cc = coef[qc];
+                if (ca == T(0) || cb == T(0) || cc == T(0)) continue;
+                const T aa = alpha[qa], ab = alpha[qb], ac = alpha[qc];
+                const T p = aa + ab, q = p + ac, mu = aa * ab / p, nu = p * ac / q, oo2q = T(0.5) / q;
+                const T prefac = ca * cb * cc * pi32 / (q * sqrt(q));
+                for (int ax = 0; ax < 3; ++ax) {
+                    const T P = (aa * A[ax] + ab * Bv[ax]) / p;
+                    const T Q = (p * P + ac * C[ax]) / q;
+                    const T AB = A[ax] - Bv[ax], PC = P - C[ax];
+                    const T QA = Q - A[ax], QB = Q - Bv[ax], QC = Q - C[ax];
+                    for (int k = 0; k <= lc; ++k)
+                        for (int j = 0; j <= lb; ++j)
+                            for (int i = 0; i <= la; ++i)
+                                for (int t = 0; t < TT; ++t) E3[t][i][j][k][ax] = T(0);
+                    E3[0][0][0][0][ax] = exp(-mu * AB * AB - nu * PC * PC);
+                    for (int k = 0; k <= lc; ++k)
+                        for (int j = 0; j <= lb; ++j)
+                            for (int i = 0; i <= la; ++i) {
+                                if (i == 0 && j == 0 && k == 0) continue;
+                                int pi_ = i, pj = j, pk = k;
+                                T X;
+                                if (i > 0) { pi_ = i - 1; X = QA; }
+                                else if (j > 0) { pj = j - 1; X = QB; }
+                                else { pk = k - 1; X = QC; }
+                                for (int t = 0; t <= i + j + k; ++t) {
+                                    const T em = t > 0 ? E3[t - 1][pi_][pj][pk][ax] : T(0);
+                                    E3[t][i][j][k][ax] = X * E3[t][pi_][pj][pk][ax] +
+                                                         T(t + 1) * E3[t + 1][pi_][pj][pk][ax] + oo2q * em;
+                                }
+                            }
+                }
+                int index1 = 0;
+                for (int ll1 = la; ll1 >= 0; --ll1)
+                    for (int n1 = 0; n1 <= la - ll1; ++n1) {
+                        const int m1 = la - ll1 - n1;
+                        int index2 = 0;
+                        for (int ll2 = lb; ll2 >= 0; --ll2)
+                            for (int n2 = 0; n2 <= lb - ll2; ++n2) {
+                                const int m2 = lb - ll2 - n2;
+                                int index3 = 0;
+                                for (int ll3 = lc; ll3 >= 0; --ll3)
+                                    for (int n3 = 0; n3 <= lc - ll3; ++n3) {
+                                        const int m3 = lc - ll3 - n3;
+                                        blk[index1 + N1 * index2 + N1 * N2 * index3] +=
+                                            E3[0][ll1][ll2][ll3][0] * E3[0][m1][m2][m3][1] * E3[0][n1][n2][n3][2] *
+                                            prefac;
+                                        ++index3;
+                                    }
+                                ++index2;
+                            }
+                        ++index1;
+                    }
+            }
+    T *ob = out + static_cast<int64_t>(N) * N * N * b;
+    for (int k = 0; k < nc; ++k)
+        for (int j = 0; j < nb; ++j)
+            for (int i = 0; i < na; ++i)
+                ob[(sh.off[sa] + i) + static_cast<int64_t>(N) * ((sh.off[sb] + j) + static_cast<int64_t>(N) * (sh.off[sc] + k))] =
+                    blk[i + na * (j + nb * k)];
+}
+
+// ---------------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------------
+template <typename T, int LMAXB>
+static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &sh)
+{
+    using R = typename Rec3Of<T>::type;
+    const Ovl3Ws w = ovl3_ws_map(c.B, sh.ns, sh.K, sh.NZ, sizeof(R));
+    aok_status st = ensure_ws(ctx, w.total);
+    if (st != AOK_OK) return st;
+    char *ws = static_cast<char *>(ctx->ws);
+    hipLaunchKernelGGL(ovl3_hist_kernel, dim3(w.nblk_h), dim3(256), 0, ctx->stream, c.sidxA, c.sidxB, c.sidxC,
+                       c.itype, c.B, sh.NZ, reinterpret_cast<int *>(ws + w.rank),
+                       reinterpret_cast<int *>(ws + w.blk_hist), ctx->d_err);
+    AOK_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL((ovl3_plan_kernel<T>), dim3(1), dim3(256), 0, ctx->stream, sh,
+                       static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha),
+                       reinterpret_cast<double *>(ws + w.coef64), reinterpret_cast<double *>(ws + w.alpha64),
+                       reinterpret_cast<int *>(ws + w.nreal), reinterpret_cast<int *>(ws + w.lidx),
+                       reinterpret_cast<int64_t *>(ws + w.grp_off), w.nblk_h,
+                       reinterpret_cast<const int *>(ws + w.blk_hist), reinterpret_cast<int *>(ws + w.blk_off),
+                       reinterpret_cast<int *>(ws + w.bcnt), reinterpret_cast<int *>(ws + w.boff),
+                       reinterpret_cast<int *>(ws + w.toff));
+    AOK_HIP(ctx, hipGetLastError());
+    const int nblk_tile = (w.maxtiles + 255) / 256;
+    const int nblk_rec = static_cast<int>((w.maxrec + 255) / 256);
+    hipLaunchKernelGGL((ovl3_fill_kernel<T>), dim3(w.nblk_h + nblk_tile + nblk_rec), dim3(256), 0, ctx->stream, sh,
+                       c.sidxA, c.sidxB, c.sidxC, c.itype, c.B, w.nblk_h, nblk_tile,
+                       reinterpret_cast<const int *>(ws + w.rank), reinterpret_cast<const int *>(ws + w.blk_off),
+                       reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<int *>(ws + w.perm),
+                       reinterpret_cast<int *>(ws + w.tile_key), w.maxtiles,
+                       reinterpret_cast<const double *>(ws + w.coef64), reinterpret_cast<const double *>(ws + w.alpha64),
+                       reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.lidx),
+                       reinterpret_cast<const int64_t *>(ws + w.grp_off), reinterpret_cast<R *>(ws + w.rec));
+    AOK_HIP(ctx, hipGetLastError());
+    time_begin(ctx, "ovl3c_fwd_fast");
+    hipLaunchKernelGGL((ovl3c_fwd_fast<T, LMAXB>), dim3(w.maxtiles), dim3(OVL_TILE), 0, ctx->stream, sh,
+                       static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<const T *>(c.posC),
+                       static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
+                       reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
+                       reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
+                       reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int64_t *>(ws + w.grp_off),
+                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out));
+    time_end(ctx);
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T, int LMAXP>
+static aok_status plain3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &sh)
+{
+    const int64_t items = c.B * sh.ns * sh.ns * sh.ns;
+    if (items == 0) return AOK_OK;
+    time_begin(ctx, "ovl3c_fwd_plain");
+    hipLaunchKernelGGL((ovl3c_fwd_plain<T, LMAXP>), dim3(static_cast<unsigned>((items + 127) / 128)), dim3(128), 0,
+                       ctx->stream, sh, c.B, static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha), c.sidxA,
+                       c.sidxB, c.sidxC, c.itype, static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
+                       static_cast<const T *>(c.posC), static_cast<T>(c.scale), static_cast<T *>(c.out), ctx->d_err);
+    time_end(ctx);
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T>
+aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c)
+{
+    OvlShape sh;
+    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, 1 << 20, sh);
+    if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
+    if (c.B < 0 || c.B > (int64_t(1) << 28)) return fail(ctx, AOK_ERR_ARG, "B out of range");
+    int lmax = 0;
+    for (int q = 0; q < sh.ns; ++q) lmax = std::max<int>(lmax, sh.ls[q]);
+    const bool fast = !(c.flags & AOK_PLAIN) && lmax <= OVL3_FAST_LMAX && sh.NZ <= OVL3_MAXNZ && sh.ns <= 24;
+    if (!fast) {
+        switch (lmax) {
+            case 0: return plain3_launch<T, 0>(ctx, c, sh);
+            case 1: return plain3_launch<T, 1>(ctx, c, sh);
+            case 2: return plain3_launch<T, 2>(ctx, c, sh);
+            case 3: return plain3_launch<T, 3>(ctx, c, sh);
+            default: return plain3_launch<T, 4>(ctx, c, sh);
+        }
+    }
+    if (c.B == 0) return AOK_OK;
+    switch (lmax) {
+        case 0: return fast3_launch<T, 0>(ctx, c, sh);
+        case 1: return fast3_launch<T, 1>(ctx, c, sh);
+        default: return fast3_launch<T, 2>(ctx, c, sh);
+    }
+}
+
+template aok_status ovl3c_run<float>(aok_ctx, const Ovl3cCall &);
+template aok_status ovl3c_run<double>(aok_ctx, const Ovl3cCall &);
+
+}  // namespace aok

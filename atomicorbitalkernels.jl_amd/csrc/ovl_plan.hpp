@@ -176,8 +176,23 @@ struct Ovl2cCall {
     bool vjp;
 };
 
+// One 3-centre call.
+struct Ovl3cCall {
+    unsigned flags;
+    const void *posA, *posB, *posC, *sidxA, *sidxB, *sidxC;
+    int itype;
+    int64_t B;
+    const int64_t *ls;
+    int64_t ns, K, NZ;
+    const void *coef, *alpha;
+    double scale;
+    void *out;
+};
+
 template <typename T>
 aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback);
+template <typename T>
+aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c);
 template <typename T>
 aok_status compile_coef_run(aok_ctx ctx, const void *zeta, const void *D, int64_t ns, int64_t K, int64_t NZ,
                             const int64_t *ls, void *coef);

@@ -1,0 +1,138 @@
+# AOKHip.jl -- drop-in `ccall` shim that routes AtomicOrbitalKernels.jl's kernel-facing
+# layer to libaok.so (include/aok.h).  Loaded as a package extension when AMDGPU.jl is
+# present; the public API (`evaluate`, `evaluate_ed`, `compile_basis`, `batch_overlap[!]`,
+# `batch_overlap_3c[!]`, `pullback_ps`, both `rrule`s, the Lux ps/st layout) is unchanged:
+# only the L3 methods that launch KernelAbstractions kernels get ROCArray methods here.
+#
+# This file cannot be executed in the build container (no Julia); its contract is
+# exercised through the same C ABI by the Python tests.
+module AOKHip
+
+using AMDGPU: ROCArray, ROCVector, ROCMatrix, stream
+using StaticArrays: SVector
+import AtomicOrbitalKernels as AOK
+import AtomicOrbitalKernels: evaluate, evaluate_ed, batch_overlap!, pullback_ps
+
+const libaok = joinpath(@__DIR__, "..", "lib", "libaok.so")
+
+const AOK_OK = Cint(0)
+const AOK_F32 = Cint(0); const AOK_F64 = Cint(1)
+const AOK_I64 = Cint(0); const AOK_I32 = Cint(1)
+const AOK_FAST = Cuint(0)
+
+_dt(::Type{Float32}) = AOK_F32
+_dt(::Type{Float64}) = AOK_F64
+
+mutable struct Ctx
+    h::Ptr{Cvoid}
+end
+const _CTX = Ref{Union{Nothing, Ctx}}(nothing)
+
+function ctx()
+    if _CTX[] === nothing
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        # borrow AMDGPU's task-local stream so ordering with Julia-side work is preserved
+        st = Ptr{Cvoid}(stream().stream)
+        rc = ccall((:aok_create, libaok), Cint, (Ref{Ptr{Cvoid}}, Cint, Ptr{Cvoid}), h, 0, st)
+        rc == AOK_OK || error("aok_create failed ($rc)")
+        _CTX[] = Ctx(h[])
+    end
+    return _CTX[]::Ctx
+end
+
+# status -> the reference's exception types (errors are raised before any launch there)
+function check(rc, what)
+    rc == AOK_OK && return nothing
+    msg = unsafe_string(ccall((:aok_last_error, libaok), Cstring, (Ptr{Cvoid},), ctx().h))
+    rc == 2 && error("$what: $msg")                  # ErrorException (unknown species)
+    rc == 1 && throw(ArgumentError("$what: $msg"))
+    error("$what: $msg (status $rc)")
+end
+
+_ip(v::AbstractVector{<:Integer}) = Vector{Int64}(Array(v))   # host int64 (structural arrays)
+
+# ---- H1: evaluate / evaluate_ed (atomicorbitals.jl:148-162, 192-210) -------------------
+function _ao_eval(basis, Rs::ROCVector{SVector{3, T}}, sidx::ROCVector{<:Integer}, ps, st, grad) where {T}
+    nX = length(Rs); nOrb = length(basis)
+    ζ = ps.Rnl.ζ; D = ps.Rnl.D
+    nRad, K, NZ = size(ζ)
+    P = similar(Rs, T, nX, nOrb)
+    dP = grad ? similar(Rs, SVector{3, T}, nX, nOrb) : nothing
+    it = eltype(sidx) == Int32 ? AOK_I32 : AOK_I64
+    poly = _ip(st.Rnl.poly); iR = _ip(st.iR); iY = _ip(st.iY)
+    L = length(st.Ylm.Flm) > 0 ? AOK._ylm_degree(basis.Ylm) : 0
+    decay = basis.Rnl isa AOK.GaussianTypeRadials ? Cint(0) : Cint(1)
+    rc = ccall((:aok_ao_eval, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64,
+                Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64, Cint, Cdouble, Cint, Ptr{Cvoid}, Ptr{Cvoid}),
+               ctx().h, _dt(T), pointer(Rs), pointer(sidx), it, nX, pointer(ζ), pointer(D), nRad, K, NZ,
+               poly, iR, iY, nOrb, L, Float64(basis.lengthscale), decay, pointer(P),
+               grad ? pointer(dP) : C_NULL)
+    check(rc, grad ? "evaluate_ed" : "evaluate")
+    return grad ? (P, dP) : P
+end
+
+evaluate(basis::AOK.AtomicOrbitals, Rs::ROCVector{<:SVector{3}}, sidx::ROCVector{<:Integer}, ps, st) =
+    _ao_eval(basis, Rs, sidx, ps, st, false)
+evaluate_ed(basis::AOK.AtomicOrbitals, Rs::ROCVector{<:SVector{3}}, sidx::ROCVector{<:Integer}, ps, st) =
+    _ao_eval(basis, Rs, sidx, ps, st, true)
+
+# ---- H1 pullback: pullback_ps + _aorb_pullback_x (atomicorbitals.jl:254-299) ----------
+function pullback_ps(∂P::ROCMatrix{T}, basis::AOK.AtomicOrbitals, Rs::ROCVector{SVector{3, T}},
+                     sidx::ROCVector{<:Integer}, ps::NamedTuple, st) where {T}
+    ζ = ps.Rnl.ζ; D = ps.Rnl.D
+    nRad, K, NZ = size(ζ)
+    ∂ζ = similar(ζ); ∂D = similar(D)
+    rc = ccall((:aok_ao_pullback, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64,
+                Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64, Cint, Cdouble, Cint, Ptr{Cvoid}, Ptr{Cvoid},
+                Ptr{Cvoid}, Ptr{Cvoid}),
+               ctx().h, _dt(T), pointer(Rs), pointer(sidx), eltype(sidx) == Int32 ? AOK_I32 : AOK_I64,
+               length(Rs), pointer(ζ), pointer(D), nRad, K, NZ, _ip(st.Rnl.poly), _ip(st.iR), _ip(st.iY),
+               length(basis), AOK._ylm_degree(basis.Ylm), Float64(basis.lengthscale),
+               basis.Rnl isa AOK.GaussianTypeRadials ? Cint(0) : Cint(1), pointer(∂P), C_NULL,
+               pointer(∂ζ), pointer(∂D))
+    check(rc, "pullback_ps")
+    return (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple())
+end
+
+# ---- H2: batch_overlap!(out, ::CartesianGTOBasis, XA, XB) (overlap_2c.jl:171-197) ----------
+function batch_overlap!(out::ROCArray{FT, 3}, basis::AOK.CartesianGTOBasis, XA::AbstractVector,
+                        XB::AbstractVector; kwargs...) where {FT}
+    B = length(XA)
+    length(XB) == B || error("XA and XB must have equal length; got $(length(XA)) and $(length(XB))")
+    posA = ROCArray(AOK._cgto_posmat(XA, FT)); posB = ROCArray(AOK._cgto_posmat(XB, FT))
+    sA = ROCArray(AOK._cgto_sidx(basis, XA)); sB = ROCArray(AOK._cgto_sidx(basis, XB))
+    coef = ROCArray(FT.(basis.coef)); α = ROCArray(FT.(basis.ζ))
+    ns, K, NZ = size(basis.coef)
+    rc = ccall((:aok_ovl2c, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Int64, Int64,
+                Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(posA), pointer(posB), pointer(sA), pointer(sB), AOK_I64, B, ns,
+               K, NZ, _ip(basis.ls), pointer(coef), pointer(α), Float64(basis.lengthscale), pointer(out))
+    check(rc, "batch_overlap!")
+    return out
+end
+
+# ---- H2 pullback: pullback_ps(∂S, orb, XA, XB, ps, st) (overlap_2c_grad.jl:296-304) ------
+function pullback_ps(∂S::ROCArray{FT, 3}, orb::AOK.GaussianTypeOrbitals, XA::AbstractVector,
+                     XB::AbstractVector, ps, st) where {FT}
+    str = AOK._compile_struct(orb)
+    B = length(XA)
+    posA = ROCArray(AOK._cgto_posmat(XA, FT)); posB = ROCArray(AOK._cgto_posmat(XB, FT))
+    zl = orb.Rnl.zlist
+    sA = ROCArray([AOK._z2i(orb.Rnl, x.S) for x in XA]); sB = ROCArray([AOK._z2i(orb.Rnl, x.S) for x in XB])
+    ζ = ps.Rnl.ζ; D = ps.Rnl.D
+    ns, K, NZ = size(ζ)
+    ∂ζ = similar(ζ); ∂D = similar(D)
+    rc = ccall((:aok_ovl2c_pullback_ps, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64,
+                Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(∂S), pointer(posA), pointer(posB), pointer(sA), pointer(sB),
+               AOK_I64, B, ns, K, NZ, _ip(str.ls), pointer(ζ), pointer(D), Float64(orb.lengthscale),
+               pointer(∂ζ), pointer(∂D))
+    check(rc, "pullback_ps")
+    return (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple())
+end
+
+end # module
