@@ -82,6 +82,8 @@ _SIGS = {
     "aok_create": [ctypes.POINTER(_vp), _int, _vp],
     "aok_destroy": [_vp],
     "aok_set_timing": [_vp, _int],
+    "aok_tune": [_vp, ctypes.c_char_p, _i64],
+    "aok_tune_get": [_vp, ctypes.c_char_p, ctypes.POINTER(_i64)],
     "aok_ao_eval": [_vp, _int, _vp, _vp, _int, _i64, _vp, _vp, _i64, _i64, _i64, _i64p, _i64p, _i64p, _i64,
                     _int, _dbl, _int, _vp, _vp],
     "aok_ao_pullback": [_vp, _int, _vp, _vp, _int, _i64, _vp, _vp, _i64, _i64, _i64, _i64p, _i64p, _i64p, _i64,
@@ -179,6 +181,15 @@ class Context:
 
     def set_timing(self, on: bool):
         self.lib.aok_set_timing(self.handle, 1 if on else 0)
+
+    def tune(self, key: str, value: int):
+        """Set a launch-shape knob (see aok_tune in include/aok.h)."""
+        self.call("aok_tune", key.encode(), int(value))
+
+    def tune_get(self, key: str) -> int:
+        v = _i64(0)
+        self.call("aok_tune_get", key.encode(), ctypes.byref(v))
+        return int(v.value)
 
     def last_kernel_ms(self) -> float:
         return float(self.lib.aok_last_kernel_ms(self.handle))

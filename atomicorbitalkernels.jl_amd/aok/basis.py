@@ -47,6 +47,12 @@ def fortran_zeros(shape, dtype, device):
     return t.permute(*reversed(range(len(shape))))
 
 
+def fortran_empty(shape, dtype, device):
+    """Uninitialised column-major tensor (for outputs the kernels fill completely)."""
+    t = torch.empty(tuple(reversed(shape)), dtype=dtype, device=device)
+    return t.permute(*reversed(range(len(shape))))
+
+
 def as_fortran(t: torch.Tensor) -> torch.Tensor:
     """Column-major copy of ``t`` unless it already is column-major contiguous."""
     if t.dim() <= 1:

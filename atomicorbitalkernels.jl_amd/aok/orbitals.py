@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import torch
 
 from . import _lib
-from .basis import AtomicOrbitals, _dtype_code, as_fortran, fortran_zeros
+from .basis import AtomicOrbitals, _dtype_code, as_fortran, fortran_empty
 
 
 class NoTangent:
@@ -114,7 +114,7 @@ def evaluate(basis: AtomicOrbitals, X, ps=None, st=None, *, sidx=None, grad=Fals
         raise _lib.AOKArgumentError("positions must be a device (HIP) tensor; libaok has no CPU path")
     ctx = ctx or _lib.context(Rs.device.index or 0)
     nX = Rs.shape[0]
-    P = fortran_zeros((nX, nOrb), dt, Rs.device)
+    P = fortran_empty((nX, nOrb), dt, Rs.device)
     dP = torch.empty((nOrb, nX, 3), dtype=dt, device=Rs.device).permute(1, 0, 2) if grad else None
     sp, it = _sidx_args(sidx)
     ctx.call("aok_ao_eval", _dtype_code(dt), Rs.data_ptr(), sp, it, nX, zeta.data_ptr(), D.data_ptr(), nRad, K, NZ,
@@ -140,8 +140,8 @@ def _pullback(dP, basis, X, ps, st, sidx, want_x, want_ps, ctx=None):
     dP = as_fortran(dP.to(dt))
     ctx = ctx or _lib.context(Rs.device.index or 0)
     dX = torch.empty((nX, 3), dtype=dt, device=Rs.device) if want_x else None
-    dz = fortran_zeros((nRad, K, NZ), dt, Rs.device) if want_ps else None
-    dD = fortran_zeros((nRad, K, NZ), dt, Rs.device) if want_ps else None
+    dz = fortran_empty((nRad, K, NZ), dt, Rs.device) if want_ps else None
+    dD = fortran_empty((nRad, K, NZ), dt, Rs.device) if want_ps else None
     sp, it = _sidx_args(sidx)
     ctx.call("aok_ao_pullback", _dtype_code(dt), Rs.data_ptr(), sp, it, nX, zeta.data_ptr(), D.data_ptr(), nRad, K,
              NZ, poly, iR, iY, nOrb, L, float(basis.lengthscale), basis.Rnl.decay, dP.data_ptr(),

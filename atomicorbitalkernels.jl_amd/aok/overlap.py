@@ -17,7 +17,7 @@ import torch
 
 from . import _lib
 from .basis import (AtomicOrbitals, CartesianGTOBasis, _dtype_code, as_fortran, compile_basis, compile_struct,
-                    fortran_zeros)
+                    fortran_empty)
 from .orbitals import NO_TANGENT, PStates
 
 
@@ -142,8 +142,8 @@ def overlap_pb_coef_alpha(dS, basis: CartesianGTOBasis, XA, XB, *, mode="fast", 
     coef = as_fortran(basis.coef.to(dt))
     zeta = as_fortran(basis.zeta.to(dt))
     ns, K, NZ = zeta.shape
-    dc = fortran_zeros(zeta.shape, dt, dS.device)
-    da = fortran_zeros(zeta.shape, dt, dS.device)
+    dc = fortran_empty(zeta.shape, dt, dS.device)
+    da = fortran_empty(zeta.shape, dt, dS.device)
     ctx = ctx or _lib.context(dS.device.index or 0)
     a, it = _sidx_ptr(sA)
     b, _ = _sidx_ptr(sB)
@@ -166,8 +166,8 @@ def pullback_ps(dS, orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", ct
         raise _lib.AOKArgumentError(f"dS must be {(off[-1], off[-1], B)}")
     pA, pB = pA.to(dt), pB.to(dt)
     ns, K, NZ = zeta.shape
-    dz = fortran_zeros(zeta.shape, dt, dS.device)
-    dD = fortran_zeros(zeta.shape, dt, dS.device)
+    dz = fortran_empty(zeta.shape, dt, dS.device)
+    dD = fortran_empty(zeta.shape, dt, dS.device)
     ctx = ctx or _lib.context(dS.device.index or 0)
     a, it = _sidx_ptr(sA)
     b, _ = _sidx_ptr(sB)
@@ -186,3 +186,40 @@ def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast"
         return NO_TANGENT, NO_TANGENT, NO_TANGENT, NO_TANGENT, dps, NO_TANGENT
 
     return S, pullback
+
+
+def batch_overlap_3c_(out, basis: CartesianGTOBasis, XA, XB, XC, *, mode="fast", ctx=None):
+    """batch_overlap_3c!(out, basis, posA, posB, posC) (src/integrals/kernels_3c.jl:183-200),
+    species-aware: XA/XB/XC are PStates or (positions, sidx) tuples, positions in the
+    basis' length unit.  ``out`` is (N, N, N, B), column-major."""
+    pA, sA = _unpack(XA, basis.zlist, "XA")
+    pB, sB = _unpack(XB, basis.zlist, "XB")
+    pC, sC = _unpack(XC, basis.zlist, "XC")
+    B = pA.shape[0]
+    if pB.shape[0] != B or pC.shape[0] != B:
+        raise _lib.AOKArgumentError("XA, XB and XC must have equal length")
+    N = basis.nbf_total
+    if tuple(out.shape) != (N, N, N, B) or not out.permute(3, 2, 1, 0).is_contiguous():
+        raise _lib.AOKArgumentError(f"out must be a column-major {(N, N, N, B)} tensor")
+    dt = out.dtype
+    pA, pB, pC = pA.to(dt), pB.to(dt), pC.to(dt)
+    coef = as_fortran(basis.coef.to(dt))
+    zeta = as_fortran(basis.zeta.to(dt))
+    ns, K, NZ = zeta.shape
+    ctx = ctx or _lib.context(out.device.index or 0)
+    a, it = _sidx_ptr(sA)
+    b, _ = _sidx_ptr(sB)
+    c, _ = _sidx_ptr(sC)
+    ctx.call("aok_ovl3c", _dtype_code(dt), _flags(mode), pA.data_ptr(), pB.data_ptr(), pC.data_ptr(), a, b, c, it,
+             B, ns, K, NZ, _lib.i64_array(basis.ls), coef.data_ptr(), zeta.data_ptr(), float(basis.lengthscale),
+             out.data_ptr())
+    return out
+
+
+def batch_overlap_3c(basis: CartesianGTOBasis, XA, XB, XC, *, mode="fast", dtype=None, ctx=None):
+    """batch_overlap_3c(basis, posA, posB, posC) -> (N, N, N, B) (kernels_3c.jl:213-222)."""
+    pA = XA.positions if isinstance(XA, PStates) else XA[0]
+    dt = dtype or pA.dtype
+    N = basis.nbf_total
+    out = torch.empty((pA.shape[0], N, N, N), dtype=dt, device=pA.device).permute(3, 2, 1, 0)
+    return batch_overlap_3c_(out, basis, XA, XB, XC, mode=mode, ctx=ctx)

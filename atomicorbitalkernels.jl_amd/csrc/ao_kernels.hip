@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const T *__restrict__ X, c
         const T x = scale * X[3 * i], y = scale * X[3 * i + 1], z = scale * X[3 * i + 2];
         int sg = load_sigma(sidx, itype, i);
         if (sg < 0 || sg >= NZ) {
-            atomicOr(err, ERR_SPECIES);
+            *err = ERR_SPECIES;
             sg = 0;
         }
         const T r2 = x * x + y * y + z * z;
@@ -175,6 +175,97 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const T *__restrict__ X, c
                         store_nt(qd + 2, scale * (gz * z + R * pick<NY>(Zz, yy)));
                     }
                     }
+            }
+        }
+    }
+}
+
+// Values-only evaluation with PPL consecutive points per lane: every orbital column of
+// a wave is one contiguous 256*PPL-byte segment written by vector stores.
+template <int PPL, typename T>
+struct VecOf;
+template <>
+struct VecOf<2, float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecOf<4, float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct VecOf<2, double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecOf<4, double> {
+    typedef double type __attribute__((ext_vector_type(4)));
+};
+
+template <int LL, int L, int PPL, typename T>
+__device__ __forceinline__ void store_shell_vec(const T (*Z)[(L + 1) * (L + 1)], const T *R, int64_t i0, int64_t nX,
+                                               int o0, T *__restrict__ P)
+{
+    if constexpr (LL <= L) {
+        using V = typename VecOf<PPL, T>::type;
+#pragma unroll
+        for (int j = 0; j <= 2 * LL; ++j) {
+            const int yy = LL * LL + j;
+            V v;
+#pragma unroll
+            for (int p = 0; p < PPL; ++p) v[p] = R[p] * Z[p][yy];
+            __builtin_nontemporal_store(v, reinterpret_cast<V *>(P + i0 + nX * (o0 + j)));
+        }
+    }
+}
+
+template <typename T, int L, int PPL>
+__global__ __launch_bounds__(256) void ao_eval_vec_kernel(const T *__restrict__ X, const void *__restrict__ sidx,
+                                                          int itype, int64_t nX, const T *__restrict__ zeta,
+                                                          const T *__restrict__ D, int nRad, int K, int NZ, T scale,
+                                                          int decay, AoRunTable runs, T *__restrict__ P,
+                                                          unsigned *__restrict__ err)
+{
+    extern __shared__ unsigned char smem_raw[];
+    ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw);
+    load_param_table(tab, zeta, D, nRad, K, NZ);
+    __syncthreads();
+    constexpr int NY = (L + 1) * (L + 1);
+    const int64_t ntile = (nX + 256 * PPL - 1) / (256 * PPL);
+    for (int64_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const int64_t i0 = (t * 256 + threadIdx.x) * PPL;
+        if (i0 >= nX) continue;  // nX is a multiple of PPL (host-checked)
+        T Z[PPL][NY], f[PPL], r[PPL];
+        int sg[PPL];
+#pragma unroll
+        for (int p = 0; p < PPL; ++p) {
+            const int64_t i = i0 + p;
+            const T x = scale * X[3 * i], y = scale * X[3 * i + 1], z = scale * X[3 * i + 2];
+            int s = load_sigma(sidx, itype, i);
+            if (s < 0 || s >= NZ) {
+                *err = ERR_SPECIES;
+                s = 0;
+            }
+            sg[p] = s;
+            const T r2 = x * x + y * y + z * z;
+            r[p] = sqrt(r2);
+            f[p] = (decay == AOK_DECAY_GAUSSIAN) ? r2 : r[p];
+            solid_harmonics<L, false, T>(x, y, z, YlmOut<T>{Z[p], nullptr, nullptr, nullptr});
+        }
+        for (int q = 0; q < runs.nrun; ++q) {
+            const int k = runs.k[q], lk = runs.l[q], o0 = runs.o0[q];
+            T R[PPL];
+#pragma unroll
+            for (int p = 0; p < PPL; ++p) {
+                T G;
+                radial<T, false>(tab + (sg[p] * nRad + k) * K, K, f[p], r[p], runs.poly[q], decay, R[p], G);
+            }
+            switch (lk) {
+                case 0: store_shell_vec<0, L, PPL>(Z, R, i0, nX, o0, P); break;
+                case 1: store_shell_vec<1, L, PPL>(Z, R, i0, nX, o0, P); break;
+                case 2: store_shell_vec<2, L, PPL>(Z, R, i0, nX, o0, P); break;
+                case 3: store_shell_vec<3, L, PPL>(Z, R, i0, nX, o0, P); break;
+                case 4: store_shell_vec<4, L, PPL>(Z, R, i0, nX, o0, P); break;
+                default: break;
             }
         }
     }
@@ -260,7 +351,7 @@ __global__ __launch_bounds__(AO_PB_THREADS) void ao_pullback_kernel(
             const T x = scale * X[3 * i], y = scale * X[3 * i + 1], z = scale * X[3 * i + 2];
             int sg = load_sigma(sidx, itype, i);
             if (sg < 0 || sg >= NZ) {
-                atomicOr(err, ERR_SPECIES);
+                *err = ERR_SPECIES;
                 sg = 0;
             }
             const T r2 = x * x + y * y + z * z;
@@ -401,14 +492,29 @@ template <typename T, int L, bool GENERIC>
 static aok_status launch_eval_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &runs)
 {
     const size_t shm = sizeof(ParamPair<T>) * a.NZ * a.nRad * a.K;
-    const int grid = static_cast<int>(std::min<int64_t>((a.nX + 255) / 256, 65536));
-    if (grid <= 0) return AOK_OK;
+    if (a.nX <= 0) return AOK_OK;
+    // tuning: points per lane (values-only, canonical maps) and the persistent grid size
+    int ppl = ctx->tune.ao_ppl > 0 ? ctx->tune.ao_ppl : (std::is_same<T, float>::value ? 4 : 2);
+    if (a.dP || GENERIC || a.nX % ppl != 0 || (ppl != 2 && ppl != 4)) ppl = 1;
+    const int64_t ntile = (a.nX + 256 * ppl - 1) / (256 * ppl);
+    const int cap = ctx->tune.ao_grid > 0 ? ctx->tune.ao_grid : 4096;
+    const int grid = static_cast<int>(std::min<int64_t>(ntile, cap));
     time_begin(ctx, a.dP ? "ao_eval_ed" : "ao_eval");
     if (a.dP)
         hipLaunchKernelGGL((ao_eval_kernel<T, L, true, GENERIC>), dim3(grid), dim3(256), shm, ctx->stream,
                            static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
                            static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
                            static_cast<T *>(a.P), static_cast<T *>(a.dP), ctx->d_err);
+    else if (ppl == 4)
+        hipLaunchKernelGGL((ao_eval_vec_kernel<T, L, 4>), dim3(grid), dim3(256), shm, ctx->stream,
+                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
+                           static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+                           static_cast<T *>(a.P), ctx->d_err);
+    else if (ppl == 2)
+        hipLaunchKernelGGL((ao_eval_vec_kernel<T, L, 2>), dim3(grid), dim3(256), shm, ctx->stream,
+                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
+                           static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+                           static_cast<T *>(a.P), ctx->d_err);
     else
         hipLaunchKernelGGL((ao_eval_kernel<T, L, false, GENERIC>), dim3(grid), dim3(256), shm, ctx->stream,
                            static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),

@@ -18,7 +18,7 @@ struct aok_context {
     // Grow-only device workspace (partials, bucket tables, primitive-pair tables).
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    // Device error word (kernels OR bits into it) and its pinned host mirror.
+    // Error word in host-mapped pinned memory: h_err (host view), d_err (device view).
     unsigned *d_err = nullptr;
     unsigned *h_err = nullptr;
     // Optional timing of the dominant kernel of a call.
@@ -26,6 +26,12 @@ struct aok_context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     const char *last_kernel = "";
+    // Tuning knobs (0 = built-in default), set by aok_tune or the autotuner.
+    struct {
+        int ao_ppl = 0;   // AO values-only: consecutive points per lane (1, 2, 4)
+        int ao_grid = 0;  // AO: persistent grid size (blocks)
+        int ovl_cap = 0;  // 2c: LDS stripe capacity (floats per lane)
+    } tune;
 };
 
 namespace aok {
@@ -50,10 +56,9 @@ inline aok_status fail(aok_ctx ctx, aok_status st, const std::string &msg)
 // Ensure at least `bytes` of workspace; returns the base pointer or nullptr.
 aok_status ensure_ws(aok_ctx ctx, size_t bytes);
 
-// Clear the device error word (a tiny kernel, not a memset, so the sequence stays
-// capturable into a graph).
+// Clear the error word (host store, before the call's first launch).
 aok_status clear_err(aok_ctx ctx);
-// Copy the error word back, synchronise the stream, translate into a status.
+// Synchronise the stream, read the error word, translate into a status.
 aok_status finish(aok_ctx ctx, const char *what);
 
 // Event-bracketed launch of the dominant kernel (only when timing is enabled).

@@ -8,8 +8,6 @@
 
 namespace aok {
 
-__global__ void clear_word(unsigned *w) { *w = 0u; }
-
 aok_status ensure_ws(aok_ctx ctx, size_t bytes)
 {
     if (bytes <= ctx->ws_bytes) return AOK_OK;
@@ -29,22 +27,23 @@ aok_status ensure_ws(aok_ctx ctx, size_t bytes)
     return AOK_OK;
 }
 
+// The error word lives in host-mapped pinned memory: the host clears it before the
+// call's first launch, kernels store flags into it (plain stores, the same value from
+// every writer), and the host reads it after the call's closing synchronisation.
 aok_status clear_err(aok_ctx ctx)
 {
-    hipLaunchKernelGGL(clear_word, dim3(1), dim3(1), 0, ctx->stream, ctx->d_err);
-    AOK_HIP(ctx, hipGetLastError());
+    *reinterpret_cast<volatile unsigned *>(ctx->h_err) = 0u;
     return AOK_OK;
 }
 
 aok_status finish(aok_ctx ctx, const char *what)
 {
-    AOK_HIP(ctx, hipMemcpyAsync(ctx->h_err, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
     AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->timing) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
     }
-    if (*ctx->h_err & ERR_SPECIES)
+    if (*reinterpret_cast<volatile unsigned *>(ctx->h_err) & ERR_SPECIES)
         return fail(ctx, AOK_ERR_SPECIES, std::string(what) + ": species index outside 1..NZ");
     return AOK_OK;
 }
@@ -83,8 +82,8 @@ aok_status aok_create(aok_ctx *out, int device, void *stream)
         }
         c->own_stream = true;
     }
-    if (hipMalloc(&c->d_err, sizeof(unsigned)) != hipSuccess ||
-        hipHostMalloc(&c->h_err, sizeof(unsigned), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc(&c->h_err, sizeof(unsigned), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_err), c->h_err, 0) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         aok_destroy(c);
         return AOK_ERR_DEVICE;
@@ -98,7 +97,6 @@ aok_status aok_destroy(aok_ctx ctx)
     if (!ctx) return AOK_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ws) (void)hipFree(ctx->ws);
-    if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->h_err) (void)hipHostFree(ctx->h_err);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -116,6 +114,38 @@ const char *aok_version(void) { return "aok-mi355x 0.1.0 (gfx950)"; }
 double aok_last_kernel_ms(aok_ctx ctx) { return ctx ? ctx->last_ms : 0.0; }
 
 const char *aok_last_kernel_name(aok_ctx ctx) { return ctx ? ctx->last_kernel : ""; }
+
+aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, key != nullptr, "key is NULL");
+    const std::string k(key);
+    if (k == "ao_ppl") {
+        AOK_REQUIRE(ctx, value == 0 || value == 1 || value == 2 || value == 4, "ao_ppl must be 0, 1, 2 or 4");
+        ctx->tune.ao_ppl = static_cast<int>(value);
+    } else if (k == "ao_grid") {
+        AOK_REQUIRE(ctx, value >= 0 && value <= (1 << 20), "ao_grid out of range");
+        ctx->tune.ao_grid = static_cast<int>(value);
+    } else if (k == "ovl_cap") {
+        AOK_REQUIRE(ctx, value == 0 || (value >= 16 && value <= 1024), "ovl_cap must be 0 or in 16..1024");
+        ctx->tune.ovl_cap = static_cast<int>(value);
+    } else {
+        return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
+    }
+    return AOK_OK;
+}
+
+aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, key != nullptr && value != nullptr, "NULL argument");
+    const std::string k(key);
+    if (k == "ao_ppl") *value = ctx->tune.ao_ppl;
+    else if (k == "ao_grid") *value = ctx->tune.ao_grid;
+    else if (k == "ovl_cap") *value = ctx->tune.ovl_cap;
+    else return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
+    return AOK_OK;
+}
 
 aok_status aok_set_timing(aok_ctx ctx, int enable)
 {

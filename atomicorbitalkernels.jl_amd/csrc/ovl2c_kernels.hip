@@ -22,6 +22,7 @@
 // float atomics in the pullback -- kept as the baseline the fast path is measured against.
 #include "aok_common.hpp"
 #include "ovl_plan.hpp"
+#include "ovl_prep.hpp"
 
 namespace aok {
 
@@ -35,13 +36,20 @@ __device__ __forceinline__ double cart_cl(int l)
     return 5.568327996831707845 * df / static_cast<double>(1 << l);  // pi^{3/2} (2l-1)!! / 2^l
 }
 
+// x^-(l + 3/2) by integer powers and one square root
+__device__ __forceinline__ double inv_pow_l32(double x, int l)
+{
+    double t = x * sqrt(x);
+    for (int q = 0; q < l; ++q) t *= x;
+    return 1.0 / t;
+}
+
 // normsq of one (shell, species) contraction (cartesian_gto_basis.jl:74-83)
 __device__ double cart_normsq(const double *Dv, const double *zv, int K, int l)
 {
-    const double e = l + 1.5;
     double s = 0.0;
     for (int i = 0; i < K; ++i)
-        for (int j = 0; j < K; ++j) s += Dv[i] * Dv[j] / pow(zv[i] + zv[j], e);
+        for (int j = 0; j < K; ++j) s += Dv[i] * Dv[j] * inv_pow_l32(zv[i] + zv[j], l);
     return cart_cl(l) * s;
 }
 
@@ -61,7 +69,7 @@ __global__ __launch_bounds__(256) void ovl_hist_kernel(const void *__restrict__ 
     if (i < B) {
         int sa = load_sigma(sidxA, itype, i), sb = load_sigma(sidxB, itype, i);
         if (sa < 0 || sa >= NZ || sb < 0 || sb >= NZ) {
-            atomicOr(err, ERR_SPECIES);
+            *err = ERR_SPECIES;
             sa = sa < 0 || sa >= NZ ? 0 : sa;
             sb = sb < 0 || sb >= NZ ? 0 : sb;
         }
@@ -146,28 +154,7 @@ __global__ __launch_bounds__(256) void ovl_plan_kernel(const OvlShape sh, int mo
         nreal[idx] = nr;
         nlist[idx] = nl;
     }
-    for (int key = threadIdx.x; key < NK; key += blockDim.x) {
-        int run = 0;
-        for (int b = 0; b < nblk_h; ++b) {
-            blk_off[static_cast<int64_t>(b) * NK + key] = run;
-            run += blk_hist[static_cast<int64_t>(b) * NK + key];
-        }
-        bcnt[key] = run;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int o = 0, t = 0;
-        for (int key = 0; key < NK; ++key) {
-            boff[key] = o;
-            toff[key] = t;
-            o += bcnt[key];
-            t += (bcnt[key] + OVL_TILE - 1) / OVL_TILE;
-        }
-        toff[NK] = t;
-    }
-    __syncthreads();
-    for (int key = threadIdx.x; key < NK; key += blockDim.x)
-        for (int b = 0; b < nblk_h; ++b) blk_off[static_cast<int64_t>(b) * NK + key] += boff[key];
+    bucket_offsets(NK, nblk_h, blk_hist, blk_off, bcnt, boff, toff);
 }
 
 // ---------------------------------------------------------------------------------
@@ -493,8 +480,8 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     const T ab[3] = {static_cast<T>(dx), static_cast<T>(dy), static_cast<T>(dz)};
     T *mine = stripe + lane * sh.str;
     const int64_t NN = static_cast<int64_t>(N) * N;
-    for (int c = 0; c < sh.nchunk; ++c) {
-        const OvlChunk ch = sh.ch[c];
+    {
+        const OvlChunk ch = sh.ch[blockIdx.y];
         const int sb = ch.sb, lb = sh.ls[sb];
         const int nrb = nreal[sgB * ns + sb];
         for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
@@ -525,7 +512,6 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
             out[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i] =
                 stripe[bl * sh.str + q];
         }
-        __syncthreads();
     }
 }
 
@@ -544,14 +530,14 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    const int pstride = (ns + 1) * K * 2;
     int *bl_b = reinterpret_cast<int *>(smem_raw);
-    double *accA = reinterpret_cast<double *>(smem_raw + 256);
-    double *accB = accA + 2 * ns * K;
-    T *stripe = reinterpret_cast<T *>(accB + 2 * ns * K);
+    double *accA = reinterpret_cast<double *>(smem_raw + 256);  // [sa - sa0][K][2], then the ket side at ns*K*2
+    T *stripe = reinterpret_cast<T *>(accA + pstride);
     TileInfo ti;
     if (!tile_info(blockIdx.x, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
-    for (int q = lane; q < 4 * ns * K; q += OVL_TILE) accA[q] = 0.0;
+    for (int q = lane; q < pstride; q += OVL_TILE) accA[q] = 0.0;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
     const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
     bl_b[lane] = b;
@@ -565,77 +551,91 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     T *mine = stripe + lane * sh.str;
     const int64_t NN = static_cast<int64_t>(N) * N;
     __syncthreads();
-    for (int c = 0; c < sh.nchunk; ++c) {
-        const OvlChunk ch = sh.ch[c];
-        const int sb = ch.sb, lb = sh.ls[sb];
-        const int nbb = nbf_of(lb), nrows = ch.nrows;
-        const int len = nrows * nbb, total = OVL_TILE * len;
-        const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
-        const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
-        // stage the cotangent stripe (zeros for inactive lanes)
-        for (int e = lane; e < total; e += OVL_TILE) {
-            const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
-            const int q = e - bl * len;
-            const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
-            const int i = q - j * nrows;
-            stripe[bl * sh.str + q] =
-                bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i]
-                             : T(0);
-        }
-        __syncthreads();
-        const int nrb = nreal[sgB * ns + sb], nlb = nlist[sgB * ns + sb];
-        double *aB = accB + 2 * sb * K;
-        for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
-            const int la = sh.ls[sa];
-            const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
-            const typename RecOf<T>::type *recs = rec + g * K * K;
-            const int nra = nreal[sgA * ns + sa], nla = nlist[sgA * ns + sa];
-            const T *gsl = mine + (sh.off[sa] - ch.row0);
-            double *aA = accA + 2 * sa * K;
+    const OvlChunk ch = sh.ch[blockIdx.y];
+    const int sb = ch.sb, lb = sh.ls[sb];
+    const int nbb = nbf_of(lb), nrows = ch.nrows;
+    const int len = nrows * nbb, total = OVL_TILE * len;
+    const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
+    const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
+    // stage the cotangent stripe (zeros for inactive lanes)
+    for (int e = lane; e < total; e += OVL_TILE) {
+        const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
+        const int q = e - bl * len;
+        const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
+        const int i = q - j * nrows;
+        stripe[bl * sh.str + q] =
+            bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i] : T(0);
+    }
+    __syncthreads();
+    const int nrb = nreal[sgB * ns + sb], nlb = nlist[sgB * ns + sb];
+    double *aB = accA + ns * K * 2;
+    for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
+        const int la = sh.ls[sa];
+        const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
+        const typename RecOf<T>::type *recs = rec + g * K * K;
+        const int nra = nreal[sgA * ns + sa], nla = nlist[sgA * ns + sa];
+        const T *gsl = mine + (sh.off[sa] - ch.row0);
+        double *aA = accA + 2 * (sa - ch.sa0) * K;
 #define AOK_BWD_CASE(A, B_)                                                                      \
     if constexpr (A <= LMAXB && B_ <= LMAXB)                                                     \
         if (la == A && lb == B_) {                                                               \
             pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gsl, nrows, aA, aB, lane);   \
             continue;                                                                            \
         }
-            AOK_LL_CASES(AOK_BWD_CASE)
+        AOK_LL_CASES(AOK_BWD_CASE)
 #undef AOK_BWD_CASE
+    }
+    __syncthreads();
+    double *pt = partial + (static_cast<int64_t>(blockIdx.x) * gridDim.y + blockIdx.y) * pstride;
+    for (int q = lane; q < pstride; q += OVL_TILE) pt[q] = accA[q];
+}
+
+// Fixed-order sum of the per-wave partials into every (species, shell, slot); padded
+// slots read their representative's entry.  One block per slot: thread t sums waves
+// t, t + 256, ... and a fixed LDS tree combines the threads.  Output in the Julia
+// [ns x K x NZ] layout.
+__global__ __launch_bounds__(256) void ovl2c_bwd_finalize(const OvlShape sh, const int *__restrict__ tile_key,
+                                                          const int *__restrict__ toff, const int *__restrict__ pcls,
+                                                          const double *__restrict__ partial, double *__restrict__ dc64,
+                                                          double *__restrict__ da64)
+{
+    __shared__ double rc[256], ra[256];
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, nchunk = sh.nchunk;
+    const int pstride = (ns + 1) * K * 2;
+    const int q = blockIdx.x;
+    const int sg = q / (ns * K), rem = q - sg * ns * K, s = rem / K, k = rem - s * K;
+    const int li = pcls[(sg * ns + s) * K + k];
+    const int nw = toff[NZ * NZ] * nchunk;
+    double c = 0.0, a = 0.0;
+    for (int w = threadIdx.x; w < nw; w += 256) {
+        const int t = w / nchunk, ci = w - t * nchunk;
+        const int key = tile_key[t];
+        const OvlChunk ch = sh.ch[ci];
+        const double *pt = partial + static_cast<int64_t>(w) * pstride;
+        if (key / NZ == sg && s >= ch.sa0 && s < ch.sa1) {
+            c += pt[2 * ((s - ch.sa0) * K + li)];
+            a += pt[2 * ((s - ch.sa0) * K + li) + 1];
+        }
+        if (key % NZ == sg && s == ch.sb) {
+            c += pt[2 * (ns * K + li)];
+            a += pt[2 * (ns * K + li) + 1];
+        }
+    }
+    rc[threadIdx.x] = c;
+    ra[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            rc[threadIdx.x] += rc[threadIdx.x + h];
+            ra[threadIdx.x] += ra[threadIdx.x + h];
         }
         __syncthreads();
     }
-    double *pt = partial + static_cast<int64_t>(blockIdx.x) * 4 * ns * K;
-    for (int q = lane; q < 4 * ns * K; q += OVL_TILE) pt[q] = accA[q];
-}
-
-// Fixed-order sum of the per-tile partials into every (species, shell, slot); padded
-// slots read their representative's entry.  Output in the Julia [ns x K x NZ] layout.
-__global__ void ovl2c_bwd_finalize(const OvlShape sh, const int *__restrict__ tile_key,
-                                   const int *__restrict__ toff, const int *__restrict__ pcls,
-                                   const double *__restrict__ partial, double *__restrict__ dc64,
-                                   double *__restrict__ da64)
-{
-    const int ns = sh.ns, K = sh.K, NZ = sh.NZ;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= NZ * ns * K) return;
-    const int sg = q / (ns * K), rem = q - sg * ns * K, s = rem / K, k = rem - s * K;
-    const int li = pcls[(sg * ns + s) * K + k];
-    const int ntiles = toff[NZ * NZ];
-    double c = 0.0, a = 0.0;
-    for (int t = 0; t < ntiles; ++t) {
-        const int key = tile_key[t];
-        const double *pt = partial + static_cast<int64_t>(t) * 4 * ns * K;
-        if (key / NZ == sg) {
-            c += pt[2 * (s * K + li)];
-            a += pt[2 * (s * K + li) + 1];
-        }
-        if (key % NZ == sg) {
-            c += pt[2 * ns * K + 2 * (s * K + li)];
-            a += pt[2 * ns * K + 2 * (s * K + li) + 1];
-        }
+    if (threadIdx.x == 0) {
+        const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+        dc64[dst] = rc[0];
+        da64[dst] = ra[0];
     }
-    const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
-    dc64[dst] = c;
-    da64[dst] = a;
 }
 
 // ---------------------------------------------------------------------------------
@@ -684,7 +684,7 @@ __global__ void ovl2c_vjp_out(const OvlShape sh, const double *__restrict__ dc64
         double gm = 0.0, hm = 0.0;
         for (int j = 0; j < K; ++j) {
             const double den = zv[m] + zv[j];
-            const double pe = pow(den, -e);
+            const double pe = inv_pow_l32(den, l);
             gm += dv[j] * pe;
             hm += dv[j] * pe / den;
         }
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(256) void ovl2c_fwd_plain(const OvlShape sh, int64_
     const int sa = static_cast<int>((item / B) % ns), sb = static_cast<int>(item / (B * ns));
     int sgA = load_sigma(sidxA, itype, b), sgB = load_sigma(sidxB, itype, b);
     if (sgA < 0 || sgA >= NZ || sgB < 0 || sgB >= NZ) {
-        atomicOr(err, ERR_SPECIES);
+        *err = ERR_SPECIES;
         return;
     }
     const int la = sh.ls[sa], lb = sh.ls[sb];
@@ -827,7 +827,7 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_plain(const OvlShape sh, int64_
     const int sa = static_cast<int>((item / B) % ns), sb = static_cast<int>(item / (B * ns));
     int sgA = load_sigma(sidxA, itype, b), sgB = load_sigma(sidxB, itype, b);
     if (sgA < 0 || sgA >= NZ || sgB < 0 || sgB >= NZ) {
-        atomicOr(err, ERR_SPECIES);
+        *err = ERR_SPECIES;
         return;
     }
     const int la = sh.ls[sa], lb = sh.ls[sb];
@@ -910,12 +910,11 @@ static int lmax_of(const int64_t *ls, int64_t ns)
     return m;
 }
 
-constexpr int OVL_STRIPE_CAP = 96;
+constexpr int OVL_STRIPE_CAP = 32;
 
 template <typename T>
-static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w)
+static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
-    char *ws = static_cast<char *>(ctx->ws);
     using R = typename RecOf<T>::type;
     const int nblk_h = w.nblk_h;
     hipLaunchKernelGGL(ovl_hist_kernel, dim3(nblk_h), dim3(256), 0, ctx->stream, c.sidxA, c.sidxB, c.itype, c.B,
@@ -946,9 +945,8 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
 }
 
 template <typename T, int LMAXB>
-static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w)
+static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
-    char *ws = static_cast<char *>(ctx->ws);
     using R = typename RecOf<T>::type;
     const size_t shm = 256 + sizeof(T) * OVL_TILE * sh.str;
     auto kern = &ovl2c_fwd_fast<T, LMAXB>;
@@ -956,7 +954,7 @@ static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
     time_begin(ctx, "ovl2c_fwd_fast");
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles), dim3(OVL_TILE), shm, ctx->stream, sh, static_cast<const T *>(c.posA),
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh, static_cast<const T *>(c.posA),
                        static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
@@ -968,17 +966,16 @@ static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
 }
 
 template <typename T, int LMAXB>
-static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w)
+static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
-    char *ws = static_cast<char *>(ctx->ws);
     using R = typename RecOf<T>::type;
-    const size_t shm = 256 + sizeof(double) * 4 * sh.ns * sh.K + sizeof(T) * OVL_TILE * sh.str;
+    const size_t shm = 256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 + sizeof(T) * OVL_TILE * sh.str;
     auto kern = &ovl2c_bwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
     time_begin(ctx, "ovl2c_bwd_fast");
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles), dim3(OVL_TILE), shm, ctx->stream, sh, static_cast<const T *>(c.dS),
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh, static_cast<const T *>(c.dS),
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
@@ -988,7 +985,7 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
     const int nslot = sh.NZ * sh.ns * sh.K;
-    hipLaunchKernelGGL(ovl2c_bwd_finalize, dim3((nslot + 127) / 128), dim3(128), 0, ctx->stream, sh,
+    hipLaunchKernelGGL(ovl2c_bwd_finalize, dim3(nslot), dim3(256), 0, ctx->stream, sh,
                        reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.toff),
                        reinterpret_cast<const int *>(ws + w.pcls), reinterpret_cast<const double *>(ws + w.partial),
                        reinterpret_cast<double *>(ws + w.dc64), reinterpret_cast<double *>(ws + w.da64));
@@ -1070,62 +1067,133 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
     return AOK_OK;
 }
 
-// Entry used by the C ABI: validation, workspace, dispatch on the basis' largest l.
-template <typename T>
-aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback)
+// Shape of a call: validation and the stripe plan.
+static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh)
 {
-    OvlShape sh;
-    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, OVL_STRIPE_CAP, sh);
+    const int cap = ctx->tune.ovl_cap > 0 ? ctx->tune.ovl_cap : OVL_STRIPE_CAP;
+    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, cap, sh);
     if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
     if (c.B < 0) return fail(ctx, AOK_ERR_ARG, "B must be >= 0");
     if (c.B > (int64_t(1) << 30)) return fail(ctx, AOK_ERR_ARG, "B too large");
-    const int lmax = lmax_of(c.ls, c.ns);
-    if (c.flags & AOK_PLAIN) {
-        switch (lmax) {
-            case 0: return plain_launch<T, 0>(ctx, c, sh, pullback);
-            case 1: return plain_launch<T, 1>(ctx, c, sh, pullback);
-            case 2: return plain_launch<T, 2>(ctx, c, sh, pullback);
-            case 3: return plain_launch<T, 3>(ctx, c, sh, pullback);
-            default: return plain_launch<T, 4>(ctx, c, sh, pullback);
-        }
+    return AOK_OK;
+}
+
+template <typename T>
+static aok_status fast_forward(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
+{
+    if (c.B == 0) return AOK_OK;
+    switch (lmax_of(c.ls, c.ns)) {
+        case 0: return fast_fwd_launch<T, 0>(ctx, c, sh, w, ws);
+        case 1: return fast_fwd_launch<T, 1>(ctx, c, sh, w, ws);
+        case 2: return fast_fwd_launch<T, 2>(ctx, c, sh, w, ws);
+        case 3: return fast_fwd_launch<T, 3>(ctx, c, sh, w, ws);
+        default: return fast_fwd_launch<T, 4>(ctx, c, sh, w, ws);
     }
-    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sizeof(typename RecOf<T>::type), pullback);
-    aok_status st = ensure_ws(ctx, w.total);
-    if (st != AOK_OK) return st;
+}
+
+template <typename T>
+static aok_status fast_pullback(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
+{
+    aok_status st = AOK_OK;
+    const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
     if (c.B == 0) {
-        if (!pullback) return AOK_OK;
-        const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
-        char *ws = static_cast<char *>(ctx->ws);
         hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
                            reinterpret_cast<double *>(ws + w.dc64));
         hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
                            reinterpret_cast<double *>(ws + w.da64));
         AOK_HIP(ctx, hipGetLastError());
-        return out_stage<T>(ctx, c, sh, reinterpret_cast<double *>(ws + w.dc64),
-                            reinterpret_cast<double *>(ws + w.da64));
-    }
-    st = fast_prep<T>(ctx, c, sh, w);
-    if (st != AOK_OK) return st;
-    if (!pullback) {
-        switch (lmax) {
-            case 0: return fast_fwd_launch<T, 0>(ctx, c, sh, w);
-            case 1: return fast_fwd_launch<T, 1>(ctx, c, sh, w);
-            case 2: return fast_fwd_launch<T, 2>(ctx, c, sh, w);
-            case 3: return fast_fwd_launch<T, 3>(ctx, c, sh, w);
-            default: return fast_fwd_launch<T, 4>(ctx, c, sh, w);
+    } else {
+        switch (lmax_of(c.ls, c.ns)) {
+            case 0: st = fast_bwd_launch<T, 0>(ctx, c, sh, w, ws); break;
+            case 1: st = fast_bwd_launch<T, 1>(ctx, c, sh, w, ws); break;
+            case 2: st = fast_bwd_launch<T, 2>(ctx, c, sh, w, ws); break;
+            case 3: st = fast_bwd_launch<T, 3>(ctx, c, sh, w, ws); break;
+            default: st = fast_bwd_launch<T, 4>(ctx, c, sh, w, ws); break;
         }
+        if (st != AOK_OK) return st;
     }
-    switch (lmax) {
-        case 0: st = fast_bwd_launch<T, 0>(ctx, c, sh, w); break;
-        case 1: st = fast_bwd_launch<T, 1>(ctx, c, sh, w); break;
-        case 2: st = fast_bwd_launch<T, 2>(ctx, c, sh, w); break;
-        case 3: st = fast_bwd_launch<T, 3>(ctx, c, sh, w); break;
-        default: st = fast_bwd_launch<T, 4>(ctx, c, sh, w); break;
-    }
-    if (st != AOK_OK) return st;
-    char *ws = static_cast<char *>(ctx->ws);
     return out_stage<T>(ctx, c, sh, reinterpret_cast<double *>(ws + w.dc64), reinterpret_cast<double *>(ws + w.da64));
 }
+
+template <typename T>
+static aok_status plain_run(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, bool pullback)
+{
+    switch (lmax_of(c.ls, c.ns)) {
+        case 0: return plain_launch<T, 0>(ctx, c, sh, pullback);
+        case 1: return plain_launch<T, 1>(ctx, c, sh, pullback);
+        case 2: return plain_launch<T, 2>(ctx, c, sh, pullback);
+        case 3: return plain_launch<T, 3>(ctx, c, sh, pullback);
+        default: return plain_launch<T, 4>(ctx, c, sh, pullback);
+    }
+}
+
+// Entry used by the C ABI: validation, workspace, dispatch on the basis' largest l.
+template <typename T>
+aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback)
+{
+    OvlShape sh;
+    aok_status st = shape_of(ctx, c, sh);
+    if (st != AOK_OK) return st;
+    if (c.flags & AOK_PLAIN) return plain_run<T>(ctx, c, sh, pullback);
+    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), pullback);
+    st = ensure_ws(ctx, w.total);
+    if (st != AOK_OK) return st;
+    char *ws = static_cast<char *>(ctx->ws);
+    if (c.B > 0 && (st = fast_prep<T>(ctx, c, sh, w, ws)) != AOK_OK) return st;
+    return pullback ? fast_pullback<T>(ctx, c, sh, w, ws) : fast_forward<T>(ctx, c, sh, w, ws);
+}
+
+// ---------------------------------------------------------------------------------
+// Plans: the species-pair bucketing and the primitive-pair records of one (sidx, params)
+// set, built once and reused by forward and pullback calls (the rrule's forward and
+// its pullback see the same batch and parameters).
+// ---------------------------------------------------------------------------------
+template <typename T>
+aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p)
+{
+    aok_status st = shape_of(ctx, c, p.sh);
+    if (st != AOK_OK) return st;
+    p.call = c;
+    if (c.flags & AOK_PLAIN) return AOK_OK;
+    p.w = ovl2c_ws_map(c.B, p.sh.ns, p.sh.K, p.sh.NZ, p.sh.nchunk, sizeof(typename RecOf<T>::type), true);
+    if (p.w.total > p.bytes) {
+        if (p.buf) {
+            AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            AOK_HIP(ctx, hipFree(p.buf));
+            p.buf = nullptr;
+            p.bytes = 0;
+        }
+        if (hipMalloc(&p.buf, p.w.total) != hipSuccess) {
+            p.buf = nullptr;
+            return fail(ctx, AOK_ERR_NOMEM, "plan workspace allocation failed");
+        }
+        p.bytes = p.w.total;
+    }
+    if (c.B > 0) return fast_prep<T>(ctx, c, p.sh, p.w, static_cast<char *>(p.buf));
+    return AOK_OK;
+}
+
+template <typename T>
+aok_status ovl2c_plan_exec(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall &io, bool pullback)
+{
+    Ovl2cCall c = p.call;
+    c.posA = io.posA;
+    c.posB = io.posB;
+    c.scale = io.scale;
+    c.out = io.out;
+    c.dS = io.dS;
+    c.o1 = io.o1;
+    c.o2 = io.o2;
+    c.vjp = io.vjp;
+    if (c.flags & AOK_PLAIN) return plain_run<T>(ctx, c, p.sh, pullback);
+    char *ws = static_cast<char *>(p.buf);
+    return pullback ? fast_pullback<T>(ctx, c, p.sh, p.w, ws) : fast_forward<T>(ctx, c, p.sh, p.w, ws);
+}
+
+template aok_status ovl2c_plan_build<float>(aok_ctx, const Ovl2cCall &, Ovl2cPlanState &);
+template aok_status ovl2c_plan_build<double>(aok_ctx, const Ovl2cCall &, Ovl2cPlanState &);
+template aok_status ovl2c_plan_exec<float>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &, bool);
+template aok_status ovl2c_plan_exec<double>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &, bool);
 
 template <typename T>
 aok_status compile_coef_run(aok_ctx ctx, const void *zeta, const void *D, int64_t ns, int64_t K, int64_t NZ,

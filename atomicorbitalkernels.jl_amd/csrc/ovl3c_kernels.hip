@@ -18,6 +18,7 @@
 // (the reference decomposition; the E3 table is zero-filled beyond t = i+j+k).
 #include "aok_common.hpp"
 #include "ovl_plan.hpp"
+#include "ovl_prep.hpp"
 
 namespace aok {
 
@@ -88,7 +89,7 @@ __device__ __forceinline__ int key3(const void *sA, const void *sB, const void *
 {
     int a = load_sigma(sA, itype, i), b = load_sigma(sB, itype, i), c = load_sigma(sC, itype, i);
     const bool bad = a < 0 || a >= NZ || b < 0 || b >= NZ || c < 0 || c >= NZ;
-    if (bad && err) atomicOr(err, ERR_SPECIES);
+    if (bad && err) *err = ERR_SPECIES;
     a = (a < 0 || a >= NZ) ? 0 : a;
     b = (b < 0 || b >= NZ) ? 0 : b;
     c = (c < 0 || c >= NZ) ? 0 : c;
@@ -142,28 +143,7 @@ __global__ __launch_bounds__(256) void ovl3_plan_kernel(const OvlShape sh, const
         }
         nreal[idx] = nr;
     }
-    for (int key = threadIdx.x; key < NK; key += blockDim.x) {
-        int run = 0;
-        for (int b = 0; b < nblk_h; ++b) {
-            blk_off[static_cast<int64_t>(b) * NK + key] = run;
-            run += blk_hist[static_cast<int64_t>(b) * NK + key];
-        }
-        bcnt[key] = run;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int o = 0, t = 0;
-        for (int key = 0; key < NK; ++key) {
-            boff[key] = o;
-            toff[key] = t;
-            o += bcnt[key];
-            t += (bcnt[key] + OVL_TILE - 1) / OVL_TILE;
-        }
-        toff[NK] = t;
-    }
-    __syncthreads();
-    for (int key = threadIdx.x; key < NK; key += blockDim.x)
-        for (int b = 0; b < nblk_h; ++b) blk_off[static_cast<int64_t>(b) * NK + key] += boff[key];
+    bucket_offsets(NK, nblk_h, blk_hist, blk_off, bcnt, boff, toff);
     // record offsets: contiguous ranges of groups per thread, then a scan of the partials
     __shared__ int64_t part[256];
     const int64_t ngrp = static_cast<int64_t>(NK) * ns * ns * ns;
@@ -432,14 +412,13 @@ __global__ __launch_bounds__(64) void ovl3c_fwd_fast(const OvlShape sh, const T 
         ac[ax] = static_cast<T>(dac);
     }
     T *ob = out + static_cast<int64_t>(N) * N * N * b;
-    for (int sc = 0; sc < ns; ++sc) {
-        const int lc = sh.ls[sc], nc = nreal[sgC * ns + sc];
-        for (int sb = 0; sb < ns; ++sb) {
-            const int lb = sh.ls[sb], nb = nreal[sgB * ns + sb];
-            for (int sa = 0; sa < ns; ++sa) {
-                const int la = sh.ls[sa], na = nreal[sgA * ns + sa];
-                const int64_t g = ((static_cast<int64_t>(key) * ns + sc) * ns + sb) * ns + sa;
-                const typename Rec3Of<T>::type *recs = rec + grp_off[g];
+    const int sc = blockIdx.y / ns, sb = blockIdx.y - sc * ns;
+    const int lc = sh.ls[sc], nc = nreal[sgC * ns + sc];
+    const int lb = sh.ls[sb], nb = nreal[sgB * ns + sb];
+    for (int sa = 0; sa < ns; ++sa) {
+        const int la = sh.ls[sa], na = nreal[sgA * ns + sa];
+        const int64_t g = ((static_cast<int64_t>(key) * ns + sc) * ns + sb) * ns + sa;
+        const typename Rec3Of<T>::type *recs = rec + grp_off[g];
 #define AOK_F3_CASE(A_, B_, C_)                                                                                 \
     if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB)                                                    \
         if (la == A_ && lb == B_ && lc == C_) {                                                                 \
@@ -447,10 +426,8 @@ __global__ __launch_bounds__(64) void ovl3c_fwd_fast(const OvlShape sh, const T 
                                       active);                                                                  \
             continue;                                                                                           \
         }
-                AOK_L3_CASES(AOK_F3_CASE)
+        AOK_L3_CASES(AOK_F3_CASE)
 #undef AOK_F3_CASE
-            }
-        }
     }
 }
 
@@ -474,7 +451,7 @@ __global__ __launch_bounds__(128) void ovl3c_fwd_plain(const OvlShape sh, int64_
     const int sa = static_cast<int>(s3 % ns), sb = static_cast<int>((s3 / ns) % ns), sc = static_cast<int>(s3 / (ns * ns));
     int ga = load_sigma(sA, itype, b), gb = load_sigma(sB, itype, b), gc = load_sigma(sC, itype, b);
     if (ga < 0 || ga >= NZ || gb < 0 || gb >= NZ || gc < 0 || gc >= NZ) {
-        atomicOr(err, ERR_SPECIES);
+        *err = ERR_SPECIES;
         return;
     }
     const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
@@ -593,7 +570,7 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
                        reinterpret_cast<const int64_t *>(ws + w.grp_off), reinterpret_cast<R *>(ws + w.rec));
     AOK_HIP(ctx, hipGetLastError());
     time_begin(ctx, "ovl3c_fwd_fast");
-    hipLaunchKernelGGL((ovl3c_fwd_fast<T, LMAXB>), dim3(w.maxtiles), dim3(OVL_TILE), 0, ctx->stream, sh,
+    hipLaunchKernelGGL((ovl3c_fwd_fast<T, LMAXB>), dim3(w.maxtiles, sh.ns * sh.ns), dim3(OVL_TILE), 0, ctx->stream, sh,
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<const T *>(c.posC),
                        static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
                        reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),

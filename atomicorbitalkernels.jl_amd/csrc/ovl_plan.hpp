@@ -120,7 +120,7 @@ struct Ovl2cWs {
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, size_t rec_bytes, bool pullback)
+inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, int nchunk, size_t rec_bytes, bool pullback)
 {
     Ovl2cWs w{};
     const size_t npar = static_cast<size_t>(ns) * K * NZ;
@@ -149,7 +149,7 @@ inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, size_t rec_bytes, 
     w.perm = take(sizeof(int) * static_cast<size_t>(B > 0 ? B : 1));
     w.tile_key = take(sizeof(int) * static_cast<size_t>(w.maxtiles));
     if (pullback) {
-        w.partial = take(sizeof(double) * static_cast<size_t>(w.maxtiles) * 2 * ns * K * 2);
+        w.partial = take(sizeof(double) * static_cast<size_t>(w.maxtiles) * nchunk * (ns + 1) * K * 2);
         w.dc64 = take(sizeof(double) * npar);
         w.da64 = take(sizeof(double) * npar);
     }
@@ -189,8 +189,23 @@ struct Ovl3cCall {
     void *out;
 };
 
+// A reusable 2-centre plan (owned device workspace + the call description).
+struct Ovl2cPlanState {
+    aok_ctx ctx = nullptr;
+    int dtype = 0;
+    Ovl2cCall call{};
+    OvlShape sh{};
+    Ovl2cWs w{};
+    void *buf = nullptr;
+    size_t bytes = 0;
+};
+
 template <typename T>
 aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback);
+template <typename T>
+aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p);
+template <typename T>
+aok_status ovl2c_plan_exec(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall &io, bool pullback);
 template <typename T>
 aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c);
 template <typename T>

@@ -233,18 +233,42 @@ def main():
         wall = (time.perf_counter() - ts) * 1e3 / 10
         bytes_alg = nX * (12 + 4 + 4 * len(orb4))
         kms = float(np.mean(ks))
-        out["ao_c4b"] = {"nX": nX, "nOrb": len(orb4), "kernel_ms": kms, "wall_ms": wall,
+        out["ao_c4b"] = {"nX": nX, "nOrb": len(orb4), "kernel_ms": kms, "wall_ms": wall, "kernel": "ao_eval",
                          "GB_per_s": bytes_alg / (kms * 1e-3) / 1e9,
                          "frac_of_hbm_peak": bytes_alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "algorithmic_bytes": bytes_alg}
         del P, X
+        # 3-centre overlaps, B = 2^14 triples over the same 3 species, FP32
+        log("3c B=2^14")
+        c4 = configs.c4a()
+        coef3 = aok.compile_coef(orb32.Rnl.zeta, orb32.Rnl.D, cb.ls, ctx=ctx)
+        nbf3 = [(l + 1) * (l + 2) // 2 for l in cb.ls]
+        off3 = [0]
+        for n in nbf3:
+            off3.append(off3[-1] + n)
+        basis3 = aok.CartesianGTOBasis(2, cb.ns, cb.K, cb.ls, nbf3, off3, off3[-1], orb32.Rnl.zeta, coef3, zlist, 1.0)
+        X3 = [(torch.as_tensor(c4[f"pos{c}"], dtype=torch.float32, device=dev).contiguous(),
+               torch.as_tensor(c4[f"sidx{c}"], dtype=torch.int32, device=dev)) for c in "ABC"]
+        V = aok.batch_overlap_3c(basis3, *X3, ctx=ctx)
+        ks = []
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(5):
+            V = aok.batch_overlap_3c(basis3, *X3, ctx=ctx)
+            ks.append(ctx.last_kernel_ms())
+        torch.cuda.synchronize()
+        wall3 = (time.perf_counter() - ts) * 1e3 / 5
+        B3 = c4["posA"].shape[0]
+        out["ovl3c_c4a"] = {"B": B3, "kernel_ms": float(np.mean(ks)), "wall_ms": wall3,
+                            "triples_per_s": B3 / (wall3 * 1e-3), "kernel_triples_per_s": B3 / (np.mean(ks) * 1e-3)}
+        del V
 
     if not args.no_cpu:
         log("CPU baseline (C oracle, FP64, 1 thread)")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
 
-        nb = 512
+        nb = 16384
         coef = O.compile_coef(cb.zeta, cb.D, cb.ls)
         dSn = np.random.default_rng(0).standard_normal((cb.N, cb.N, nb))
         ts = time.perf_counter()

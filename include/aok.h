@@ -183,6 +183,16 @@ aok_status aok_ovl3c(aok_ctx ctx, aok_dtype dtype, unsigned flags,
                      int64_t ns, int64_t K, int64_t NZ, const int64_t *ls,
                      const void *coef, const void *alpha, double lengthscale, void *out);
 
+/* ---- tuning -------------------------------------------------------------------------------- */
+/* No reference counterpart (the reference has no tunables): set / read a per-context
+ * launch-shape knob used by the kernels and by the autotuner.  Keys:
+ *   "ao_ppl"   AO values-only evaluation: consecutive points per lane (1, 2 or 4; 0 = default)
+ *   "ao_grid"  AO kernels: persistent grid size in workgroups (0 = default)
+ *   "ovl_cap"  2-centre kernels: LDS stripe capacity in elements per lane (0 = default)
+ * Unknown keys or out-of-range values return AOK_ERR_ARG. */
+aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value);
+aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value);
+
 /* ---- measurement hooks ----------------------------------------------------------------- */
 /* Device time (ms) of the dominant kernel of the most recent call on this context, taken
  * with HIP events on the context stream around that kernel's launch (0 if none). */
