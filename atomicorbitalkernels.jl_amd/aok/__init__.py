@@ -14,14 +14,14 @@ from .basis import (ANG2BOHR, AtomicOrbitals, CartesianGTOBasis, GaussianTypeRad
                     gaussian_orbitals, length_factor, slater_orbitals)
 from .orbitals import (NO_TANGENT, NoTangent, PStates, evaluate, evaluate_ed, pullback_ps, pullback_x,
                        rrule_evaluate)
-from .overlap import (batch_overlap, batch_overlap_, batch_overlap_3c, batch_overlap_3c_, overlap_pb_coef_alpha,
+from .overlap import (Ovl2cPlan, batch_overlap, batch_overlap_, batch_overlap_3c, batch_overlap_3c_, overlap_pb_coef_alpha,
                       rrule_batch_overlap)
 from .overlap import pullback_ps as overlap_pullback_ps
 
 __all__ = [
     "ANG2BOHR", "AOKArgumentError", "AOKDeviceError", "AOKError", "AOKLibraryError", "AOKLmaxError",
     "AOKSpeciesError", "AtomicOrbitals", "CartesianGTOBasis", "Context", "GaussianTypeRadials", "NO_TANGENT",
-    "NoTangent", "PStates", "SlaterTypeRadials", "as_fortran", "batch_overlap", "batch_overlap_", "batch_overlap_3c",
+    "NoTangent", "Ovl2cPlan", "PStates", "SlaterTypeRadials", "as_fortran", "batch_overlap", "batch_overlap_", "batch_overlap_3c",
     "batch_overlap_3c_", "compile_basis",
     "compile_coef", "compile_coef_pb", "context", "evaluate", "evaluate_ed", "fortran_from_numpy", "fortran_zeros",
     "gaussian_orbitals", "length_factor", "overlap_pb_coef_alpha", "overlap_pullback_ps", "pullback_ps",

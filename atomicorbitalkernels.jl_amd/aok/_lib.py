@@ -31,6 +31,8 @@ AOK_DECAY_GAUSSIAN = 0
 AOK_DECAY_SLATER = 1
 AOK_FAST = 0
 AOK_PLAIN = 1
+AOK_PARAMS_COEF = 0
+AOK_PARAMS_PS = 1
 
 AO_LMAX = 4
 OVL_LMAX = 4
@@ -76,6 +78,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _int = ctypes.c_int
 _uint = ctypes.c_uint
 _dbl = ctypes.c_double
+c_void_p = ctypes.c_void_p
+byref = ctypes.byref
 
 # name -> argtypes (restype aok_status = int unless noted)
 _SIGS = {
@@ -97,6 +101,11 @@ _SIGS = {
                            _dbl, _vp, _vp],
     "aok_ovl2c_pullback_ps": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp,
                               _vp, _dbl, _vp, _vp],
+    "aok_ovl2c_plan_create": [_vp, _int, _uint, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _int, _vp, _vp,
+                              _dbl, ctypes.POINTER(_vp)],
+    "aok_ovl2c_plan_forward": [_vp, _vp, _vp, _vp],
+    "aok_ovl2c_plan_pullback": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "aok_ovl2c_plan_destroy": [_vp],
     "aok_ovl3c": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp,
                   _dbl, _vp],
 }

@@ -177,12 +177,75 @@ def pullback_ps(dS, orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", ct
     return {"Rnl": {"zeta": dz, "D": dD}, "Ylm": {}}
 
 
-def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", ctx=None):
-    """rrule(batch_overlap, orb, XA, XB, ps, st) (overlap_2c_grad.jl:309-318)."""
-    S = batch_overlap(orb, XA, XB, ps, st, mode=mode, ctx=ctx)
+class Ovl2cPlan:
+    """A 2-centre plan (aok_ovl2c_plan_*): the species-pair bucketing of one batch and the
+    primitive-pair records of one parameter set, shared by a forward and its pullback.
+    Holds the species-index and parameter tensors it was built from."""
+
+    def __init__(self, ls, sA, sB, B, p1, p2, params, lengthscale, dtype, mode="fast", ctx=None):
+        self.ctx = ctx or _lib.context(p1.device.index or 0)
+        self._keep = (sA, sB, p1, p2)
+        self.dtype = dtype
+        self.B = B
+        self.shape = tuple(p1.shape)
+        a, it = _sidx_ptr(sA)
+        b, _ = _sidx_ptr(sB)
+        ns, K, NZ = p1.shape
+        h = _lib.c_void_p()
+        rc = self.ctx.lib.aok_ovl2c_plan_create(self.ctx.handle, _dtype_code(dtype), _flags(mode), a, b, it, B, ns, K,
+                                                NZ, _lib.i64_array(ls), params, p1.data_ptr(), p2.data_ptr(),
+                                                float(lengthscale), _lib.byref(h))
+        self.ctx.check(rc, "aok_ovl2c_plan_create")
+        self.handle = h
+
+    def forward(self, pA, pB, out):
+        rc = self.ctx.lib.aok_ovl2c_plan_forward(self.handle, pA.data_ptr(), pB.data_ptr(), out.data_ptr())
+        self.ctx.check(rc, "aok_ovl2c_plan_forward")
+        return out
+
+    def pullback(self, dS, pA, pB):
+        d1 = fortran_empty(self.shape, self.dtype, dS.device)
+        d2 = fortran_empty(self.shape, self.dtype, dS.device)
+        rc = self.ctx.lib.aok_ovl2c_plan_pullback(self.handle, dS.data_ptr(), pA.data_ptr(), pB.data_ptr(),
+                                                  d1.data_ptr(), d2.data_ptr())
+        self.ctx.check(rc, "aok_ovl2c_plan_pullback")
+        return d1, d2
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.aok_ovl2c_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", dtype=None, ctx=None):
+    """rrule(batch_overlap, orb, XA, XB, ps, st) (overlap_2c_grad.jl:309-318) -> (S, pullback);
+    pullback(dS) -> (NoTangent, NoTangent, NoTangent, NoTangent, dps, NoTangent).  Forward
+    and pullback share one plan (coefficient compilation, bucketing, pair records)."""
+    if orb.Rnl.decay != _lib.AOK_DECAY_GAUSSIAN:
+        raise _lib.AOKArgumentError("the overlap kernels integrate plain Gaussians")
+    ls, _, off = compile_struct(orb)
+    pA, sA, pB, sB = _pair_inputs(XA, XB, orb.Rnl.zlist)
+    zeta, D = ps["Rnl"]["zeta"], ps["Rnl"]["D"]
+    dt = dtype or torch.promote_types(pA.dtype, torch.promote_types(zeta.dtype, D.dtype))
+    zeta, D = as_fortran(zeta.to(dt)), as_fortran(D.to(dt))
+    pA, pB = pA.to(dt), pB.to(dt)
+    B = pA.shape[0]
+    N = off[-1]
+    plan = Ovl2cPlan(ls, sA, sB, B, zeta, D, _lib.AOK_PARAMS_PS, orb.lengthscale, dt, mode=mode,
+                     ctx=ctx or _lib.context(pA.device.index or 0))
+    S = plan.forward(pA, pB, _out_tensor(N, B, dt, pA.device))
 
     def pullback(dS):
-        dps = pullback_ps(dS, orb, XA, XB, ps, st, mode=mode, ctx=ctx)
+        if tuple(dS.shape) != (N, N, B):
+            raise _lib.AOKArgumentError(f"dS must be {(N, N, B)}")
+        dz, dD = plan.pullback(as_fortran(dS.to(dt)), pA, pB)
+        dps = {"Rnl": {"zeta": dz, "D": dD}, "Ylm": {}}
         return NO_TANGENT, NO_TANGENT, NO_TANGENT, NO_TANGENT, dps, NO_TANGENT
 
     return S, pullback

@@ -7,8 +7,13 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/aok.h"
+
+namespace aok {
+struct Ovl2cPlanState;
+}
 
 struct aok_context {
     int device = 0;
@@ -18,11 +23,18 @@ struct aok_context {
     // Grow-only device workspace (partials, bucket tables, primitive-pair tables).
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // Workspace of the most recently destroyed 2-centre plan, handed to the next one.
+    void *plan_cache = nullptr;
+    size_t plan_cache_bytes = 0;
+    // Live plans of this context: destroying the context releases their workspaces and
+    // detaches them (a detached plan only accepts aok_ovl2c_plan_destroy).
+    std::vector<aok::Ovl2cPlanState *> plans;
     // Error word in host-mapped pinned memory: h_err (host view), d_err (device view).
     unsigned *d_err = nullptr;
     unsigned *h_err = nullptr;
     // Optional timing of the dominant kernel of a call.
     bool timing = false;
+    bool timed = false;  // the current call recorded ev0/ev1
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     const char *last_kernel = "";
@@ -67,6 +79,7 @@ inline void time_begin(aok_ctx ctx, const char *name)
     if (ctx->timing) {
         (void)hipEventRecord(ctx->ev0, ctx->stream);
         ctx->last_kernel = name;
+        ctx->timed = true;
     }
 }
 inline void time_end(aok_ctx ctx)

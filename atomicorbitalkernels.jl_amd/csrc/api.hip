@@ -33,15 +33,21 @@ aok_status ensure_ws(aok_ctx ctx, size_t bytes)
 aok_status clear_err(aok_ctx ctx)
 {
     *reinterpret_cast<volatile unsigned *>(ctx->h_err) = 0u;
+    ctx->timed = false;
+    ctx->last_ms = 0.0;
+    ctx->last_kernel = "";
     return AOK_OK;
 }
 
 aok_status finish(aok_ctx ctx, const char *what)
 {
     AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->timing) {
+    if (ctx->timing && ctx->timed) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
+            ctx->last_ms = ms;
+        else
+            (void)hipGetLastError();  // do not leave a timing failure for the caller's next HIP call
     }
     if (*reinterpret_cast<volatile unsigned *>(ctx->h_err) & ERR_SPECIES)
         return fail(ctx, AOK_ERR_SPECIES, std::string(what) + ": species index outside 1..NZ");
@@ -97,6 +103,11 @@ aok_status aok_destroy(aok_ctx ctx)
     if (!ctx) return AOK_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->plan_cache) (void)hipFree(ctx->plan_cache);
+    for (Ovl2cPlanState *p : ctx->plans) {
+        if (p->buf) (void)hipFree(p->buf);
+        p->buf = nullptr, p->bytes = 0, p->ctx = nullptr;
+    }
     if (ctx->h_err) (void)hipHostFree(ctx->h_err);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -321,6 +332,129 @@ aok_status aok_ovl2c_pullback_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, c
     c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.alpha = zeta, c.D = D, c.mode = 1, c.scale = lengthscale;
     c.dS = dS, c.o1 = dzeta, c.o2 = dD, c.vjp = true;
     return ovl2c_entry(ctx, dtype, c, true, "aok_ovl2c_pullback_ps");
+}
+
+// ---- H2 2-centre plans -------------------------------------------------------------------
+struct aok_ovl2c_plan_s {
+    aok::Ovl2cPlanState st;
+};
+
+aok_status aok_ovl2c_plan_create(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *sidxA, const void *sidxB,
+                                 aok_itype itype, int64_t B, int64_t ns, int64_t K, int64_t NZ, const int64_t *ls,
+                                 int params, const void *p1, const void *p2, double lengthscale,
+                                 aok_ovl2c_plan *plan)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, plan != nullptr, "plan is NULL");
+    *plan = nullptr;
+    AOK_REQUIRE(ctx, params == AOK_PARAMS_COEF || params == AOK_PARAMS_PS, "params must be AOK_PARAMS_COEF or _PS");
+    AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
+    AOK_REQUIRE(ctx, !bad_itype(itype), "itype must be AOK_I64 or AOK_I32");
+    AOK_REQUIRE(ctx, (flags & ~AOK_PLAIN) == 0u, "unknown flags");
+    AOK_REQUIRE(ctx, ls != nullptr, "ls is NULL");
+    AOK_REQUIRE(ctx, ns >= 1 && ns <= OVL_MAXSH, "ns out of range");
+    AOK_REQUIRE(ctx, B >= 0, "B must be >= 0");
+    AOK_REQUIRE(ctx, p1 && p2, "parameters are NULL");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    auto *pl = new (std::nothrow) aok_ovl2c_plan_s;
+    if (!pl) return fail(ctx, AOK_ERR_NOMEM, "plan allocation failed");
+    Ovl2cPlanState &p = pl->st;
+    p.ctx = ctx;
+    p.dtype = dtype;
+    for (int64_t s = 0; s < ns; ++s) p.ls[s] = ls[s];
+    Ovl2cCall c{};
+    c.flags = flags;
+    c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
+    c.ls = p.ls, c.ns = ns, c.K = K, c.NZ = NZ, c.scale = lengthscale;
+    if (params == AOK_PARAMS_COEF) {
+        c.coef = p1, c.alpha = p2, c.mode = 0, c.vjp = false;
+    } else {
+        c.alpha = p1, c.D = p2, c.mode = 1, c.vjp = true;
+    }
+    // Take over the workspace of the last destroyed plan of this context.
+    p.buf = ctx->plan_cache, p.bytes = ctx->plan_cache_bytes;
+    ctx->plan_cache = nullptr, ctx->plan_cache_bytes = 0;
+    aok_status st = clear_err(ctx);
+    if (st == AOK_OK)
+        st = dtype == AOK_F32 ? ovl2c_plan_build<float>(ctx, c, p) : ovl2c_plan_build<double>(ctx, c, p);
+    if (st == AOK_OK) st = finish(ctx, "aok_ovl2c_plan_create");
+    if (st != AOK_OK) {
+        aok_ovl2c_plan_destroy(pl);
+        return st;
+    }
+    ctx->plans.push_back(&p);
+    *plan = pl;
+    return AOK_OK;
+}
+
+static aok_status plan_exec(aok_ovl2c_plan plan, const Ovl2cCall &io, bool pullback, const char *what)
+{
+    aok_ctx ctx = plan->st.ctx;
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    st = plan->st.dtype == AOK_F32 ? ovl2c_plan_exec<float>(ctx, plan->st, io, pullback)
+                                   : ovl2c_plan_exec<double>(ctx, plan->st, io, pullback);
+    if (st != AOK_OK) return st;
+    return finish(ctx, what);
+}
+
+aok_status aok_ovl2c_plan_forward(aok_ovl2c_plan plan, const void *posA, const void *posB, void *out)
+{
+    if (!plan || !plan->st.ctx) return AOK_ERR_ARG;
+    aok_ctx ctx = plan->st.ctx;
+    const int64_t B = plan->st.call.B;
+    AOK_REQUIRE(ctx, B == 0 || (posA && posB), "positions are NULL");
+    AOK_REQUIRE(ctx, B == 0 || out != nullptr, "out is NULL");
+    Ovl2cCall io{};
+    io.posA = posA, io.posB = posB, io.out = out, io.scale = plan->st.call.scale;
+    return plan_exec(plan, io, false, "aok_ovl2c_plan_forward");
+}
+
+aok_status aok_ovl2c_plan_pullback(aok_ovl2c_plan plan, const void *dS, const void *posA, const void *posB,
+                                   void *d1, void *d2)
+{
+    if (!plan || !plan->st.ctx) return AOK_ERR_ARG;
+    aok_ctx ctx = plan->st.ctx;
+    const int64_t B = plan->st.call.B;
+    AOK_REQUIRE(ctx, B == 0 || (posA && posB), "positions are NULL");
+    AOK_REQUIRE(ctx, B == 0 || dS != nullptr, "dS is NULL");
+    AOK_REQUIRE(ctx, d1 && d2, "parameter cotangents are NULL");
+    Ovl2cCall io{};
+    io.posA = posA, io.posB = posB, io.dS = dS, io.o1 = d1, io.o2 = d2, io.scale = plan->st.call.scale;
+    return plan_exec(plan, io, true, "aok_ovl2c_plan_pullback");
+}
+
+aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan)
+{
+    if (!plan) return AOK_OK;
+    aok_ctx ctx = plan->st.ctx;
+    void *buf = plan->st.buf;
+    const size_t bytes = plan->st.bytes;
+    if (ctx) {
+        auto &v = ctx->plans;
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i] == &plan->st) {
+                v.erase(v.begin() + static_cast<std::ptrdiff_t>(i));
+                break;
+            }
+    }
+    delete plan;
+    if (!buf) return AOK_OK;  // detached plans hold no workspace
+    if (!ctx) {
+        (void)hipFree(buf);
+        return AOK_OK;
+    }
+    // Keep the larger of the two buffers for the next plan; every call is blocking, so
+    // no work on the context stream still reads either.
+    (void)hipSetDevice(ctx->device);
+    if (bytes > ctx->plan_cache_bytes) {
+        if (ctx->plan_cache) (void)hipFree(ctx->plan_cache);
+        ctx->plan_cache = buf, ctx->plan_cache_bytes = bytes;
+    } else {
+        (void)hipFree(buf);
+    }
+    return AOK_OK;
 }
 
 aok_status aok_ovl3c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *posA, const void *posB,

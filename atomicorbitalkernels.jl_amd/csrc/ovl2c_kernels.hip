@@ -53,6 +53,53 @@ __device__ double cart_normsq(const double *Dv, const double *zv, int K, int l)
     return cart_cl(l) * s;
 }
 
+// One (species, shell) handled by a whole block: zeta, D of its K slots in FP64 and the
+// fixed-order row sums
+//   g[m] = sum_j D_j (zeta_m + zeta_j)^-(l+3/2),  h[m] = sum_j D_j (zeta_m + zeta_j)^-(l+5/2),
+// so that normsq = c_l sum_m D_m g[m] (cartesian_gto_basis.jl:74-83) and g, h are the
+// sums of the normalisation VJP (overlap_2c_grad.jl:251-286).  Terms are spread over the
+// block's threads; each row is then summed by one thread in j order.
+struct ShellNorm {
+    double z[OVL_MAXK], d[OVL_MAXK], g[OVL_MAXK], h[OVL_MAXK];
+    double tg[OVL_MAXK * OVL_MAXK], th[OVL_MAXK * OVL_MAXK];
+    double nsq;
+};
+
+template <typename T>
+__device__ void shell_norm(const T *__restrict__ zetaT, const T *__restrict__ DT, int64_t q0, int64_t qstep, int K,
+                           int l, bool want_h, ShellNorm &S)
+{
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        S.z[k] = static_cast<double>(zetaT[q0 + qstep * k]);
+        S.d[k] = static_cast<double>(DT[q0 + qstep * k]);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < K * K; t += blockDim.x) {
+        const int m = t / K, j = t - m * K;
+        const double x = S.z[m] + S.z[j];
+        const double pe = inv_pow_l32(x, l);
+        S.tg[t] = S.d[j] * pe;
+        if (want_h) S.th[t] = S.d[j] * pe / x;
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+        double g = 0.0, h = 0.0;
+        for (int j = 0; j < K; ++j) {
+            g += S.tg[m * K + j];
+            if (want_h) h += S.th[m * K + j];
+        }
+        S.g[m] = g;
+        S.h[m] = h;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int m = 0; m < K; ++m) a += S.d[m] * S.g[m];
+        S.nsq = cart_cl(l) * a;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------------
 // Prep 1: species-pair keys, stable in-block ranks, per-block histograms.
 // ---------------------------------------------------------------------------------
@@ -60,10 +107,8 @@ __global__ __launch_bounds__(256) void ovl_hist_kernel(const void *__restrict__ 
                                                        int itype, int64_t B, int NZ, int *__restrict__ rank,
                                                        int *__restrict__ blk_hist, unsigned *__restrict__ err)
 {
-    __shared__ int keys[256];
-    __shared__ int hist[OVL_MAXNZ * OVL_MAXNZ];
+    __shared__ int whist[4 * OVL_MAXNZ * OVL_MAXNZ];
     const int NK = NZ * NZ;
-    for (int q = threadIdx.x; q < NK; q += 256) hist[q] = 0;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     int key = -1;
     if (i < B) {
@@ -75,16 +120,8 @@ __global__ __launch_bounds__(256) void ovl_hist_kernel(const void *__restrict__ 
         }
         key = sa * NZ + sb;
     }
-    keys[threadIdx.x] = key;
-    __syncthreads();
-    if (i < B) {
-        int r = 0;
-        for (int j = 0; j < static_cast<int>(threadIdx.x); ++j) r += (keys[j] == key);
-        rank[i] = r;
-        atomicAdd(&hist[key], 1);
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < NK; q += 256) blk_hist[static_cast<int64_t>(blockIdx.x) * NK + q] = hist[q];
+    const int r = block_rank(key, NK, whist, blk_hist + static_cast<int64_t>(blockIdx.x) * NK);
+    if (i < B) rank[i] = r;
 }
 
 // ---------------------------------------------------------------------------------
@@ -105,56 +142,59 @@ __global__ __launch_bounds__(256) void ovl_plan_kernel(const OvlShape sh, int mo
                                                        int *__restrict__ bcnt, int *__restrict__ boff,
                                                        int *__restrict__ toff)
 {
-    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, NK = NZ * NZ;
-    for (int idx = threadIdx.x; idx < NZ * ns; idx += blockDim.x) {
-        const int sg = idx / ns, s = idx - sg * ns;
-        double zv[OVL_MAXK], dv[OVL_MAXK], cv[OVL_MAXK];
-        for (int k = 0; k < K; ++k) {
-            const int64_t q = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
-            zv[k] = static_cast<double>(zetaT[q]);
-            if (mode == 0) {
-                cv[k] = static_cast<double>(coefT[q]);
-            } else {
-                dv[k] = static_cast<double>(DT[q]);
-            }
-        }
-        if (mode == 1) {
-            const double nsq = cart_normsq(dv, zv, K, sh.ls[s]);
-            const double inv = nsq > 0.0 ? 1.0 / sqrt(nsq) : 0.0;
-            for (int k = 0; k < K; ++k) cv[k] = nsq > 0.0 ? dv[k] * inv : 0.0;
-        }
-        int nr = 0;
-        int *li = lidx + static_cast<int64_t>(idx) * K;
-        int *pc = pcls + static_cast<int64_t>(idx) * K;
-        for (int k = 0; k < K; ++k) {
-            const int64_t q = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
-            coef64[q] = cv[k];
-            alpha64[q] = zv[k];
-            // the forward's skip test is made in the kernel precision (overlap_2c.jl:202)
-            if (static_cast<T>(cv[k]) != T(0)) {
-                pc[k] = nr;
-                li[nr++] = k;
-            }
-        }
-        int nl = nr;
-        for (int k = 0; k < K; ++k) {
-            if (static_cast<T>(cv[k]) != T(0)) continue;
-            int rep = -1;
-            for (int e = nr; e < nl; ++e)
-                if (zv[li[e]] == zv[k]) {
-                    rep = e;
-                    break;
-                }
-            if (rep < 0) {
-                rep = nl;
-                li[nl++] = k;
-            }
-            pc[k] = rep;
-        }
-        nreal[idx] = nr;
-        nlist[idx] = nl;
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ;
+    if (blockIdx.x == 0) {  // block 0: bucket offsets of the batch
+        bucket_offsets(NZ * NZ, nblk_h, blk_hist, blk_off, bcnt, boff, toff);
+        return;
     }
-    bucket_offsets(NK, nblk_h, blk_hist, blk_off, bcnt, boff, toff);
+    // blocks 1..NZ*ns: one (species, shell) each
+    __shared__ ShellNorm S;
+    __shared__ double cv[OVL_MAXK];
+    const int idx = blockIdx.x - 1;
+    const int sg = idx / ns, s = idx - sg * ns;
+    const int64_t q0 = s + static_cast<int64_t>(ns) * K * sg;
+    if (mode == 1) {
+        shell_norm(zetaT, DT, q0, ns, K, sh.ls[s], false, S);
+        const double nsq = S.nsq;
+        const double inv = nsq > 0.0 ? 1.0 / sqrt(nsq) : 0.0;
+        for (int k = threadIdx.x; k < K; k += blockDim.x) cv[k] = nsq > 0.0 ? S.d[k] * inv : 0.0;
+    } else {
+        for (int k = threadIdx.x; k < K; k += blockDim.x) {
+            S.z[k] = static_cast<double>(zetaT[q0 + static_cast<int64_t>(ns) * k]);
+            cv[k] = static_cast<double>(coefT[q0 + static_cast<int64_t>(ns) * k]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    // one wave builds the primitive list: real slots in slot order, then the first slot
+    // of every distinct padded exponent (K <= 32 lanes)
+    const int k = threadIdx.x;
+    const bool valid = k < K;
+    // the forward's skip test is made in the kernel precision (overlap_2c.jl:202)
+    const bool real = valid && static_cast<T>(cv[k]) != T(0);
+    const unsigned long long mr = __ballot(real);
+    int first = k;
+    if (valid && !real)
+        for (int k2 = 0; k2 < k; ++k2)
+            if (!((mr >> k2) & 1ull) && S.z[k2] == S.z[k]) {
+                first = k2;
+                break;
+            }
+    const bool rep = valid && !real && first == k;
+    const unsigned long long mp = __ballot(rep);
+    const int nr = __popcll(mr);
+    if (valid) {
+        const int64_t q = q0 + static_cast<int64_t>(ns) * k;
+        coef64[q] = cv[k];
+        alpha64[q] = S.z[k];
+        const int pos = real ? __popcll(mr & ((1ull << k) - 1ull)) : nr + __popcll(mp & ((1ull << first) - 1ull));
+        pcls[static_cast<int64_t>(idx) * K + k] = pos;
+        if (real || rep) lidx[static_cast<int64_t>(idx) * K + pos] = k;
+    }
+    if (k == 0) {
+        nreal[idx] = nr;
+        nlist[idx] = nr + __popcll(mp);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -590,129 +630,158 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     for (int q = lane; q < pstride; q += OVL_TILE) pt[q] = accA[q];
 }
 
-// Fixed-order sum of the per-wave partials into every (species, shell, slot); padded
-// slots read their representative's entry.  One block per slot: thread t sums waves
-// t, t + 256, ... and a fixed LDS tree combines the threads.  Output in the Julia
-// [ns x K x NZ] layout.
-__global__ __launch_bounds__(256) void ovl2c_bwd_finalize(const OvlShape sh, const int *__restrict__ tile_key,
-                                                          const int *__restrict__ toff, const int *__restrict__ pcls,
-                                                          const double *__restrict__ partial, double *__restrict__ dc64,
-                                                          double *__restrict__ da64)
+// ---------------------------------------------------------------------------------
+// Pullback reduction, stage 1: block (slot, side, partner species).  A primitive slot
+// (species sg, shell s, k) receives bra-side sums from the tiles of species pairs
+// (sg, partner) in the chunks whose row range holds s, and ket-side sums from the tiles
+// of (partner, sg) in the chunks of column shell s.  Tiles of one species pair are
+// contiguous, so each block reads only its own rows; the block sums them in a fixed
+// order and writes part[(side * NZ + partner) * npar + q] (q the Julia [ns x K x NZ]
+// index), dcoef first, then dalpha at + 2 * NZ * npar.  Stage 2 (the sum over the
+// 2 * NZ parts) runs in the output stage.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const int *__restrict__ toff,
+                                                        const int *__restrict__ pcls,
+                                                        const double *__restrict__ partial, double *__restrict__ part)
 {
-    __shared__ double rc[256], ra[256];
+    __shared__ int rel_ci[OVL_MAXCH], rel_col[OVL_MAXCH];
+    __shared__ int wcnt[4];
+    __shared__ double red[256];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, nchunk = sh.nchunk;
     const int pstride = (ns + 1) * K * 2;
+    const int64_t npar = static_cast<int64_t>(NZ) * ns * K;
     const int q = blockIdx.x;
     const int sg = q / (ns * K), rem = q - sg * ns * K, s = rem / K, k = rem - s * K;
     const int li = pcls[(sg * ns + s) * K + k];
-    const int nw = toff[NZ * NZ] * nchunk;
+    const int side = blockIdx.y / NZ, partner = blockIdx.y - side * NZ;
+    const int key = side == 0 ? sg * NZ + partner : partner * NZ + sg;
+    // relevant chunks, compacted in chunk order with ballots
+    const int ci = threadIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    bool use = false;
+    int col = 0;
+    if (ci < nchunk) {
+        const OvlChunk ch = sh.ch[ci];
+        if (side == 0) {
+            use = s >= ch.sa0 && s < ch.sa1;
+            col = 2 * ((s - ch.sa0) * K + li);
+        } else {
+            use = ch.sb == s;
+            col = 2 * (ns * K + li);
+        }
+    }
+    const unsigned long long m = __ballot(use);
+    if (lane == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    int base = 0, nrel = 0;
+    for (int w = 0; w < 4; ++w) {
+        if (w < wave) base += wcnt[w];
+        nrel += wcnt[w];
+    }
+    if (use) {
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        rel_ci[pos] = ci;
+        rel_col[pos] = col;
+    }
+    __syncthreads();
+    const int t0 = toff[key], nw = (toff[key + 1] - t0) * nrel;
     double c = 0.0, a = 0.0;
     for (int w = threadIdx.x; w < nw; w += 256) {
-        const int t = w / nchunk, ci = w - t * nchunk;
-        const int key = tile_key[t];
-        const OvlChunk ch = sh.ch[ci];
-        const double *pt = partial + static_cast<int64_t>(w) * pstride;
-        if (key / NZ == sg && s >= ch.sa0 && s < ch.sa1) {
-            c += pt[2 * ((s - ch.sa0) * K + li)];
-            a += pt[2 * ((s - ch.sa0) * K + li) + 1];
-        }
-        if (key % NZ == sg && s == ch.sb) {
-            c += pt[2 * (ns * K + li)];
-            a += pt[2 * (ns * K + li) + 1];
-        }
+        const int t = w / nrel, r = w - t * nrel;
+        const double *pt = partial + (static_cast<int64_t>(t0 + t) * nchunk + rel_ci[r]) * pstride + rel_col[r];
+        c += pt[0];
+        a += pt[1];
     }
-    rc[threadIdx.x] = c;
-    ra[threadIdx.x] = a;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (threadIdx.x < h) {
-            rc[threadIdx.x] += rc[threadIdx.x + h];
-            ra[threadIdx.x] += ra[threadIdx.x + h];
-        }
-        __syncthreads();
-    }
+    c = block_sum256(c, red);
+    a = block_sum256(a, red);
     if (threadIdx.x == 0) {
         const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
-        dc64[dst] = rc[0];
-        da64[dst] = ra[0];
+        part[blockIdx.y * npar + dst] = c;
+        part[(2 * NZ + blockIdx.y) * npar + dst] = a;
     }
 }
 
 // ---------------------------------------------------------------------------------
 // Output stages: cast, or normalisation VJP (overlap_2c_grad.jl:251-304) in FP64.
 // ---------------------------------------------------------------------------------
+// Parameter cotangent q summed over its nparts partial sums (stride pstride), in order.
+__device__ __forceinline__ double sum_parts(const double *__restrict__ p, int nparts, int64_t pstride, int64_t q)
+{
+    double v = 0.0;
+    if (p)
+        for (int i = 0; i < nparts; ++i) v += p[i * pstride + q];
+    return v;
+}
+
 template <typename T>
-__global__ void ovl2c_cast_out(int n, const double *__restrict__ a, const double *__restrict__ b, T *__restrict__ oa,
-                               T *__restrict__ ob)
+__global__ void ovl2c_cast_out(int n, const double *__restrict__ pc, const double *__restrict__ pa, int nparts,
+                               int64_t pstride, T *__restrict__ oa, T *__restrict__ ob)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    if (oa) oa[q] = static_cast<T>(a[q]);
-    if (ob) ob[q] = static_cast<T>(b[q]);
+    if (oa) oa[q] = static_cast<T>(sum_parts(pc, nparts, pstride, q));
+    if (ob) ob[q] = static_cast<T>(sum_parts(pa, nparts, pstride, q));
 }
 
-// dD, dzeta from (dcoef, dalpha): one thread per (species, shell).
+// dD, dzeta from (dcoef, dalpha) in FP64: one block per (species, shell).
+//   dD_m = dc_m / n - 2 c_l g_m s,  dzeta_m = da_m + 2 c_l (l + 3/2) D_m h_m s,
+//   s = (sum_i dc_i D_i) / (2 n^3)
 template <typename T>
-__global__ void ovl2c_vjp_out(const OvlShape sh, const double *__restrict__ dc64, const double *__restrict__ da64,
-                              const T *__restrict__ zetaT, const T *__restrict__ DT, T *__restrict__ dzeta,
-                              T *__restrict__ dD)
+__global__ __launch_bounds__(256) void ovl2c_vjp_out(const OvlShape sh, const double *__restrict__ pc,
+                                                     const double *__restrict__ pa, int nparts, int64_t pstride,
+                                                     const T *__restrict__ zetaT, const T *__restrict__ DT,
+                                                     T *__restrict__ dzeta, T *__restrict__ dD)
 {
-    const int ns = sh.ns, K = sh.K, NZ = sh.NZ;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= NZ * ns) return;
-    const int sg = idx / ns, s = idx - sg * ns, l = sh.ls[s];
-    double zv[OVL_MAXK], dv[OVL_MAXK];
-    int64_t q[OVL_MAXK];
-    for (int k = 0; k < K; ++k) {
-        q[k] = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
-        zv[k] = static_cast<double>(zetaT[q[k]]);
-        dv[k] = static_cast<double>(DT[q[k]]);
+    __shared__ ShellNorm S;
+    __shared__ double dcs[OVL_MAXK], das[OVL_MAXK];
+    __shared__ double s2n3;
+    const int ns = sh.ns, K = sh.K;
+    const int sg = blockIdx.x / ns, s = blockIdx.x - sg * ns, l = sh.ls[s];
+    const int64_t q0 = s + static_cast<int64_t>(ns) * K * sg;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const int64_t q = q0 + static_cast<int64_t>(ns) * k;
+        dcs[k] = sum_parts(pc, nparts, pstride, q);
+        das[k] = sum_parts(pa, nparts, pstride, q);
     }
-    const double nsq = cart_normsq(dv, zv, K, l);
+    shell_norm(zetaT, DT, q0, ns, K, l, true, S);
+    const double nsq = S.nsq;
     if (!(nsq > 0.0)) {
-        for (int k = 0; k < K; ++k) {
-            if (dD) dD[q[k]] = T(0);
-            if (dzeta) dzeta[q[k]] = static_cast<T>(da64[q[k]]);
+        for (int k = threadIdx.x; k < K; k += blockDim.x) {
+            const int64_t q = q0 + static_cast<int64_t>(ns) * k;
+            if (dD) dD[q] = T(0);
+            if (dzeta) dzeta[q] = static_cast<T>(das[k]);
         }
         return;
     }
     const double n = sqrt(nsq), c = cart_cl(l), e = l + 1.5;
-    double dot = 0.0;
-    for (int k = 0; k < K; ++k) dot += dc64[q[k]] * dv[k];
-    const double s2n3 = dot / (2.0 * n * n * n);
-    for (int m = 0; m < K; ++m) {
-        double gm = 0.0, hm = 0.0;
-        for (int j = 0; j < K; ++j) {
-            const double den = zv[m] + zv[j];
-            const double pe = inv_pow_l32(den, l);
-            gm += dv[j] * pe;
-            hm += dv[j] * pe / den;
-        }
-        gm *= 2.0 * c;
-        hm *= -2.0 * c * e * dv[m];
-        if (dD) dD[q[m]] = static_cast<T>(dc64[q[m]] / n - gm * s2n3);
-        if (dzeta) dzeta[q[m]] = static_cast<T>(da64[q[m]] - hm * s2n3);
+    if (threadIdx.x == 0) {
+        double dot = 0.0;
+        for (int k = 0; k < K; ++k) dot += dcs[k] * S.d[k];
+        s2n3 = dot / (2.0 * n * n * n);
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < K; m += blockDim.x) {
+        const int64_t q = q0 + static_cast<int64_t>(ns) * m;
+        const double gm = 2.0 * c * S.g[m];
+        const double hm = -2.0 * c * e * S.d[m] * S.h[m];
+        if (dD) dD[q] = static_cast<T>(dcs[m] / n - gm * s2n3);
+        if (dzeta) dzeta[q] = static_cast<T>(das[m] - hm * s2n3);
     }
 }
 
-// coef = D / sqrt(normsq) per (species, shell) in FP64 (cartesian_gto_basis.jl:110-120)
+// coef = D / sqrt(normsq) per (species, shell) in FP64 (cartesian_gto_basis.jl:110-120),
+// one block per (species, shell)
 template <typename T>
-__global__ void compile_coef_kernel(const OvlShape sh, const T *__restrict__ zetaT, const T *__restrict__ DT,
-                                    T *__restrict__ coef)
+__global__ __launch_bounds__(256) void compile_coef_kernel(const OvlShape sh, const T *__restrict__ zetaT,
+                                                           const T *__restrict__ DT, T *__restrict__ coef)
 {
-    const int ns = sh.ns, K = sh.K, NZ = sh.NZ;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= NZ * ns) return;
-    const int sg = idx / ns, s = idx - sg * ns;
-    double zv[OVL_MAXK], dv[OVL_MAXK];
-    int64_t q[OVL_MAXK];
-    for (int k = 0; k < K; ++k) {
-        q[k] = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
-        zv[k] = static_cast<double>(zetaT[q[k]]);
-        dv[k] = static_cast<double>(DT[q[k]]);
-    }
-    const double nsq = cart_normsq(dv, zv, K, sh.ls[s]);
-    for (int k = 0; k < K; ++k) coef[q[k]] = nsq > 0.0 ? static_cast<T>(dv[k] / sqrt(nsq)) : T(0);
+    __shared__ ShellNorm S;
+    const int ns = sh.ns, K = sh.K;
+    const int sg = blockIdx.x / ns, s = blockIdx.x - sg * ns;
+    const int64_t q0 = s + static_cast<int64_t>(ns) * K * sg;
+    shell_norm(zetaT, DT, q0, ns, K, sh.ls[s], false, S);
+    const double nsq = S.nsq;
+    for (int k = threadIdx.x; k < K; k += blockDim.x)
+        coef[q0 + static_cast<int64_t>(ns) * k] = nsq > 0.0 ? static_cast<T>(S.d[k] / sqrt(nsq)) : T(0);
 }
 
 // ---------------------------------------------------------------------------------
@@ -921,7 +990,7 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
                        sh.NZ, reinterpret_cast<int *>(ws + w.rank), reinterpret_cast<int *>(ws + w.blk_hist),
                        ctx->d_err);
     AOK_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL((ovl_plan_kernel<T>), dim3(1), dim3(256), 0, ctx->stream, sh, c.mode,
+    hipLaunchKernelGGL((ovl_plan_kernel<T>), dim3(1 + sh.NZ * sh.ns), dim3(256), 0, ctx->stream, sh, c.mode,
                        static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha), static_cast<const T *>(c.D),
                        reinterpret_cast<double *>(ws + w.coef64), reinterpret_cast<double *>(ws + w.alpha64),
                        reinterpret_cast<int *>(ws + w.nreal), reinterpret_cast<int *>(ws + w.nlist),
@@ -985,27 +1054,27 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
     const int nslot = sh.NZ * sh.ns * sh.K;
-    hipLaunchKernelGGL(ovl2c_bwd_finalize, dim3(nslot), dim3(256), 0, ctx->stream, sh,
-                       reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.toff),
-                       reinterpret_cast<const int *>(ws + w.pcls), reinterpret_cast<const double *>(ws + w.partial),
-                       reinterpret_cast<double *>(ws + w.dc64), reinterpret_cast<double *>(ws + w.da64));
+    hipLaunchKernelGGL(ovl2c_bwd_reduce, dim3(nslot, 2 * sh.NZ), dim3(256), 0, ctx->stream, sh,
+                       reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const int *>(ws + w.pcls),
+                       reinterpret_cast<const double *>(ws + w.partial), reinterpret_cast<double *>(ws + w.part2));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
 
+// (dcoef, dalpha) = sums of nparts partial arrays (stride pstride) -> the caller's outputs:
+// cast, or the normalisation VJP (overlap_2c_grad.jl:251-304) in FP64.
 template <typename T>
-static aok_status out_stage(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const double *dc64,
-                            const double *da64)
+static aok_status out_stage(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const double *pc, const double *pa,
+                            int nparts, int64_t pstride)
 {
     const int npar = sh.NZ * sh.ns * sh.K;
     if (c.vjp) {
-        const int nidx = sh.NZ * sh.ns;
-        hipLaunchKernelGGL((ovl2c_vjp_out<T>), dim3((nidx + 63) / 64), dim3(64), 0, ctx->stream, sh, dc64, da64,
-                           static_cast<const T *>(c.alpha), static_cast<const T *>(c.D), static_cast<T *>(c.o1),
-                           static_cast<T *>(c.o2));
+        hipLaunchKernelGGL((ovl2c_vjp_out<T>), dim3(sh.NZ * sh.ns), dim3(256), 0, ctx->stream, sh, pc, pa, nparts,
+                           pstride, static_cast<const T *>(c.alpha), static_cast<const T *>(c.D),
+                           static_cast<T *>(c.o1), static_cast<T *>(c.o2));
     } else {
-        hipLaunchKernelGGL((ovl2c_cast_out<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dc64,
-                           da64, static_cast<T *>(c.o1), static_cast<T *>(c.o2));
+        hipLaunchKernelGGL((ovl2c_cast_out<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, pc, pa,
+                           nparts, pstride, static_cast<T *>(c.o1), static_cast<T *>(c.o2));
     }
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
@@ -1025,7 +1094,7 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
     const T *coef = static_cast<const T *>(c.coef);
     if (c.mode == 1) {  // compile coef in FP64 into the workspace first
         T *cw = reinterpret_cast<T *>(ws);
-        hipLaunchKernelGGL((compile_coef_kernel<T>), dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh,
+        hipLaunchKernelGGL((compile_coef_kernel<T>), dim3(sh.NZ * sh.ns), dim3(256), 0, ctx->stream, sh,
                            static_cast<const T *>(c.alpha), static_cast<const T *>(c.D), cw);
         AOK_HIP(ctx, hipGetLastError());
         coef = cw;
@@ -1062,7 +1131,7 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
         hipLaunchKernelGGL((widen_kernel<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, dcoef,
                            dalpha, dc64, da64);
         AOK_HIP(ctx, hipGetLastError());
-        return out_stage<T>(ctx, c, sh, dc64, da64);
+        return out_stage<T>(ctx, c, sh, dc64, da64, 1, npar);
     }
     return AOK_OK;
 }
@@ -1096,23 +1165,18 @@ static aok_status fast_pullback(aok_ctx ctx, const Ovl2cCall &c, const OvlShape 
 {
     aok_status st = AOK_OK;
     const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
-    if (c.B == 0) {
-        hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
-                           reinterpret_cast<double *>(ws + w.dc64));
-        hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
-                           reinterpret_cast<double *>(ws + w.da64));
-        AOK_HIP(ctx, hipGetLastError());
-    } else {
-        switch (lmax_of(c.ls, c.ns)) {
-            case 0: st = fast_bwd_launch<T, 0>(ctx, c, sh, w, ws); break;
-            case 1: st = fast_bwd_launch<T, 1>(ctx, c, sh, w, ws); break;
-            case 2: st = fast_bwd_launch<T, 2>(ctx, c, sh, w, ws); break;
-            case 3: st = fast_bwd_launch<T, 3>(ctx, c, sh, w, ws); break;
-            default: st = fast_bwd_launch<T, 4>(ctx, c, sh, w, ws); break;
-        }
-        if (st != AOK_OK) return st;
+    if (c.B == 0)  // no pairs: zero cotangents (sums over no parts)
+        return out_stage<T>(ctx, c, sh, nullptr, nullptr, 0, npar);
+    switch (lmax_of(c.ls, c.ns)) {
+        case 0: st = fast_bwd_launch<T, 0>(ctx, c, sh, w, ws); break;
+        case 1: st = fast_bwd_launch<T, 1>(ctx, c, sh, w, ws); break;
+        case 2: st = fast_bwd_launch<T, 2>(ctx, c, sh, w, ws); break;
+        case 3: st = fast_bwd_launch<T, 3>(ctx, c, sh, w, ws); break;
+        default: st = fast_bwd_launch<T, 4>(ctx, c, sh, w, ws); break;
     }
-    return out_stage<T>(ctx, c, sh, reinterpret_cast<double *>(ws + w.dc64), reinterpret_cast<double *>(ws + w.da64));
+    if (st != AOK_OK) return st;
+    const double *part = reinterpret_cast<const double *>(ws + w.part2);
+    return out_stage<T>(ctx, c, sh, part, part + 2 * sh.NZ * npar, 2 * sh.NZ, npar);
 }
 
 template <typename T>
@@ -1184,7 +1248,6 @@ aok_status ovl2c_plan_exec(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall
     c.dS = io.dS;
     c.o1 = io.o1;
     c.o2 = io.o2;
-    c.vjp = io.vjp;
     if (c.flags & AOK_PLAIN) return plain_run<T>(ctx, c, p.sh, pullback);
     char *ws = static_cast<char *>(p.buf);
     return pullback ? fast_pullback<T>(ctx, c, p.sh, p.w, ws) : fast_forward<T>(ctx, c, p.sh, p.w, ws);
@@ -1202,7 +1265,7 @@ aok_status compile_coef_run(aok_ctx ctx, const void *zeta, const void *D, int64_
     OvlShape sh;
     const std::string msg = build_shape(ns, K, NZ, ls, 64, 1 << 30, sh);
     if (!msg.empty()) return fail(ctx, AOK_ERR_ARG, msg);
-    hipLaunchKernelGGL((compile_coef_kernel<T>), dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh,
+    hipLaunchKernelGGL((compile_coef_kernel<T>), dim3(sh.NZ * sh.ns), dim3(256), 0, ctx->stream, sh,
                        static_cast<const T *>(zeta), static_cast<const T *>(D), static_cast<T *>(coef));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
@@ -1216,16 +1279,14 @@ aok_status compile_coef_pb_run(aok_ctx ctx, const void *dcoef, const void *zeta,
     const std::string msg = build_shape(ns, K, NZ, ls, 64, 1 << 30, sh);
     if (!msg.empty()) return fail(ctx, AOK_ERR_ARG, msg);
     const int64_t npar = static_cast<int64_t>(NZ) * ns * K;
-    aok_status st = ensure_ws(ctx, 2 * align256(sizeof(double) * npar));
+    aok_status st = ensure_ws(ctx, align256(sizeof(double) * npar));
     if (st != AOK_OK) return st;
     double *dc64 = static_cast<double *>(ctx->ws);
-    double *zero64 = reinterpret_cast<double *>(static_cast<char *>(ctx->ws) + align256(sizeof(double) * npar));
-    hipLaunchKernelGGL((fill_zero<double>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar, zero64);
     hipLaunchKernelGGL((widen_kernel<T>), dim3((npar + 255) / 256), dim3(256), 0, ctx->stream, npar,
                        static_cast<const T *>(dcoef), static_cast<const T *>(dcoef), dc64, dc64);
-    hipLaunchKernelGGL((ovl2c_vjp_out<T>), dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh, dc64,
-                       zero64, static_cast<const T *>(zeta), static_cast<const T *>(D), static_cast<T *>(dzeta_norm),
-                       static_cast<T *>(dD));
+    hipLaunchKernelGGL((ovl2c_vjp_out<T>), dim3(sh.NZ * sh.ns), dim3(256), 0, ctx->stream, sh, dc64,
+                       static_cast<const double *>(nullptr), 1, npar, static_cast<const T *>(zeta),
+                       static_cast<const T *>(D), static_cast<T *>(dzeta_norm), static_cast<T *>(dD));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }

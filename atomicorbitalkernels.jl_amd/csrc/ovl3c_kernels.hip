@@ -101,22 +101,12 @@ __global__ __launch_bounds__(256) void ovl3_hist_kernel(const void *__restrict__
                                                         int *__restrict__ rank, int *__restrict__ blk_hist,
                                                         unsigned *__restrict__ err)
 {
-    __shared__ int keys[256];
-    __shared__ int hist[OVL3_MAXNZ * OVL3_MAXNZ * OVL3_MAXNZ];
+    __shared__ int whist[4 * OVL3_MAXNZ * OVL3_MAXNZ * OVL3_MAXNZ];
     const int NK = NZ * NZ * NZ;
-    for (int q = threadIdx.x; q < NK; q += 256) hist[q] = 0;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
     const int key = i < B ? key3(sA, sB, sC, itype, i, NZ, err) : -1;
-    keys[threadIdx.x] = key;
-    __syncthreads();
-    if (i < B) {
-        int r = 0;
-        for (int j = 0; j < static_cast<int>(threadIdx.x); ++j) r += (keys[j] == key);
-        rank[i] = r;
-        atomicAdd(&hist[key], 1);
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < NK; q += 256) blk_hist[static_cast<int64_t>(blockIdx.x) * NK + q] = hist[q];
+    const int r = block_rank(key, NK, whist, blk_hist + static_cast<int64_t>(blockIdx.x) * NK);
+    if (i < B) rank[i] = r;
 }
 
 // One block: real-primitive lists, bucket offsets, and the exclusive scan of the
@@ -145,7 +135,7 @@ __global__ __launch_bounds__(256) void ovl3_plan_kernel(const OvlShape sh, const
     }
     bucket_offsets(NK, nblk_h, blk_hist, blk_off, bcnt, boff, toff);
     // record offsets: contiguous ranges of groups per thread, then a scan of the partials
-    __shared__ int64_t part[256];
+    __shared__ int64_t wtot[4];
     const int64_t ngrp = static_cast<int64_t>(NK) * ns * ns * ns;
     const int64_t per = (ngrp + blockDim.x - 1) / blockDim.x;
     const int64_t g0 = per * threadIdx.x, g1 = min(ngrp, g0 + per);
@@ -156,21 +146,12 @@ __global__ __launch_bounds__(256) void ovl3_plan_kernel(const OvlShape sh, const
         const int a = key / (NZ * NZ), b = (key / NZ) % NZ, c = key % NZ;
         return static_cast<int64_t>(nreal[a * ns + sa]) * nreal[b * ns + sb] * nreal[c * ns + sc];
     };
+    __syncthreads();  // nreal of every (species, shell) is written
     int64_t s = 0;
     for (int64_t g = g0; g < g1; ++g) s += count(g);
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t run = 0;
-        for (int q = 0; q < static_cast<int>(blockDim.x); ++q) {
-            const int64_t v = part[q];
-            part[q] = run;
-            run += v;
-        }
-        grp_off[ngrp] = run;
-    }
-    __syncthreads();
-    s = part[threadIdx.x];
+    int64_t total = 0;
+    s = block_excl_scan64(s, wtot, total);
+    if (threadIdx.x == 0) grp_off[ngrp] = total;
     for (int64_t g = g0; g < g1; ++g) {
         grp_off[g] = s;
         s += count(g);

@@ -113,7 +113,7 @@ inline std::string build_shape(int64_t ns, int64_t K, int64_t NZ, const int64_t 
 struct Ovl2cWs {
     size_t coef64, alpha64, nreal, nlist, lidx, pcls, rec;
     size_t blk_hist, rank, blk_off, bcnt, boff, toff, perm, tile_key;
-    size_t partial, dc64, da64;
+    size_t partial, part2;
     size_t total;
     int nblk_h, NK, maxtiles;
 };
@@ -150,8 +150,7 @@ inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, int nchunk, size_t
     w.tile_key = take(sizeof(int) * static_cast<size_t>(w.maxtiles));
     if (pullback) {
         w.partial = take(sizeof(double) * static_cast<size_t>(w.maxtiles) * nchunk * (ns + 1) * K * 2);
-        w.dc64 = take(sizeof(double) * npar);
-        w.da64 = take(sizeof(double) * npar);
+        w.part2 = take(sizeof(double) * 4 * static_cast<size_t>(NZ) * npar);
     }
     w.total = o;
     return w;
@@ -193,6 +192,7 @@ struct Ovl3cCall {
 struct Ovl2cPlanState {
     aok_ctx ctx = nullptr;
     int dtype = 0;
+    int64_t ls[OVL_MAXSH] = {};  // host copy of the shell l's (call.ls points here)
     Ovl2cCall call{};
     OvlShape sh{};
     Ovl2cWs w{};

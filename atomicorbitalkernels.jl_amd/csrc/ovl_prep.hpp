@@ -6,52 +6,141 @@
 
 namespace aok {
 
-// Bucket offsets from per-block histograms, inside one 256-thread block: for every key a
-// block-wide exclusive scan over the hist blocks (thread t owns a contiguous run of
-// blocks), then the scan over keys; blk_off[b][key] = first output slot of block b's
-// elements with that key (stable order).
-__device__ inline void bucket_offsets(int NK, int nblk_h, const int *__restrict__ blk_hist, int *__restrict__ blk_off,
-                               int *__restrict__ bcnt, int *__restrict__ boff, int *__restrict__ toff)
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ int wave_incl_scan(int v)
 {
-    __shared__ int tsum[256];
-    __shared__ int sboff[OVL_MAXNZ * OVL_MAXNZ > 512 ? OVL_MAXNZ * OVL_MAXNZ : 512];
-    const int per = (nblk_h + 255) / 256;
-    const int b0 = threadIdx.x * per, b1 = min(nblk_h, b0 + per);
-    for (int key = 0; key < NK; ++key) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_incl_scan64(int64_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// Fixed-order sum of one double per thread over a 256-thread block (LDS tree); the
+// result is returned to every thread.  `red` holds 256 doubles.
+__device__ __forceinline__ double block_sum256(double v, double *red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Exclusive prefix sum over a block of whole waves; `wtot` holds one int64 per wave.
+// `total` receives the block total.
+__device__ inline int64_t block_excl_scan64(int64_t v, int64_t *wtot, int64_t &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int64_t incl = wave_incl_scan64(v);
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    int64_t base = 0, tot = 0;
+    for (int w = 0; w < nwave; ++w) {
+        if (w < wave) base += wtot[w];
+        tot += wtot[w];
+    }
+    total = tot;
+    __syncthreads();
+    return base + incl - v;
+}
+
+constexpr int OVL_MAXKEYS = 512;  // species pairs (16^2) or triples (8^3)
+
+// Bucket offsets from per-block histograms, inside one block of whole waves.  Every wave
+// takes keys w, w + nwave, ...; lane i owns a contiguous run of histogram blocks, and a
+// wave prefix sum gives each block's offset within the key.  The scan over keys then
+// gives boff (first element of each key's bucket), toff (first tile of each key, 64
+// elements per tile) and blk_off[b][key] = first output slot of block b's elements with
+// that key (stable order).
+__device__ inline void bucket_offsets(int NK, int nblk_h, const int *__restrict__ blk_hist, int *__restrict__ blk_off,
+                                      int *__restrict__ bcnt, int *__restrict__ boff, int *__restrict__ toff)
+{
+    __shared__ int s_cnt[OVL_MAXKEYS];
+    __shared__ int s_base[OVL_MAXKEYS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int per = (nblk_h + 63) / 64;
+    const int b0 = lane * per, b1 = min(nblk_h, b0 + per);
+    for (int key = wave; key < NK; key += nwave) {
         int s = 0;
         for (int b = b0; b < b1; ++b) s += blk_hist[static_cast<int64_t>(b) * NK + key];
-        tsum[threadIdx.x] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int run = 0;
-            for (int q = 0; q < 256; ++q) {
-                const int v = tsum[q];
-                tsum[q] = run;
-                run += v;
+        const int incl = wave_incl_scan(s);
+        if (lane == 63) s_cnt[key] = incl;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        int carry_o = 0, carry_t = 0;
+        for (int k0 = 0; k0 < NK; k0 += 64) {
+            const int key = k0 + lane;
+            const int c = key < NK ? s_cnt[key] : 0;
+            const int t = (c + OVL_TILE - 1) / OVL_TILE;
+            const int ic = wave_incl_scan(c), it = wave_incl_scan(t);
+            if (key < NK) {
+                const int o = carry_o + ic - c;
+                s_base[key] = o;
+                bcnt[key] = c;
+                boff[key] = o;
+                toff[key] = carry_t + it - t;
             }
-            bcnt[key] = run;
+            carry_o += __shfl(ic, 63, 64);
+            carry_t += __shfl(it, 63, 64);
         }
-        __syncthreads();
-        int run = tsum[threadIdx.x];
+        if (lane == 0) toff[NK] = carry_t;
+    }
+    __syncthreads();
+    for (int key = wave; key < NK; key += nwave) {
+        int s = 0;
+        for (int b = b0; b < b1; ++b) s += blk_hist[static_cast<int64_t>(b) * NK + key];
+        int run = s_base[key] + wave_incl_scan(s) - s;
         for (int b = b0; b < b1; ++b) {
             blk_off[static_cast<int64_t>(b) * NK + key] = run;
             run += blk_hist[static_cast<int64_t>(b) * NK + key];
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        int o = 0, t = 0;
-        for (int key = 0; key < NK; ++key) {
-            boff[key] = o;
-            sboff[key] = o;
-            toff[key] = t;
-            o += bcnt[key];
-            t += (bcnt[key] + OVL_TILE - 1) / OVL_TILE;
-        }
-        toff[NK] = t;
+}
+
+// Stable in-block ranks and per-block key histogram (256 threads, one element each;
+// key < 0 marks no element).  Lanes of one wave with equal keys are grouped by ballots
+// (one round per distinct key in the wave); earlier waves' counts come from LDS.
+// `whist` holds 4 * NK ints.  Returns the element's rank among equal keys of the block.
+__device__ inline int block_rank(int key, int NK, int *whist, int *__restrict__ blk_hist_row)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = threadIdx.x; q < 4 * NK; q += blockDim.x) whist[q] = 0;
+    __syncthreads();
+    unsigned long long todo = __ballot(key >= 0);
+    int r = 0;
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const unsigned long long m = __ballot(key == lk);
+        if (key == lk) r = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == leader) whist[wave * NK + lk] = __popcll(m);
+        todo &= ~m;
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < NK * nblk_h; q += blockDim.x) blk_off[q] += sboff[q % NK];
+    if (key >= 0)
+        for (int w = 0; w < wave; ++w) r += whist[w * NK + key];
+    for (int q = threadIdx.x; q < NK; q += blockDim.x)
+        blk_hist_row[q] = whist[q] + whist[NK + q] + whist[2 * NK + q] + whist[3 * NK + q];
+    return r;
 }
 
 }  // namespace aok

@@ -3,9 +3,9 @@
 Workload (the configuration the metric is quoted on): 2-centre Cartesian-Gaussian
 overlaps of B = 65536 atom pairs over 3 synthetic double-zeta-shaped species
 (9s4p1d -> 3s2p1d, N = 15 Cartesian functions, K = 9 primitive slots, exponents
-log-uniform in [0.1, 7000], mixed-sign general contractions), FP32.  One step = the
-forward overlap S (batch_overlap with ps) + the (zeta, D) pullback of a fixed
-cotangent dS (pullback_ps), both through libaok.so; inputs resident in HBM.
+log-uniform in [0.1, 7000], mixed-sign general contractions), FP32.  One step =
+rrule(batch_overlap) on (ps): the forward overlap S, then the (zeta, D) pullback of a
+fixed cotangent dS, sharing one plan, all through libaok.so; inputs resident in HBM.
 
 Also reported (extra keys): forward-only and pullback-only pairs/s, the plain-FP64
 GPU form (the reference decomposition) and the speedup over it, the FP32 error
@@ -123,10 +123,11 @@ def main():
     kern = {"ovl2c_fwd_fast": [], "ovl2c_bwd_fast": []}
 
     def step(record):
-        S = aok.batch_overlap(orb32, XA32, XB32, ps32, ctx=ctx)
+        # rrule(batch_overlap): forward S, then the pullback of dS to (zeta, D)
+        S, pb = aok.rrule_batch_overlap(orb32, XA32, XB32, ps32, ctx=ctx)
         if record:
             kern[ctx.last_kernel_name()].append(ctx.last_kernel_ms())
-        dps = aok.overlap_pullback_ps(dS32, orb32, XA32, XB32, ps32, ctx=ctx)
+        dps = pb(dS32)[4]
         if record:
             kern[ctx.last_kernel_name()].append(ctx.last_kernel_ms())
         return S, dps
@@ -210,6 +211,29 @@ def main():
             "fwd": tp["fwd"][1] / fwd_ms, "bwd": tp["bwd"][1] / bwd_ms,
             "fwd+bwd": (tp["fwd"][1] + tp["bwd"][1]) / (fwd_ms + bwd_ms),
             "step_wall": (tp["fwd"][0] + tp["bwd"][0]) / ms_step}
+        # HBM ceilings measured with torch's own copy and fill kernels (2 GiB buffers): the
+        # copy kernel is also the scale reference for the FETCH_SIZE / WRITE_SIZE counters
+        log("HBM copy / fill ceilings")
+        nb_cal = 2 << 30
+        src = torch.ones(nb_cal // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        for _ in range(3):
+            dst.copy_(src)
+            dst.fill_(0.5)
+        ev[0].record()
+        for _ in range(10):
+            dst.copy_(src)
+        ev[1].record()
+        ev[2].record()
+        for _ in range(10):
+            dst.fill_(0.5)
+        ev[3].record()
+        torch.cuda.synchronize()
+        out["hbm_ceilings"] = {"copy_GB_per_s": 2 * nb_cal * 10 / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e9,
+                               "fill_GB_per_s": nb_cal * 10 / (ev[2].elapsed_time(ev[3]) * 1e-3) / 1e9,
+                               "bytes": nb_cal, "kernels": "torch copy_ / fill_ (float32)"}
+        del src, dst
         # AO evaluation at nX = 2^24, 30 orbitals, 3 species, FP32 (HBM-bound)
         log("AO nX=2^24 evaluation")
         c4 = configs.c4b_basis()
@@ -236,6 +260,7 @@ def main():
         out["ao_c4b"] = {"nX": nX, "nOrb": len(orb4), "kernel_ms": kms, "wall_ms": wall, "kernel": "ao_eval",
                          "GB_per_s": bytes_alg / (kms * 1e-3) / 1e9,
                          "frac_of_hbm_peak": bytes_alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "frac_of_fill_ceiling": bytes_alg / (kms * 1e-3) / 1e9 / out["hbm_ceilings"]["fill_GB_per_s"],
                          "algorithmic_bytes": bytes_alg}
         del P, X
         # 3-centre overlaps, B = 2^14 triples over the same 3 species, FP32

@@ -169,6 +169,38 @@ aok_status aok_ovl2c_pullback_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, c
                                  const void *zeta, const void *D, double lengthscale,
                                  void *dzeta, void *dD);
 
+/* ---- H2: 2-centre plans (forward + pullback sharing one preparation) ----------------- */
+/* rrule(batch_overlap, orb, XA, XB, ps, st) (overlap_2c_grad.jl:309-318) runs the forward
+ * and, later, pullback_ps on the same batch and parameters; the reference recompiles the
+ * coefficients and re-walks the shell pairs in each.  A plan does the per-(sidx, params)
+ * work once — the FP64 coefficient compilation (_compiled_from_ps, :22-30), the
+ * species-pair bucketing of the batch and the primitive-pair records — and both calls
+ * reuse it.
+ *   params = AOK_PARAMS_COEF: (p1, p2) = (coef, alpha), pullback writes (dcoef, dalpha)
+ *            as aok_ovl2c_pullback;
+ *   params = AOK_PARAMS_PS:   (p1, p2) = (zeta, D), pullback writes (dzeta, dD) as
+ *            aok_ovl2c_pullback_ps.
+ * The plan references sidxA/sidxB/p1/p2 (device) until it is destroyed: they must stay
+ * valid and unchanged.  Positions are per call.  A plan is used on its context's stream
+ * and, like the context, by one host thread at a time.  Results are identical to the
+ * corresponding one-shot entry points. */
+#define AOK_PARAMS_COEF 0
+#define AOK_PARAMS_PS 1
+typedef struct aok_ovl2c_plan_s *aok_ovl2c_plan;
+
+aok_status aok_ovl2c_plan_create(aok_ctx ctx, aok_dtype dtype, unsigned flags,
+                                 const void *sidxA, const void *sidxB, aok_itype itype,
+                                 int64_t B, int64_t ns, int64_t K, int64_t NZ,
+                                 const int64_t *ls, int params, const void *p1,
+                                 const void *p2, double lengthscale, aok_ovl2c_plan *plan);
+/* Forward: out [N*N*B] as aok_ovl2c / aok_ovl2c_ps. */
+aok_status aok_ovl2c_plan_forward(aok_ovl2c_plan plan, const void *posA, const void *posB,
+                                  void *out);
+/* Pullback of dS [N*N*B]: (d1, d2) [ns*K*NZ] as described above. */
+aok_status aok_ovl2c_plan_pullback(aok_ovl2c_plan plan, const void *dS, const void *posA,
+                                   const void *posB, void *d1, void *d2);
+aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan);
+
 /* ---- H2: 3-centre overlap ------------------------------------------------------------ */
 /* Replaces batch_overlap_3c!(out, basis::CompiledBasis, posA, posB, posC)
  * (src/integrals/kernels_3c.jl:183-200, kernel batch_V_kernel! :12-162), generalised to

@@ -206,3 +206,73 @@ def test_c1_full_batch_fp32_vs_fp64(dev):
     Sr = O.ovl2c(rounded(d["posA"], torch.float32)[sub], rounded(d["posB"], torch.float32)[sub], d["sidxA"][sub],
                  d["sidxB"][sub], cb.ls, coef, rounded(cb.zeta, torch.float32))
     assert blockwise_rel(res[torch.float64][:, :, sub], Sr) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_plan_matches_one_shot(dev, dtype):
+    """aok_ovl2c_plan_*: forward and pullback equal the one-shot entry points bit for bit,
+    for both parameter forms, with new positions on a reused plan and with a second,
+    smaller plan that takes over the first plan's workspace."""
+    for B in (700, 129, 0):
+        d = _c1_small(max(B, 1))
+        cb = d["basis"]
+        log(f"2c plan B={B} dtype={dtype}")
+        orb = _orb(cb, dtype, dev)
+        ps = orb.static_params()
+        pA, pB = dev_rows(d["posA"][:B], dtype, dev), dev_rows(d["posB"][:B], dtype, dev)
+        sA = torch.as_tensor(d["sidxA"][:B], device=dev)
+        sB = torch.as_tensor(d["sidxB"][:B], device=dev)
+        dS = dev_f(np.random.default_rng(3).standard_normal((cb.N, cb.N, B)), dtype, dev)
+        S, pb = aok.rrule_batch_overlap(orb, (pA, sA), (pB, sB), ps, None)
+        S1 = aok.batch_overlap(orb, (pA, sA), (pB, sB), ps, None)
+        assert torch.equal(S, S1)
+        dps = pb(dS)[4]
+        dps1 = aok.overlap_pullback_ps(dS, orb, (pA, sA), (pB, sB), ps, None)
+        assert torch.equal(dps["Rnl"]["zeta"], dps1["Rnl"]["zeta"])
+        assert torch.equal(dps["Rnl"]["D"], dps1["Rnl"]["D"])
+        # coefficient form, positions changed between calls on the same plan
+        basis = aok.compile_basis(orb)
+        coef = aok.as_fortran(basis.coef.to(dtype))
+        alpha = aok.as_fortran(basis.zeta.to(dtype))
+        plan = aok.Ovl2cPlan(basis.ls, sA, sB, B, coef, alpha, aok._lib.AOK_PARAMS_COEF, basis.lengthscale, dtype)
+        for shift in (0.0, 0.37):
+            qA = (pA + shift).contiguous()
+            out = torch.empty((B, cb.N, cb.N), dtype=dtype, device=dev).permute(2, 1, 0)
+            plan.forward(qA, pB, out)
+            ref = aok.batch_overlap(basis, (qA, sA), (pB, sB))
+            assert torch.equal(out, ref)
+            dc, da = plan.pullback(aok.as_fortran(dS), qA, pB)
+            dcr, dar = aok.overlap_pb_coef_alpha(dS, basis, (qA, sA), (pB, sB))
+            assert torch.equal(dc, dcr) and torch.equal(da, dar)
+        plan.close()
+
+
+def test_timing_hooks_leave_no_error(dev):
+    """aok_set_timing: the dominant kernel's event time is reported per call, and calls
+    that launch no timed kernel (plan creation) report 0 and leave no pending HIP error."""
+    d = _c1_small(300)
+    cb = d["basis"]
+    log("2c timing hooks")
+    ctx = aok.Context(0)
+    ctx.set_timing(True)
+    orb = _orb(cb, torch.float32, dev)
+    ps = orb.static_params()
+    pA, pB = dev_rows(d["posA"], torch.float32, dev), dev_rows(d["posB"], torch.float32, dev)
+    sA = torch.as_tensor(d["sidxA"], device=dev)
+    sB = torch.as_tensor(d["sidxB"], device=dev)
+    S, pb = aok.rrule_batch_overlap(orb, (pA, sA), (pB, sB), ps, None, ctx=ctx)
+    assert ctx.last_kernel_name() == "ovl2c_fwd_fast" and ctx.last_kernel_ms() > 0
+    x = torch.ones(1000, device=dev) * 2  # a later torch launch sees no stale HIP error
+    torch.cuda.synchronize()
+    assert float(x.sum()) == 2000.0
+    pb(torch.ones_like(S))
+    assert ctx.last_kernel_name() == "ovl2c_bwd_fast" and ctx.last_kernel_ms() > 0
+    # destroying the context detaches the plan still held by pb: using it is an error,
+    # destroying it later is safe and leaves no HIP error behind
+    ctx.close()
+    with pytest.raises(aok.AOKArgumentError):
+        pb(torch.ones_like(S))
+    del pb
+    y = torch.ones(10, device=dev)
+    torch.cuda.synchronize()
+    assert float(y.sum()) == 10.0
