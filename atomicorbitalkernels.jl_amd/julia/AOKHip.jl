@@ -135,4 +135,67 @@ function pullback_ps(∂S::ROCArray{FT, 3}, orb::AOK.GaussianTypeOrbitals, XA::A
     return (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple())
 end
 
+# ---- H2 rrule: forward + pullback on one plan (overlap_2c_grad.jl:309-318) -------------
+import ChainRulesCore: rrule, NoTangent
+const AOK_PARAMS_PS = Cint(1)
+
+mutable struct Plan2c
+    h::Ptr{Cvoid}
+    keep::Any          # sidx / parameter arrays the plan references until destroyed
+end
+function _plan_destroy(p::Plan2c)
+    p.h == C_NULL || ccall((:aok_ovl2c_plan_destroy, libaok), Cint, (Ptr{Cvoid},), p.h)
+    p.h = C_NULL
+end
+
+function rrule(::typeof(AOK.batch_overlap), orb::AOK.GaussianTypeOrbitals, XA::AbstractVector,
+               XB::AbstractVector, ps, st)
+    str = AOK._compile_struct(orb)
+    FT = promote_type(eltype(ps.Rnl.ζ), eltype(ps.Rnl.D))
+    B = length(XA)
+    length(XB) == B || error("XA and XB must have equal length; got $(length(XA)) and $(length(XB))")
+    posA = ROCArray(AOK._cgto_posmat(XA, FT)); posB = ROCArray(AOK._cgto_posmat(XB, FT))
+    sA = ROCArray([AOK._z2i(orb.Rnl, x.S) for x in XA]); sB = ROCArray([AOK._z2i(orb.Rnl, x.S) for x in XB])
+    ζ = ROCArray{FT}(ps.Rnl.ζ); D = ROCArray{FT}(ps.Rnl.D)
+    ns, K, NZ = size(ζ)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:aok_ovl2c_plan_create, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Int64, Int64, Int64, Ptr{Int64},
+                Cint, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ref{Ptr{Cvoid}}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(sA), pointer(sB), AOK_I64, B, ns, K, NZ, _ip(str.ls),
+               AOK_PARAMS_PS, pointer(ζ), pointer(D), Float64(orb.lengthscale), h)
+    check(rc, "batch_overlap (plan)")
+    plan = finalizer(_plan_destroy, Plan2c(h[], (sA, sB, ζ, D)))
+    S = similar(posA, FT, str.nbf_total, str.nbf_total, B)
+    rc = ccall((:aok_ovl2c_plan_forward, libaok), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+               plan.h, pointer(posA), pointer(posB), pointer(S))
+    check(rc, "batch_overlap")
+    function batch_overlap_pullback(∂S)
+        ∂Sd = ROCArray{FT}(∂S); ∂ζ = similar(ζ); ∂D = similar(D)
+        rc = ccall((:aok_ovl2c_plan_pullback, libaok), Cint,
+                   (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                   plan.h, pointer(∂Sd), pointer(posA), pointer(posB), pointer(∂ζ), pointer(∂D))
+        check(rc, "pullback_ps")
+        return NoTangent(), NoTangent(), NoTangent(), NoTangent(), (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple()),
+               NoTangent()
+    end
+    return S, batch_overlap_pullback
+end
+
+# ---- H2 3-centre: batch_overlap_3c!(out, basis, posA, posB, posC) (kernels_3c.jl:183-200) --
+# A CompiledBasis without species maps onto one species (NZ = 1, sidx = NULL) whose
+# primitive slots are zero-padded to the longest contraction.
+function batch_overlap_3c!(out::ROCArray{FT, 4}, ls::Vector{Int}, coef::ROCArray{FT, 3}, α::ROCArray{FT, 3},
+                           posA::ROCMatrix{FT}, posB::ROCMatrix{FT}, posC::ROCMatrix{FT}) where {FT}
+    ns, K, NZ = size(coef)
+    B = size(posA, 2)
+    rc = ccall((:aok_ovl3c, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
+                Int64, Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(posA), pointer(posB), pointer(posC), C_NULL, C_NULL, C_NULL,
+               AOK_I64, B, ns, K, NZ, ls, pointer(coef), pointer(α), 1.0, pointer(out))
+    check(rc, "batch_overlap_3c!")
+    return out
+end
+
 end # module

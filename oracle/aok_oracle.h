@@ -15,6 +15,16 @@
  *   pos A/B/C  : double[3*B]
  *   out 2c     : double[N*N*B]           [r + N*(c + N*b)]
  *   out 3c     : double[N*N*N*B]
+ *
+ * Pins (tests/test_oracle.py): Gauss-Hermite quadrature golden vectors for every 2c
+ * shell pair up to (4,4) and 3c triples, composed with the reference's alias map;
+ * closed forms (s-s same centre, l <= 2 harmonics, self-overlap 1); 3c -> 2c reduction;
+ * finite differences of every gradient and pullback; addition theorem, homogeneity and
+ * orthonormality of the harmonics.
+ * Parity unpinned: orc_ylm's assignment of m to cos/sin and its sign convention within
+ * each l (SpheriCart.jl 0.3.1, not vendored).  The tests above fix everything except
+ * a signed permutation of the 2l+1 functions of each l; no reference test or fixture
+ * distinguishes those.
  */
 #ifndef AOK_ORACLE_H
 #define AOK_ORACLE_H
