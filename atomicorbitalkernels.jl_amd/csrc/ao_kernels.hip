@@ -37,23 +37,31 @@ __device__ __forceinline__ T a_to_zeta()
     return std::is_same<T, float>::value ? T(0.69314718055994530942) : T(1);
 }
 
+// (zeta*log2e or zeta, D) per (species, radial, primitive) at q = (sg*nRad + k)*K + m,
+// built once per call (FP64 products rounded once) and copied into each block's LDS.
 template <typename T>
-__device__ void load_param_table(ParamPair<T> *tab, const T *__restrict__ zeta,
-                                 const T *__restrict__ D, int nRad, int K, int NZ)
+__global__ void ao_param_kernel(const T *__restrict__ zeta, const T *__restrict__ D, int nRad, int K, int NZ,
+                                ParamPair<T> *__restrict__ tab)
 {
     const int n = NZ * nRad * K;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int sg = q / (nRad * K);
-        const int rem = q - sg * nRad * K;
-        const int k = rem / K;
-        const int m = rem - k * K;
-        const int64_t src = k + static_cast<int64_t>(nRad) * (m + static_cast<int64_t>(K) * sg);
-        const double z = static_cast<double>(zeta[src]);
-        ParamPair<T> pp;
-        pp.a = std::is_same<T, float>::value ? T(z * 1.4426950408889634) : T(z);
-        pp.d = D[src];
-        tab[q] = pp;
-    }
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int sg = q / (nRad * K);
+    const int rem = q - sg * nRad * K;
+    const int k = rem / K;
+    const int m = rem - k * K;
+    const int64_t src = k + static_cast<int64_t>(nRad) * (m + static_cast<int64_t>(K) * sg);
+    const double z = static_cast<double>(zeta[src]);
+    ParamPair<T> pp;
+    pp.a = std::is_same<T, float>::value ? T(z * 1.4426950408889634) : T(z);
+    pp.d = D[src];
+    tab[q] = pp;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_param_table(ParamPair<T> *tab, const ParamPair<T> *__restrict__ gtab, int n)
+{
+    for (int q = threadIdx.x; q < n; q += blockDim.x) tab[q] = gtab[q];
 }
 
 template <typename T>
@@ -128,14 +136,14 @@ __device__ __forceinline__ T pick(const T *v, int y)
 
 template <typename T, int L, bool GRAD, bool GENERIC>
 __global__ __launch_bounds__(256) void ao_eval_kernel(const T *__restrict__ X, const void *__restrict__ sidx,
-                                                      int itype, int64_t nX, const T *__restrict__ zeta,
-                                                      const T *__restrict__ D, int nRad, int K, int NZ, T scale,
+                                                      int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
+                                                       int nRad, int K, int NZ, T scale,
                                                       int decay, AoRunTable runs, T *__restrict__ P,
                                                       T *__restrict__ dP, unsigned *__restrict__ err)
 {
     extern __shared__ unsigned char smem_raw[];
     ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw);
-    load_param_table(tab, zeta, D, nRad, K, NZ);
+    load_param_table(tab, gtab, NZ * nRad * K);
     __syncthreads();
     constexpr int NY = (L + 1) * (L + 1);
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nX;
@@ -220,14 +228,14 @@ __device__ __forceinline__ void store_shell_vec(const T (*Z)[(L + 1) * (L + 1)],
 
 template <typename T, int L, int PPL>
 __global__ __launch_bounds__(256) void ao_eval_vec_kernel(const T *__restrict__ X, const void *__restrict__ sidx,
-                                                          int itype, int64_t nX, const T *__restrict__ zeta,
-                                                          const T *__restrict__ D, int nRad, int K, int NZ, T scale,
+                                                          int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
+                                                           int nRad, int K, int NZ, T scale,
                                                           int decay, AoRunTable runs, T *__restrict__ P,
                                                           unsigned *__restrict__ err)
 {
     extern __shared__ unsigned char smem_raw[];
     ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw);
-    load_param_table(tab, zeta, D, nRad, K, NZ);
+    load_param_table(tab, gtab, NZ * nRad * K);
     __syncthreads();
     constexpr int NY = (L + 1) * (L + 1);
     const int64_t ntile = (nX + 256 * PPL - 1) / (256 * PPL);
@@ -308,8 +316,8 @@ constexpr int AO_PB_NZG = 4;      // species handled per pass
 
 template <typename T, int L, bool GENERIC>
 __global__ __launch_bounds__(AO_PB_THREADS) void ao_pullback_kernel(
-    const T *__restrict__ X, const void *__restrict__ sidx, int itype, int64_t nX, const T *__restrict__ zeta,
-    const T *__restrict__ D, int nRad, int K, int NZ, T scale, int decay, AoRunTable runs,
+    const T *__restrict__ X, const void *__restrict__ sidx, int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
+     int nRad, int K, int NZ, T scale, int decay, AoRunTable runs,
     const T *__restrict__ dPin, T *__restrict__ dX, double *__restrict__ partial, int want_params,
     unsigned *__restrict__ err)
 {
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(AO_PB_THREADS) void ao_pullback_kernel(
     off += sizeof(T) * TILE;
     int *sgb = reinterpret_cast<int *>(smem_raw + off);
 
-    load_param_table(tab, zeta, D, nRad, K, NZ);
+    load_param_table(tab, gtab, NZ * nRad * K);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     double *myacc = wacc + wave * nslot;
     for (int q = lane; q < nslot; q += 64) myacc[q] = 0.0;
@@ -488,37 +496,56 @@ __global__ void ao_pullback_finalize(const double *__restrict__ partial, int nbl
 // ---------------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------------
+// The per-call parameter table, at byte offset `at` of the context workspace.
+template <typename T>
+static aok_status param_table(aok_ctx ctx, const AoArgs &a, size_t at, const ParamPair<T> *&gtab)
+{
+    const int n = a.NZ * a.nRad * a.K;
+    aok_status st = ensure_ws(ctx, at + sizeof(ParamPair<T>) * n);
+    if (st != AOK_OK) return st;
+    ParamPair<T> *t = reinterpret_cast<ParamPair<T> *>(static_cast<char *>(ctx->ws) + at);
+    hipLaunchKernelGGL((ao_param_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                       static_cast<const T *>(a.zeta), static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, t);
+    AOK_HIP(ctx, hipGetLastError());
+    gtab = t;
+    return AOK_OK;
+}
+
 template <typename T, int L, bool GENERIC>
 static aok_status launch_eval_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &runs)
 {
     const size_t shm = sizeof(ParamPair<T>) * a.NZ * a.nRad * a.K;
     if (a.nX <= 0) return AOK_OK;
-    // tuning: points per lane (values-only, canonical maps) and the persistent grid size
-    int ppl = ctx->tune.ao_ppl > 0 ? ctx->tune.ao_ppl : (std::is_same<T, float>::value ? 4 : 2);
+    const ParamPair<T> *gtab = nullptr;
+    aok_status st = param_table<T>(ctx, a, 0, gtab);
+    if (st != AOK_OK) return st;
+    // launch shape: points per lane (values-only, canonical maps) and the grid cap.  The
+    // defaults (one point per lane, one 256-point tile per block) measured best on MI355X.
+    int ppl = ctx->tune.ao_ppl > 0 ? ctx->tune.ao_ppl : 1;
     if (a.dP || GENERIC || a.nX % ppl != 0 || (ppl != 2 && ppl != 4)) ppl = 1;
     const int64_t ntile = (a.nX + 256 * ppl - 1) / (256 * ppl);
-    const int cap = ctx->tune.ao_grid > 0 ? ctx->tune.ao_grid : 4096;
+    const int cap = ctx->tune.ao_grid > 0 ? ctx->tune.ao_grid : (1 << 20);
     const int grid = static_cast<int>(std::min<int64_t>(ntile, cap));
     time_begin(ctx, a.dP ? "ao_eval_ed" : "ao_eval");
     if (a.dP)
         hipLaunchKernelGGL((ao_eval_kernel<T, L, true, GENERIC>), dim3(grid), dim3(256), shm, ctx->stream,
-                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
-                           static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, gtab,
+                            a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
                            static_cast<T *>(a.P), static_cast<T *>(a.dP), ctx->d_err);
     else if (ppl == 4)
         hipLaunchKernelGGL((ao_eval_vec_kernel<T, L, 4>), dim3(grid), dim3(256), shm, ctx->stream,
-                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
-                           static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, gtab,
+                            a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
                            static_cast<T *>(a.P), ctx->d_err);
     else if (ppl == 2)
         hipLaunchKernelGGL((ao_eval_vec_kernel<T, L, 2>), dim3(grid), dim3(256), shm, ctx->stream,
-                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
-                           static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, gtab,
+                            a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
                            static_cast<T *>(a.P), ctx->d_err);
     else
         hipLaunchKernelGGL((ao_eval_kernel<T, L, false, GENERIC>), dim3(grid), dim3(256), shm, ctx->stream,
-                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
-                           static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+                           static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, gtab,
+                            a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
                            static_cast<T *>(a.P), nullptr, ctx->d_err);
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
@@ -585,8 +612,9 @@ static aok_status launch_pb_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &ru
     const size_t shm = ao_pb_shmem<T>(a);
     if (shm > 160 * 1024)
         return fail(ctx, AOK_ERR_UNSUPPORTED, "ao_pullback: basis too large for the LDS tile");
-    const size_t part_bytes = sizeof(double) * static_cast<size_t>(ntab) * 2 * grid;
-    aok_status st = ensure_ws(ctx, part_bytes);
+    const size_t part_bytes = align256(sizeof(double) * static_cast<size_t>(ntab) * 2 * grid);
+    const ParamPair<T> *gtab = nullptr;
+    aok_status st = param_table<T>(ctx, a, part_bytes, gtab);
     if (st != AOK_OK) return st;
     double *partial = static_cast<double *>(ctx->ws);
     if (shm > 64 * 1024)
@@ -594,8 +622,8 @@ static aok_status launch_pb_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &ru
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
     time_begin(ctx, "ao_pullback");
     hipLaunchKernelGGL((ao_pullback_kernel<T, L, GENERIC>), dim3(grid), dim3(AO_PB_THREADS), shm, ctx->stream,
-                       static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, static_cast<const T *>(a.zeta),
-                       static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+                       static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, gtab,
+                        a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
                        static_cast<const T *>(a.dPin), static_cast<T *>(dX), partial, want ? 1 : 0, ctx->d_err);
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());

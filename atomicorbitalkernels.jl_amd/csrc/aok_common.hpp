@@ -50,6 +50,8 @@ namespace aok {
 
 constexpr unsigned ERR_SPECIES = 1u;
 
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
 // Status helpers ---------------------------------------------------------------
 inline aok_status fail(aok_ctx ctx, aok_status st, const std::string &msg)
 {

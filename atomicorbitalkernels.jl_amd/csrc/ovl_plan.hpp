@@ -118,7 +118,6 @@ struct Ovl2cWs {
     int nblk_h, NK, maxtiles;
 };
 
-inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, int nchunk, size_t rec_bytes, bool pullback)
 {

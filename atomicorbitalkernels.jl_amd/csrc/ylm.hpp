@@ -77,7 +77,8 @@ __device__ __forceinline__ void ylm_column(const T x, const T y, const T z, cons
     if constexpr (LL <= L) {
         T P, Pz = T(0), Pr = T(0);
         if constexpr (LL == M) {
-            P = T(ylm_diag(M));
+            constexpr T d = T(ylm_diag(M));  // constant-evaluated: no run-time square roots
+            P = d;
         } else {
             constexpr T a = T(ylm_A(LL, M));
             constexpr T b = T(ylm_B(LL, M));

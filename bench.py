@@ -262,7 +262,31 @@ def main():
                          "frac_of_hbm_peak": bytes_alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "frac_of_fill_ceiling": bytes_alg / (kms * 1e-3) / 1e9 / out["hbm_ceilings"]["fill_GB_per_s"],
                          "algorithmic_bytes": bytes_alg}
-        del P, X
+        # evaluate_ed and the rrule pullback (dX, dzeta, dD) at the same size
+        gx2 = torch.Generator(device=dev).manual_seed(5)
+        dPin = torch.randn((len(orb4), nX), generator=gx2, device=dev, dtype=torch.float32).t()
+        for name, fn, nbytes in (
+                ("ao_c4b_eval_ed", lambda: aok.evaluate_ed(orb4, X, sidx=sidx, ctx=ctx),
+                 nX * (12 + 4 + 16 * len(orb4))),
+                ("ao_c4b_pullback", lambda: aok.rrule_evaluate(orb4, X, sidx=sidx, ctx=ctx)[1](dPin),
+                 nX * (12 + 4 + 4 * len(orb4) + 12))):
+            log(name)
+            fn()
+            torch.cuda.synchronize()
+            ks = []
+            ts = time.perf_counter()
+            for _ in range(5):
+                fn()
+                if ctx.last_kernel_name() in ("ao_eval_ed", "ao_pullback"):
+                    ks.append(ctx.last_kernel_ms())
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - ts) * 1e3 / 5
+            kms = float(np.mean(ks))
+            out[name] = {"kernel": ctx.last_kernel_name(), "kernel_ms": kms, "wall_ms": wall,
+                         "GB_per_s": nbytes / (kms * 1e-3) / 1e9,
+                         "frac_of_hbm_peak": nbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes": nbytes}
+        del P, X, dPin
         # 3-centre overlaps, B = 2^14 triples over the same 3 species, FP32
         log("3c B=2^14")
         c4 = configs.c4a()
