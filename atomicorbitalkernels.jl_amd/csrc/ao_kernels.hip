@@ -160,9 +160,10 @@ __global__ __launch_bounds__(256) void ao_eval_kernel(const T *__restrict__ X, c
         T Z[NY], Zx[GRAD ? NY : 1], Zy[GRAD ? NY : 1], Zz[GRAD ? NY : 1];
         solid_harmonics<L, GRAD, T>(x, y, z, YlmOut<T>{Z, Zx, Zy, Zz});
         for (int q = 0; q < runs.nrun; ++q) {
-            const int k = runs.k[q], lk = runs.l[q], o0 = runs.o0[q];
+            const uint32_t rw = runs.run[q];
+            const int k = run_k(rw), lk = run_l(rw), o0 = run_o0(rw);
             T R, G = T(0);
-            radial<T, GRAD>(tab + (sg * nRad + k) * K, K, f, r, runs.poly[q], decay, R, G);
+            radial<T, GRAD>(tab + (sg * nRad + k) * K, K, f, r, run_poly(runs, k), decay, R, G);
             switch (lk) {
                 case 0: store_shell<0, L, GRAD>(Z, Zx, Zy, Zz, R, G, x, y, z, scale, i, nX, o0, P, dP); break;
                 case 1: store_shell<1, L, GRAD>(Z, Zx, Zy, Zz, R, G, x, y, z, scale, i, nX, o0, P, dP); break;
@@ -260,12 +261,13 @@ __global__ __launch_bounds__(256) void ao_eval_vec_kernel(const T *__restrict__ 
             solid_harmonics<L, false, T>(x, y, z, YlmOut<T>{Z[p], nullptr, nullptr, nullptr});
         }
         for (int q = 0; q < runs.nrun; ++q) {
-            const int k = runs.k[q], lk = runs.l[q], o0 = runs.o0[q];
+            const uint32_t rw = runs.run[q];
+            const int k = run_k(rw), lk = run_l(rw), o0 = run_o0(rw);
             T R[PPL];
 #pragma unroll
             for (int p = 0; p < PPL; ++p) {
                 T G;
-                radial<T, false>(tab + (sg[p] * nRad + k) * K, K, f[p], r[p], runs.poly[q], decay, R[p], G);
+                radial<T, false>(tab + (sg[p] * nRad + k) * K, K, f[p], r[p], run_poly(runs, k), decay, R[p], G);
             }
             switch (lk) {
                 case 0: store_shell_vec<0, L, PPL>(Z, R, i0, nX, o0, P); break;
@@ -279,11 +281,11 @@ __global__ __launch_bounds__(256) void ao_eval_vec_kernel(const T *__restrict__ 
     }
 }
 
-// Pullback contraction of one full shell run: dr += sum_j w_j Z_j, g += w_j grad P_j.
+// Pullback contraction of one full shell run: dr += sum_j w_j Z_j and W_y += w_j R for
+// its harmonics y (the spatial gradient is contracted once per point from W).
 template <int LL, int L, typename T>
-__device__ __forceinline__ void contract_shell(const T *Z, const T *Zx, const T *Zy, const T *Zz, T R, T G, T x,
-                                               T y, T z, int64_t i, int64_t nX, int o0,
-                                               const T *__restrict__ dPin, T &dr, T &gx, T &gy, T &gz)
+__device__ __forceinline__ void contract_shell(const T *Z, T *W, T R, int64_t i, int64_t nX, int o0,
+                                               const T *__restrict__ dPin, T &dr)
 {
     if constexpr (LL <= L) {
 #pragma unroll
@@ -291,206 +293,241 @@ __device__ __forceinline__ void contract_shell(const T *Z, const T *Zx, const T 
             const int yy = LL * LL + j;
             const T w = dPin[i + nX * (o0 + j)];
             dr += w * Z[yy];
-            const T wg = w * G * Z[yy];
-            const T wr = w * R;
-            gx += wg * x + wr * Zx[yy];
-            gy += wg * y + wr * Zy[yy];
-            gz += wg * z + wr * Zz[yy];
+            W[yy] += w * R;
         }
     }
 }
 
 // ---------------------------------------------------------------------------------
-// Pullback.  Phase A (one lane per point): dX = s * sum_o dP_io grad P_io and
-// dR_k = sum_{o in k} dP_io Z_io, staged per lane in LDS.  Phase B: for every radial k
-// and primitive m, the per-species sums dD[k,m,sg] += dR_k r^p e and
-// dzeta[k,m,sg] -= dR_k r^p D f e are accumulated in registers with a species mask,
-// reduced over the wave and added into per-wave FP64 LDS accumulators; the block
-// writes one FP64 partial per slot and a second kernel sums the partials in a fixed
-// order (deterministic, no float atomics).
+// Pullback, one 256-point tile per block iteration.
+//  Phase A (one lane per point): dX = s * sum_o dP_io grad P_io, and dR_k =
+//    sum_{o in k} dP_io Z_io into an LDS row; rows are ordered by species (the point's
+//    rank among the tile's points of its species, by wave ballots).
+//  Phase B (one lane per parameter item): item = (sub, species, k, m); its lane walks
+//    its share of that species' rows and accumulates
+//      dD[k,m,sg] += dR_k r^p e,   dzeta[k,m,sg] -= dR_k r^p D f e,   e = exp(-zeta f)
+//    in FP32 over the tile, then adds the tile sum into its own FP64 LDS accumulator.
+//  No cross-lane reductions and no atomics: every sum has a fixed order.  The block
+//  writes one FP64 partial per item and ao_pullback_finalize sums them (block per slot,
+//  fixed LDS tree).
 // ---------------------------------------------------------------------------------
 constexpr int AO_PB_THREADS = 256;
-constexpr int AO_PB_PPT = 2;      // points per thread per tile
-constexpr int AO_PB_KC = 8;       // primitive chunk held in registers
-constexpr int AO_PB_NZG = 4;      // species handled per pass
+
+struct AoPbLayout {
+    int nslot, nsub, nitem, nRadP;
+    size_t tab, dr, f, cnt, acc, total;
+};
+
+// LDS plan of one pullback block.
+template <typename T>
+__host__ __device__ inline AoPbLayout ao_pb_layout(int nRad, int K, int NZ)
+{
+    AoPbLayout y;
+    y.nslot = NZ * nRad * K;
+    y.nsub = y.nslot >= AO_PB_THREADS ? 1 : AO_PB_THREADS / y.nslot;
+    y.nitem = y.nslot * y.nsub;
+    y.nRadP = nRad | 1;  // odd row stride: rows of different points start in different banks
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = (o + bytes + 15) & ~size_t(15);
+        return at;
+    };
+    y.tab = take(sizeof(ParamPair<T>) * y.nslot);
+    y.dr = take(sizeof(T) * AO_PB_THREADS * y.nRadP);
+    y.f = take(sizeof(T) * AO_PB_THREADS);
+    y.cnt = take(sizeof(int) * (4 * NZ + NZ + 1));
+    y.acc = take(sizeof(double) * 2 * y.nitem);
+    y.total = o;
+    return y;
+}
 
 template <typename T, int L, bool GENERIC>
 __global__ __launch_bounds__(AO_PB_THREADS) void ao_pullback_kernel(
     const T *__restrict__ X, const void *__restrict__ sidx, int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
-     int nRad, int K, int NZ, T scale, int decay, AoRunTable runs,
-    const T *__restrict__ dPin, T *__restrict__ dX, double *__restrict__ partial, int want_params,
-    unsigned *__restrict__ err)
+    int nRad, int K, int NZ, T scale, int decay, AoRunTable runs, const T *__restrict__ dPin, T *__restrict__ dX,
+    double *__restrict__ partial, int want_params, unsigned *__restrict__ err)
 {
-    constexpr int TILE = AO_PB_THREADS * AO_PB_PPT;
     constexpr int NY = (L + 1) * (L + 1);
+    constexpr int TILE = AO_PB_THREADS;
     extern __shared__ unsigned char smem_raw[];
-    // LDS layout: param table | wave accumulators (double) | dR[nRad][TILE] | f[TILE] | sg[TILE]
-    ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw);
-    const int ntab = NZ * nRad * K;
-    const int nslot = ntab * 2;
-    size_t off = (sizeof(ParamPair<T>) * ntab + 15) & ~size_t(15);
-    double *wacc = reinterpret_cast<double *>(smem_raw + off);
-    off += sizeof(double) * nslot * (AO_PB_THREADS / 64);
-    T *dRb = reinterpret_cast<T *>(smem_raw + off);
-    off += sizeof(T) * nRad * TILE;
-    T *fb = reinterpret_cast<T *>(smem_raw + off);
-    off += sizeof(T) * TILE;
-    int *sgb = reinterpret_cast<int *>(smem_raw + off);
+    const AoPbLayout y = ao_pb_layout<T>(nRad, K, NZ);
+    ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw + y.tab);
+    T *dru = reinterpret_cast<T *>(smem_raw + y.dr);  // [row][nRadP], rows in species order
+    T *fu = reinterpret_cast<T *>(smem_raw + y.f);
+    int *whist = reinterpret_cast<int *>(smem_raw + y.cnt);  // [4 waves][NZ]
+    int *soff = whist + 4 * NZ;                              // [NZ + 1]
+    double *acc = reinterpret_cast<double *>(smem_raw + y.acc);
 
-    load_param_table(tab, gtab, NZ * nRad * K);
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    double *myacc = wacc + wave * nslot;
-    for (int q = lane; q < nslot; q += 64) myacc[q] = 0.0;
+    load_param_table(tab, gtab, y.nslot);
+    for (int q = threadIdx.x; q < 2 * y.nitem; q += AO_PB_THREADS) acc[q] = 0.0;
     __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
     const int64_t ntiles = (nX + TILE - 1) / TILE;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        // ---- phase A ----
-#pragma unroll
-        for (int p = 0; p < AO_PB_PPT; ++p) {
-            const int slot = p * AO_PB_THREADS + threadIdx.x;
-            const int64_t i = t * TILE + slot;
-            for (int k = 0; k < nRad; ++k) dRb[k * TILE + slot] = T(0);
-            if (i >= nX) {
-                sgb[slot] = -1;
-                fb[slot] = T(0);
-                continue;
-            }
-            const T x = scale * X[3 * i], y = scale * X[3 * i + 1], z = scale * X[3 * i + 2];
-            int sg = load_sigma(sidx, itype, i);
+        const int64_t i = t * TILE + threadIdx.x;
+        int sg = -1;
+        if (i < nX) {
+            sg = load_sigma(sidx, itype, i);
             if (sg < 0 || sg >= NZ) {
                 *err = ERR_SPECIES;
                 sg = 0;
             }
-            const T r2 = x * x + y * y + z * z;
+        }
+        // row of this point: its position in the tile's species-ordered point list
+        // (rank among the wave's points of the same species by ballots, plus the counts
+        // of earlier species and waves), so phase B reads each species' rows contiguously
+        int pos = threadIdx.x;
+        if (want_params) {
+            for (int q = threadIdx.x; q < 4 * NZ; q += AO_PB_THREADS) whist[q] = 0;
+            __syncthreads();
+            unsigned long long todo = __ballot(sg >= 0);
+            int rk = 0;
+            while (todo) {
+                const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+                const int ls = __shfl(sg, leader, 64);
+                const unsigned long long m = __ballot(sg == ls);
+                if (sg == ls) rk = __popcll(m & ((1ull << lane) - 1ull));
+                if (lane == leader) whist[wave * NZ + ls] = __popcll(m);
+                todo &= ~m;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int o = 0;
+                for (int g = 0; g < NZ; ++g) {
+                    soff[g] = o;
+                    o += whist[g] + whist[NZ + g] + whist[2 * NZ + g] + whist[3 * NZ + g];
+                }
+                soff[NZ] = o;
+            }
+            __syncthreads();
+            if (sg >= 0) {
+                pos = soff[sg] + rk;
+                for (int w = 0; w < wave; ++w) pos += whist[w * NZ + sg];
+            }
+        }
+        // ---- phase A ----
+        if (i < nX) {
+            T *myrow = dru + pos * y.nRadP;
+            for (int k = 0; k < nRad; ++k) myrow[k] = T(0);
+            const T x = scale * X[3 * i], y3 = scale * X[3 * i + 1], z = scale * X[3 * i + 2];
+            const T r2 = x * x + y3 * y3 + z * z;
             const T r = sqrt(r2);
             const T f = (decay == AOK_DECAY_GAUSSIAN) ? r2 : r;
-            T Z[NY], Zx[NY], Zy[NY], Zz[NY];
-            solid_harmonics<L, true, T>(x, y, z, YlmOut<T>{Z, Zx, Zy, Zz});
-            T gx = T(0), gy = T(0), gz = T(0);
+            // dX = s (sum_k G_k dR_k) x + s sum_y W_y grad Z_y,  W_y = sum_{o: iY[o]=y} dP_io R_iR[o]
+            T Z[NY], W[NY];
+            solid_harmonics<L, false, T>(x, y3, z, YlmOut<T>{Z, nullptr, nullptr, nullptr});
+#pragma unroll
+            for (int q = 0; q < NY; ++q) W[q] = T(0);
+            T sgd = T(0);
             for (int q = 0; q < runs.nrun; ++q) {
-                const int k = runs.k[q], lk = runs.l[q], o0 = runs.o0[q];
+                const uint32_t rw = runs.run[q];
+            const int k = run_k(rw), lk = run_l(rw), o0 = run_o0(rw);
                 T R, G;
-                radial<T, true>(tab + (sg * nRad + k) * K, K, f, r, runs.poly[q], decay, R, G);
+                radial<T, true>(tab + (sg * nRad + k) * K, K, f, r, run_poly(runs, k), decay, R, G);
                 T dr = T(0);
                 switch (lk) {
-                    case 0: contract_shell<0, L>(Z, Zx, Zy, Zz, R, G, x, y, z, i, nX, o0, dPin, dr, gx, gy, gz); break;
-                    case 1: contract_shell<1, L>(Z, Zx, Zy, Zz, R, G, x, y, z, i, nX, o0, dPin, dr, gx, gy, gz); break;
-                    case 2: contract_shell<2, L>(Z, Zx, Zy, Zz, R, G, x, y, z, i, nX, o0, dPin, dr, gx, gy, gz); break;
-                    case 3: contract_shell<3, L>(Z, Zx, Zy, Zz, R, G, x, y, z, i, nX, o0, dPin, dr, gx, gy, gz); break;
-                    case 4: contract_shell<4, L>(Z, Zx, Zy, Zz, R, G, x, y, z, i, nX, o0, dPin, dr, gx, gy, gz); break;
+                    case 0: contract_shell<0, L>(Z, W, R, i, nX, o0, dPin, dr); break;
+                    case 1: contract_shell<1, L>(Z, W, R, i, nX, o0, dPin, dr); break;
+                    case 2: contract_shell<2, L>(Z, W, R, i, nX, o0, dPin, dr); break;
+                    case 3: contract_shell<3, L>(Z, W, R, i, nX, o0, dPin, dr); break;
+                    case 4: contract_shell<4, L>(Z, W, R, i, nX, o0, dPin, dr); break;
                     default:
                         if constexpr (GENERIC) {
-                        const int yy = -1 - lk;
-                        const T w = dPin[i + nX * o0];
-                        const T zv = pick<NY>(Z, yy);
-                        dr += w * zv;
-                        const T wg = w * G * zv;
-                        gx += wg * x + w * R * pick<NY>(Zx, yy);
-                        gy += wg * y + w * R * pick<NY>(Zy, yy);
-                        gz += wg * z + w * R * pick<NY>(Zz, yy);
-                    }
+                            const int yy = -1 - lk;
+                            const T w = dPin[i + nX * o0];
+                            dr += w * pick<NY>(Z, yy);
+#pragma unroll
+                            for (int q2 = 0; q2 < NY; ++q2) W[q2] += (q2 == yy) ? w * R : T(0);
+                        }
                 }
-                dRb[k * TILE + slot] += dr;
+                sgd += G * dr;
+                myrow[k] += dr;
             }
             if (dX != nullptr) {
+                T gx = sgd * x, gy = sgd * y3, gz = sgd * z;
+                solid_harmonics_grad_dot<L, T>(x, y3, z, W, gx, gy, gz);
                 dX[3 * i] = scale * gx;
                 dX[3 * i + 1] = scale * gy;
                 dX[3 * i + 2] = scale * gz;
             }
-            sgb[slot] = sg;
-            fb[slot] = f;
+            fu[pos] = f;
         }
         if (!want_params) continue;
-        // ---- phase B ---- (each thread reads only its own LDS slots: no barrier needed)
-        for (int g0 = 0; g0 < NZ; g0 += AO_PB_NZG) {
-            const int nzg = min(AO_PB_NZG, NZ - g0);
-            for (int k = 0; k < nRad; ++k) {
-                const int pk = runs.kpoly[k];
-                for (int m0 = 0; m0 < K; m0 += AO_PB_KC) {
-                    T accD[AO_PB_NZG][AO_PB_KC], accZ[AO_PB_NZG][AO_PB_KC];
-#pragma unroll
-                    for (int q = 0; q < AO_PB_NZG; ++q)
-#pragma unroll
-                        for (int m = 0; m < AO_PB_KC; ++m) accD[q][m] = accZ[q][m] = T(0);
-#pragma unroll
-                    for (int p = 0; p < AO_PB_PPT; ++p) {
-                        const int slot = p * AO_PB_THREADS + threadIdx.x;
-                        const int sg = sgb[slot];
-                        const T f = fb[slot];
-                        T dr = dRb[k * TILE + slot];
-                        if (pk != 0) {
-                            const T r = (decay == AOK_DECAY_GAUSSIAN) ? sqrt(f) : f;
-                            dr *= int_pow(r, pk);
-                        }
-                        const int sgs = sg < 0 ? 0 : sg;
-                        const ParamPair<T> *row = tab + (sgs * nRad + k) * K;
-                        T mask[AO_PB_NZG];
-#pragma unroll
-                        for (int q = 0; q < AO_PB_NZG; ++q) mask[q] = (sg == g0 + q) ? T(1) : T(0);
-#pragma unroll
-                        for (int m = 0; m < AO_PB_KC; ++m) {
-                            if (m0 + m < K) {
-                                const ParamPair<T> pp = row[m0 + m];
-                                const T w = dr * exp_param<T>(pp.a, f);
-                                const T wz = -w * pp.d * f;
-#pragma unroll
-                                for (int q = 0; q < AO_PB_NZG; ++q) {
-                                    accD[q][m] += mask[q] * w;
-                                    accZ[q][m] += mask[q] * wz;
-                                }
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int q = 0; q < AO_PB_NZG; ++q) {
-                        if (q >= nzg) break;
-#pragma unroll
-                        for (int m = 0; m < AO_PB_KC; ++m) {
-                            if (m0 + m >= K) break;
-                            const double sD = wave_sum(static_cast<double>(accD[q][m]));
-                            const double sZ = wave_sum(static_cast<double>(accZ[q][m]));
-                            const int base = 2 * (((g0 + q) * nRad + k) * K + m0 + m);
-                            if (lane == 0) {
-                                myacc[base] += sD;
-                                myacc[base + 1] += sZ;
-                            }
-                        }
-                    }
-                }
+        __syncthreads();
+        // ---- phase B ----
+        for (int it = threadIdx.x; it < y.nitem; it += AO_PB_THREADS) {
+            const int u = it / y.nslot, slot = it - u * y.nslot;  // slot = (g*nRad + k)*K + m
+            const int g = slot / (nRad * K), rem = slot - g * nRad * K, k = rem / K;
+            const ParamPair<T> pp = tab[slot];
+            const int pk = run_poly(runs, k);
+            const int n0 = soff[g], cnt = soff[g + 1] - n0;
+            const int j0 = n0 + (cnt * u) / y.nsub, j1 = n0 + (cnt * (u + 1)) / y.nsub;
+            const T *col = dru + k;
+            T sD = T(0), sZ = T(0);
+            auto term = [&](int j) {
+                const T f = fu[j];
+                T dr = col[j * y.nRadP];
+                if (pk != 0) dr *= int_pow((decay == AOK_DECAY_GAUSSIAN) ? sqrt(f) : f, pk);
+                const T w = dr * exp_param<T>(pp.a, f);
+                sD += w;
+                sZ -= w * pp.d * f;
+            };
+            int j = j0;
+            for (; j + 4 <= j1; j += 4) {
+                term(j);
+                term(j + 1);
+                term(j + 2);
+                term(j + 3);
             }
+            for (; j < j1; ++j) term(j);
+            acc[2 * it] += static_cast<double>(sD);
+            acc[2 * it + 1] += static_cast<double>(sZ);
         }
+        __syncthreads();  // LDS rows are rewritten by the next tile
     }
     if (!want_params) return;
-    __syncthreads();
-    const int nw = AO_PB_THREADS / 64;
-    for (int q = threadIdx.x; q < nslot; q += AO_PB_THREADS) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += wacc[w * nslot + q];
-        partial[static_cast<int64_t>(q) * gridDim.x + blockIdx.x] = s;
-    }
+    // partial[b][item][2] (items of this block; the accumulators are owned per thread)
+    for (int q = threadIdx.x; q < 2 * y.nitem; q += AO_PB_THREADS)
+        partial[static_cast<int64_t>(blockIdx.x) * 2 * y.nitem + q] = acc[q];
 }
 
-// Sum the per-block partials of every (sg, k, m) slot in block order and scatter to
-// the Julia [nRad x K x NZ] layout.
+// Per slot: sum its nsub items over all blocks in a fixed order (block per slot, thread
+// stride over blocks, LDS tree), then scatter to the Julia [nRad x K x NZ] layout.
 template <typename T>
-__global__ void ao_pullback_finalize(const double *__restrict__ partial, int nblk, int nRad, int K, int NZ,
-                                     T *__restrict__ dzeta, T *__restrict__ dD)
+__global__ __launch_bounds__(256) void ao_pullback_finalize(const double *__restrict__ partial, int nblk, int nRad,
+                                                            int K, int NZ, int nsub, T *__restrict__ dzeta,
+                                                            T *__restrict__ dD)
 {
-    const int ntab = NZ * nRad * K;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ntab) return;
+    __shared__ double rd[256], rz[256];
+    const int nslot = NZ * nRad * K, nitem = nslot * nsub;
+    const int q = blockIdx.x;
     double sD = 0.0, sZ = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        sD += partial[static_cast<int64_t>(2 * q) * nblk + b];
-        sZ += partial[static_cast<int64_t>(2 * q + 1) * nblk + b];
+    if (partial)
+        for (int b = threadIdx.x; b < nblk; b += 256)
+            for (int u = 0; u < nsub; ++u) {
+                const double *pb = partial + (static_cast<int64_t>(b) * nitem + u * nslot + q) * 2;
+                sD += pb[0];
+                sZ += pb[1];
+            }
+    rd[threadIdx.x] = sD;
+    rz[threadIdx.x] = sZ;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) {
+            rd[threadIdx.x] += rd[threadIdx.x + h];
+            rz[threadIdx.x] += rz[threadIdx.x + h];
+        }
+        __syncthreads();
     }
+    if (threadIdx.x != 0) return;
     const int sg = q / (nRad * K);
     const int rem = q - sg * nRad * K;
     const int k = rem / K, m = rem - k * K;
     const int64_t dst = k + static_cast<int64_t>(nRad) * (m + static_cast<int64_t>(K) * sg);
-    if (dD) dD[dst] = static_cast<T>(sD);
-    if (dzeta) dzeta[dst] = static_cast<T>(sZ);
+    if (dD) dD[dst] = static_cast<T>(rd[0]);
+    if (dzeta) dzeta[dst] = static_cast<T>(rz[0]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -555,7 +592,7 @@ static aok_status launch_eval_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &
 static bool runs_canonical(const AoRunTable &runs)
 {
     for (int q = 0; q < runs.nrun; ++q)
-        if (runs.l[q] < 0) return false;
+        if (run_l(runs.run[q]) < 0) return false;
     return true;
 }
 
@@ -579,58 +616,46 @@ aok_status ao_eval_launch(aok_ctx ctx, const AoArgs &a, const AoRunTable &runs)
 }
 
 
-template <typename T>
-static size_t ao_pb_shmem(const AoArgs &a)
-{
-    constexpr int TILE = AO_PB_THREADS * AO_PB_PPT;
-    const int ntab = a.NZ * a.nRad * a.K;
-    size_t off = (sizeof(ParamPair<T>) * ntab + 15) & ~size_t(15);
-    off += sizeof(double) * ntab * 2 * (AO_PB_THREADS / 64);
-    off += sizeof(T) * a.nRad * TILE + sizeof(T) * TILE + sizeof(int) * TILE;
-    return off;
-}
-
 template <typename T, int L, bool GENERIC>
 static aok_status launch_pb_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &runs, void *dX, void *dzeta,
                               void *dD)
 {
-    constexpr int TILE = AO_PB_THREADS * AO_PB_PPT;
-    const int64_t ntiles = (a.nX + TILE - 1) / TILE;
-    const int grid = static_cast<int>(std::min<int64_t>(ntiles, 2048));
-    const int ntab = a.NZ * a.nRad * a.K;
+    const int64_t ntiles = (a.nX + AO_PB_THREADS - 1) / AO_PB_THREADS;
+    const int grid = static_cast<int>(std::min<int64_t>(ntiles, ctx->tune.ao_grid > 0 ? ctx->tune.ao_grid : 2048));
+    const AoPbLayout y = ao_pb_layout<T>(a.nRad, a.K, a.NZ);
     const bool want = (dzeta != nullptr) || (dD != nullptr);
     if (grid <= 0) {
         if (want) {
             // no points: zero cotangents (the reference returns zero-filled arrays)
-            hipLaunchKernelGGL((ao_pullback_finalize<T>), dim3((ntab + 255) / 256), dim3(256), 0, ctx->stream,
-                               static_cast<const double *>(nullptr), 0, a.nRad, a.K, a.NZ, static_cast<T *>(dzeta),
-                               static_cast<T *>(dD));
+            hipLaunchKernelGGL((ao_pullback_finalize<T>), dim3(y.nslot), dim3(256), 0, ctx->stream,
+                               static_cast<const double *>(nullptr), 0, a.nRad, a.K, a.NZ, y.nsub,
+                               static_cast<T *>(dzeta), static_cast<T *>(dD));
             AOK_HIP(ctx, hipGetLastError());
         }
         return AOK_OK;
     }
-    const size_t shm = ao_pb_shmem<T>(a);
+    const size_t shm = y.total;
     if (shm > 160 * 1024)
         return fail(ctx, AOK_ERR_UNSUPPORTED, "ao_pullback: basis too large for the LDS tile");
-    const size_t part_bytes = align256(sizeof(double) * static_cast<size_t>(ntab) * 2 * grid);
+    const size_t part_bytes = align256(sizeof(double) * 2 * static_cast<size_t>(y.nitem) * grid);
     const ParamPair<T> *gtab = nullptr;
     aok_status st = param_table<T>(ctx, a, part_bytes, gtab);
     if (st != AOK_OK) return st;
     double *partial = static_cast<double *>(ctx->ws);
+    auto kern = &ao_pullback_kernel<T, L, GENERIC>;
     if (shm > 64 * 1024)
-        AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&ao_pullback_kernel<T, L, GENERIC>),
+        AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
     time_begin(ctx, "ao_pullback");
-    hipLaunchKernelGGL((ao_pullback_kernel<T, L, GENERIC>), dim3(grid), dim3(AO_PB_THREADS), shm, ctx->stream,
-                       static_cast<const T *>(a.X), a.sidx, a.itype, a.nX, gtab,
-                        a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(AO_PB_THREADS), shm, ctx->stream, static_cast<const T *>(a.X),
+                       a.sidx, a.itype, a.nX, gtab, a.nRad, a.K, a.NZ, static_cast<T>(a.scale), a.decay, runs,
                        static_cast<const T *>(a.dPin), static_cast<T *>(dX), partial, want ? 1 : 0, ctx->d_err);
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
     if (want) {
-        hipLaunchKernelGGL((ao_pullback_finalize<T>), dim3((ntab + 255) / 256), dim3(256), 0, ctx->stream,
-                           static_cast<const double *>(partial), grid, a.nRad, a.K, a.NZ, static_cast<T *>(dzeta),
-                           static_cast<T *>(dD));
+        hipLaunchKernelGGL((ao_pullback_finalize<T>), dim3(y.nslot), dim3(256), 0, ctx->stream,
+                           static_cast<const double *>(partial), grid, a.nRad, a.K, a.NZ, y.nsub,
+                           static_cast<T *>(dzeta), static_cast<T *>(dD));
         AOK_HIP(ctx, hipGetLastError());
     }
     return AOK_OK;

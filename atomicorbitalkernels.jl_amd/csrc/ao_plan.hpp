@@ -18,14 +18,23 @@ namespace aok {
 constexpr int AO_MAXRUN = 192;
 constexpr int AO_MAXRAD = 256;
 
+// Runs packed in 32-bit words so the kernels read them with scalar loads (the table is
+// indexed by a wave-uniform counter):  k | (l + 128) << 8 | o0 << 16, and r^poly of
+// radial k in byte k % 4 of kpoly4[k / 4].
 struct AoRunTable {
     int nrun;
-    int16_t k[AO_MAXRUN];     // radial index (0-based)
-    int16_t l[AO_MAXRUN];     // >= 0 full shell; < 0 single orbital with harmonic -1-l
-    int16_t o0[AO_MAXRUN];    // first orbital column
-    int16_t poly[AO_MAXRUN];  // r^poly of radial k
-    int16_t kpoly[AO_MAXRAD]; // r^poly per radial (pullback)
+    int norb;
+    uint32_t run[AO_MAXRUN];
+    uint32_t kpoly4[AO_MAXRAD / 4];
 };
+
+__host__ __device__ inline int run_k(uint32_t w) { return static_cast<int>(w & 0xffu); }
+__host__ __device__ inline int run_l(uint32_t w) { return static_cast<int>((w >> 8) & 0xffu) - 128; }
+__host__ __device__ inline int run_o0(uint32_t w) { return static_cast<int>(w >> 16); }
+__host__ __device__ inline int run_poly(const AoRunTable &t, int k)
+{
+    return static_cast<int>((t.kpoly4[k >> 2] >> (8 * (k & 3))) & 0xffu);
+}
 
 struct AoArgs {
     const void *X;
@@ -48,11 +57,14 @@ inline std::string build_runs(const int64_t *poly, const int64_t *iR, const int6
                               int64_t nRad, int L, AoRunTable &t)
 {
     t.nrun = 0;
+    t.norb = static_cast<int>(nOrb);
     if (nRad > AO_MAXRAD) return "nRad exceeds " + std::to_string(AO_MAXRAD);
+    if (nOrb > 65535) return "nOrb too large";
+    for (int q = 0; q < AO_MAXRAD / 4; ++q) t.kpoly4[q] = 0u;
     for (int64_t k = 0; k < nRad; ++k) {
         const int64_t p = poly ? poly[k] : 0;
         if (p < 0 || p > 64) return "poly out of range";
-        t.kpoly[k] = static_cast<int16_t>(p);
+        t.kpoly4[k >> 2] |= static_cast<uint32_t>(p) << (8 * (k & 3));
     }
     const int nY = (L + 1) * (L + 1);
     int64_t o = 0;
@@ -71,19 +83,12 @@ inline std::string build_runs(const int64_t *poly, const int64_t *iR, const int6
                     full = false;
                     break;
                 }
-        const int q = t.nrun++;
-        t.k[q] = static_cast<int16_t>(k);
-        t.o0[q] = static_cast<int16_t>(o);
-        t.poly[q] = t.kpoly[k];
-        if (full) {
-            t.l[q] = static_cast<int16_t>(l);
-            o += 2 * l + 1;
-        } else {
-            t.l[q] = static_cast<int16_t>(-1 - y);
-            o += 1;
-        }
+        // l >= 0: full shell of 2l + 1 columns; l < 0: one column with harmonic -1 - l
+        const int lenc = full ? l : static_cast<int>(-1 - y);
+        t.run[t.nrun++] = static_cast<uint32_t>(k) | (static_cast<uint32_t>(lenc + 128) << 8) |
+                          (static_cast<uint32_t>(o) << 16);
+        o += full ? 2 * l + 1 : 1;
     }
-    if (nOrb > 32767) return "nOrb too large";
     return std::string();
 }
 

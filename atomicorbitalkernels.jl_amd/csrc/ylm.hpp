@@ -69,10 +69,37 @@ struct YlmOut {
     T *Zz;
 };
 
-template <int L, bool GRAD, typename T, int M, int LL>
+// Sinks of the recursion: store every Z (and gradient), or contract the gradients with
+// weights W as they are produced: g = sum_y W_y grad Z_y (no gradient arrays kept).
+template <typename T>
+struct YlmStore {
+    YlmOut<T> o;
+    __device__ __forceinline__ void val(int idx, T v) { o.Z[idx] = v; }
+    __device__ __forceinline__ void grad(int idx, T gx, T gy, T gz)
+    {
+        o.Zx[idx] = gx;
+        o.Zy[idx] = gy;
+        o.Zz[idx] = gz;
+    }
+};
+
+template <typename T>
+struct YlmGradDot {
+    const T *W;
+    T gx, gy, gz;
+    __device__ __forceinline__ void val(int, T) {}
+    __device__ __forceinline__ void grad(int idx, T ax, T ay, T az)
+    {
+        gx += W[idx] * ax;
+        gy += W[idx] * ay;
+        gz += W[idx] * az;
+    }
+};
+
+template <int L, bool GRAD, typename T, int M, int LL, typename S>
 __device__ __forceinline__ void ylm_column(const T x, const T y, const T z, const T r2, const T *c,
                                            const T *s, T Pm1, T Pzm1, T Prm1, T Pm2, T Pzm2, T Prm2,
-                                           YlmOut<T> o)
+                                           S &o)
 {
     if constexpr (LL <= L) {
         T P, Pz = T(0), Pr = T(0);
@@ -98,34 +125,26 @@ __device__ __forceinline__ void ylm_column(const T x, const T y, const T z, cons
         }
         constexpr int base = LL * LL + LL;
         if constexpr (M == 0) {
-            o.Z[base] = P;
-            if constexpr (GRAD) {
-                o.Zx[base] = T(2) * x * Pr;
-                o.Zy[base] = T(2) * y * Pr;
-                o.Zz[base] = Pz + T(2) * z * Pr;
-            }
+            o.val(base, P);
+            if constexpr (GRAD) o.grad(base, T(2) * x * Pr, T(2) * y * Pr, Pz + T(2) * z * Pr);
         } else {
-            o.Z[base + M] = P * c[M];
-            o.Z[base - M] = P * s[M];
+            o.val(base + M, P * c[M]);
+            o.val(base - M, P * s[M]);
             if constexpr (GRAD) {
                 const T pzz = Pz + T(2) * z * Pr;
                 const T tx = T(2) * x * Pr, ty = T(2) * y * Pr;
                 const T mP = T(M) * P;
-                o.Zx[base + M] = tx * c[M] + mP * c[M - 1];
-                o.Zy[base + M] = ty * c[M] - mP * s[M - 1];
-                o.Zz[base + M] = pzz * c[M];
-                o.Zx[base - M] = tx * s[M] + mP * s[M - 1];
-                o.Zy[base - M] = ty * s[M] + mP * c[M - 1];
-                o.Zz[base - M] = pzz * s[M];
+                o.grad(base + M, tx * c[M] + mP * c[M - 1], ty * c[M] - mP * s[M - 1], pzz * c[M]);
+                o.grad(base - M, tx * s[M] + mP * s[M - 1], ty * s[M] + mP * c[M - 1], pzz * s[M]);
             }
         }
         ylm_column<L, GRAD, T, M, LL + 1>(x, y, z, r2, c, s, P, Pz, Pr, Pm1, Pzm1, Prm1, o);
     }
 }
 
-template <int L, bool GRAD, typename T, int M>
+template <int L, bool GRAD, typename T, int M, typename S>
 __device__ __forceinline__ void ylm_columns(const T x, const T y, const T z, const T r2, const T *c,
-                                            const T *s, YlmOut<T> o)
+                                            const T *s, S &o)
 {
     if constexpr (M <= L) {
         ylm_column<L, GRAD, T, M, M>(x, y, z, r2, c, s, T(0), T(0), T(0), T(0), T(0), T(0), o);
@@ -133,9 +152,8 @@ __device__ __forceinline__ void ylm_columns(const T x, const T y, const T z, con
     }
 }
 
-// Z[(L+1)^2] (and, with GRAD, Zx/Zy/Zz) at (x, y, z).
-template <int L, bool GRAD, typename T>
-__device__ __forceinline__ void solid_harmonics(const T x, const T y, const T z, YlmOut<T> o)
+template <int L, bool GRAD, typename T, typename S>
+__device__ __forceinline__ void ylm_run(const T x, const T y, const T z, S &o)
 {
     T c[L + 1], s[L + 1];
     c[0] = T(1);
@@ -147,6 +165,26 @@ __device__ __forceinline__ void solid_harmonics(const T x, const T y, const T z,
     }
     const T r2 = x * x + y * y + z * z;
     ylm_columns<L, GRAD, T, 0>(x, y, z, r2, c, s, o);
+}
+
+// Z[(L+1)^2] (and, with GRAD, Zx/Zy/Zz) at (x, y, z).
+template <int L, bool GRAD, typename T>
+__device__ __forceinline__ void solid_harmonics(const T x, const T y, const T z, YlmOut<T> o)
+{
+    YlmStore<T> st{o};
+    ylm_run<L, GRAD, T>(x, y, z, st);
+}
+
+// sum_y W[y] grad Z_y at (x, y, z), accumulated into (gx, gy, gz).
+template <int L, typename T>
+__device__ __forceinline__ void solid_harmonics_grad_dot(const T x, const T y, const T z, const T *W, T &gx, T &gy,
+                                                         T &gz)
+{
+    YlmGradDot<T> d{W, gx, gy, gz};
+    ylm_run<L, true, T>(x, y, z, d);
+    gx = d.gx;
+    gy = d.gy;
+    gz = d.gz;
 }
 
 }  // namespace aok
