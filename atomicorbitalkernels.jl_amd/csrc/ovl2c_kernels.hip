@@ -521,7 +521,7 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     T *mine = stripe + lane * sh.str;
     const int64_t NN = static_cast<int64_t>(N) * N;
     {
-        const OvlChunk ch = sh.ch[blockIdx.y];
+        const OvlChunk ch = unpack_chunk(sh.ch[blockIdx.y]);
         const int sb = ch.sb, lb = sh.ls[sb];
         const int nrb = nreal[sgB * ns + sb];
         for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     T *mine = stripe + lane * sh.str;
     const int64_t NN = static_cast<int64_t>(N) * N;
     __syncthreads();
-    const OvlChunk ch = sh.ch[blockIdx.y];
+    const OvlChunk ch = unpack_chunk(sh.ch[blockIdx.y]);
     const int sb = ch.sb, lb = sh.ls[sb];
     const int nbb = nbf_of(lb), nrows = ch.nrows;
     const int len = nrows * nbb, total = OVL_TILE * len;
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
     bool use = false;
     int col = 0;
     if (ci < nchunk) {
-        const OvlChunk ch = sh.ch[ci];
+        const OvlChunk ch = unpack_chunk(sh.ch[ci]);
         if (side == 0) {
             use = s >= ch.sa0 && s < ch.sa1;
             col = 2 * ((s - ch.sa0) * K + li);

@@ -28,17 +28,39 @@ constexpr int OVL_MAXK = 32;     // primitive slots per shell
 constexpr int OVL_MAXNZ = 16;    // species
 constexpr int OVL_TILE = 64;     // batch elements per wave
 
+// One stripe chunk: bra shells sa0..sa1-1 (rows row0..row0+nrows-1) x ket shell sb.
 struct OvlChunk {
-    int16_t sb, sa0, sa1, row0, nrows;
+    int sb, sa0, sa1, row0, nrows;
 };
+
+// Chunks travel packed in two 32-bit words (sb | sa0 << 8 | sa1 << 16, row0 | nrows << 16):
+// the shape is a kernel argument indexed by wave-uniform counters, and 32-bit fields are
+// what the kernels can read with scalar loads.
+struct OvlChunkP {
+    uint32_t s, r;
+};
+
+__host__ __device__ inline OvlChunkP pack_chunk(const OvlChunk &c)
+{
+    return OvlChunkP{static_cast<uint32_t>(c.sb) | (static_cast<uint32_t>(c.sa0) << 8) |
+                         (static_cast<uint32_t>(c.sa1) << 16),
+                     static_cast<uint32_t>(c.row0) | (static_cast<uint32_t>(c.nrows) << 16)};
+}
+
+__host__ __device__ inline OvlChunk unpack_chunk(OvlChunkP p)
+{
+    return OvlChunk{static_cast<int>(p.s & 0xffu), static_cast<int>((p.s >> 8) & 0xffu),
+                    static_cast<int>((p.s >> 16) & 0xffu), static_cast<int>(p.r & 0xffffu),
+                    static_cast<int>(p.r >> 16)};
+}
 
 struct OvlShape {
     int ns, K, NZ, N;
     int nchunk;
     int str;  // LDS floats per lane (odd)
-    int8_t ls[OVL_MAXSH];
-    int16_t off[OVL_MAXSH + 1];
-    OvlChunk ch[OVL_MAXCH];
+    int ls[OVL_MAXSH];
+    int off[OVL_MAXSH + 1];
+    OvlChunkP ch[OVL_MAXCH];
 };
 
 // One primitive pair (record) of the fast kernels.  mu2 = alpha*beta/p (times log2(e)
@@ -78,8 +100,8 @@ inline std::string build_shape(int64_t ns, int64_t K, int64_t NZ, const int64_t 
     s.off[0] = 0;
     for (int64_t q = 0; q < ns; ++q) {
         if (ls[q] < 0 || ls[q] > lmax_allowed) return "shell l out of range (max " + std::to_string(lmax_allowed) + ")";
-        s.ls[q] = static_cast<int8_t>(ls[q]);
-        s.off[q + 1] = static_cast<int16_t>(s.off[q] + nbf_of(static_cast<int>(ls[q])));
+        s.ls[q] = static_cast<int>(ls[q]);
+        s.off[q + 1] = s.off[q] + nbf_of(static_cast<int>(ls[q]));
     }
     s.N = s.off[ns];
     int maxlen = 0;
@@ -95,12 +117,7 @@ inline std::string build_shape(int64_t ns, int64_t K, int64_t NZ, const int64_t 
                 sa1 = sa + 1;
             }
             if (s.nchunk >= OVL_MAXCH) return "too many stripe chunks";
-            OvlChunk &c = s.ch[s.nchunk++];
-            c.sb = static_cast<int16_t>(sb);
-            c.sa0 = static_cast<int16_t>(sa);
-            c.sa1 = static_cast<int16_t>(sa1);
-            c.row0 = s.off[sa];
-            c.nrows = static_cast<int16_t>(rows);
+            s.ch[s.nchunk++] = pack_chunk(OvlChunk{sb, sa, sa1, s.off[sa], rows});
             maxlen = std::max(maxlen, rows * nb);
             sa = sa1;
         }
