@@ -43,7 +43,6 @@ struct aok_context {
         int ao_ppl = 0;   // AO values-only: consecutive points per lane (1, 2, 4)
         int ao_grid = 0;  // AO: persistent grid size (blocks)
         int ovl_cap = 0;  // 2c: LDS stripe capacity (floats per lane)
-        int ovl_group = 0;  // 2c/3c: tiles per XCD group of the (tile, chunk) order (0 = tile fastest)
     } tune;
 };
 

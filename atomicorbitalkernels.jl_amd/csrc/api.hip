@@ -140,9 +140,6 @@ aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
     } else if (k == "ovl_cap") {
         AOK_REQUIRE(ctx, value == 0 || (value >= 16 && value <= 1024), "ovl_cap must be 0 or in 16..1024");
         ctx->tune.ovl_cap = static_cast<int>(value);
-    } else if (k == "ovl_group") {
-        AOK_REQUIRE(ctx, value >= 0 && value <= 4096, "ovl_group must be in 0..4096");
-        ctx->tune.ovl_group = static_cast<int>(value);
     } else {
         return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
     }
@@ -157,7 +154,6 @@ aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value)
     if (k == "ao_ppl") *value = ctx->tune.ao_ppl;
     else if (k == "ao_grid") *value = ctx->tune.ao_grid;
     else if (k == "ovl_cap") *value = ctx->tune.ovl_cap;
-    else if (k == "ovl_group") *value = ctx->tune.ovl_group;
     else return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
     return AOK_OK;
 }

@@ -489,37 +489,6 @@ __device__ __forceinline__ bool tile_info(int t, int NK, const int *__restrict__
     return true;
 }
 
-// XCD-aware (tile, chunk) of a block.  MI355X hands workgroup ids round-robin to its 8
-// XCDs, each with its own L2.  Blocks of one XCD take the tiles t = xcd (mod 8) in groups
-// of G: the G tiles of a group run chunk 0, then chunk 1, ... .  Neighbouring blocks share
-// a chunk (same shell-pair code path and records), and the 128-byte lines that the chunks
-// of one tile share (column segments of its N x N blocks) meet in one L2 while resident,
-// so partial-line writes merge and shared reads hit.  G = 0 keeps the plain (tile,
-// chunk) order with the tile fastest.
-constexpr int OVL_NXCD = 8;
-
-__device__ __forceinline__ void tile_chunk(int nchunk, int G, int &t, int &c)
-{
-    const int bid = static_cast<int>(blockIdx.x);
-    if (G <= 0) {
-        c = bid / (gridDim.x / nchunk);
-        t = bid - c * (gridDim.x / nchunk);
-        return;
-    }
-    const int x = bid % OVL_NXCD, j = bid / OVL_NXCD;
-    const int grp = j / (G * nchunk), r = j - grp * (G * nchunk);
-    c = r / G;
-    t = (grp * G + (r - c * G)) * OVL_NXCD + x;
-}
-
-// Blocks of a (tile, chunk) launch: tiles rounded up to whole groups of G per XCD.
-inline int tile_chunk_grid(int maxtiles, int nchunk, int G)
-{
-    if (G <= 0) return maxtiles * nchunk;
-    const int per = OVL_NXCD * G;
-    return nchunk * per * ((maxtiles + per - 1) / per);
-}
-
 // ---------------------------------------------------------------------------------
 // Fast forward
 // ---------------------------------------------------------------------------------
@@ -530,15 +499,14 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ boff, const int *__restrict__ bcnt,
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
-                                                     const int *__restrict__ nreal, T *__restrict__ out, int tgroup)
+                                                     const int *__restrict__ nreal, T *__restrict__ out)
 {
     extern __shared__ unsigned char smem_raw[];
     int *bl_b = reinterpret_cast<int *>(smem_raw);
     T *stripe = reinterpret_cast<T *>(smem_raw + 256);
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     TileInfo ti;
-    int tile, chunk;
-    tile_chunk(sh.nchunk, tgroup, tile, chunk);
+    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
@@ -599,7 +567,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
                                                      const int *__restrict__ nreal, const int *__restrict__ nlist,
-                                                     double *__restrict__ partial, int tgroup)
+                                                     double *__restrict__ partial)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -608,8 +576,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     double *accA = reinterpret_cast<double *>(smem_raw + 256);  // [sa - sa0][K][2], then the ket side at ns*K*2
     T *stripe = reinterpret_cast<T *>(accA + pstride);
     TileInfo ti;
-    int tile, chunk;
-    tile_chunk(sh.nchunk, tgroup, tile, chunk);
+    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     for (int q = lane; q < pstride; q += OVL_TILE) accA[q] = 0.0;
@@ -1057,14 +1024,13 @@ static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    const int G = ctx->tune.ovl_group;
     time_begin(ctx, "ovl2c_fwd_fast");
-    hipLaunchKernelGGL(kern, dim3(tile_chunk_grid(w.maxtiles, sh.nchunk, G)), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
-                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), G);
+                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out));
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
@@ -1079,16 +1045,15 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    const int G = ctx->tune.ovl_group;
     time_begin(ctx, "ovl2c_bwd_fast");
-    hipLaunchKernelGGL(kern, dim3(tile_chunk_grid(w.maxtiles, sh.nchunk, G)), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.dS),
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.nlist),
-                       reinterpret_cast<double *>(ws + w.partial), G);
+                       reinterpret_cast<double *>(ws + w.partial));
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
     const int nslot = sh.NZ * sh.ns * sh.K;

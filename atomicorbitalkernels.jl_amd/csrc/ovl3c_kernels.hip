@@ -360,24 +360,11 @@ __global__ __launch_bounds__(64) void ovl3c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename Rec3Of<T>::type *__restrict__ rec,
                                                      const int64_t *__restrict__ grp_off,
-                                                     const int *__restrict__ nreal, T *__restrict__ out, int tgroup)
+                                                     const int *__restrict__ nreal, T *__restrict__ out, int maxtiles)
 {
     const int ns = sh.ns, NZ = sh.NZ, N = sh.N, NK = NZ * NZ * NZ;
-    // (tile, shell pair (s_c, s_b)) of the block, grouped per XCD as the 2-centre kernels'
-    // (tile, chunk): G tiles of one XCD run each shell pair in turn (G = 0: tile fastest)
-    int t, cb;
-    {
-        const int bid = static_cast<int>(blockIdx.x), nc = ns * ns;
-        if (tgroup <= 0) {
-            cb = bid / (gridDim.x / nc);
-            t = bid - cb * (gridDim.x / nc);
-        } else {
-            const int x = bid % 8, j = bid / 8;
-            const int grp = j / (tgroup * nc), r = j - grp * (tgroup * nc);
-            cb = r / tgroup;
-            t = (grp * tgroup + (r - cb * tgroup)) * 8 + x;
-        }
-    }
+    // (tile, shell pair (s_c, s_b)) of the block, tile fastest
+    const int t = static_cast<int>(blockIdx.x) % maxtiles, cb = static_cast<int>(blockIdx.x) / maxtiles;
     if (t >= toff[NK]) return;
     const int key = tile_key[t];
     const int first = boff[key] + (t - toff[key]) * OVL_TILE;
@@ -565,15 +552,14 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
                        reinterpret_cast<const int64_t *>(ws + w.grp_off), reinterpret_cast<R *>(ws + w.rec));
     AOK_HIP(ctx, hipGetLastError());
     time_begin(ctx, "ovl3c_fwd_fast");
-    const int G = ctx->tune.ovl_group, nc = sh.ns * sh.ns;
-    const int grid = G <= 0 ? w.maxtiles * nc : nc * 8 * G * ((w.maxtiles + 8 * G - 1) / (8 * G));
+    const int grid = w.maxtiles * sh.ns * sh.ns;
     hipLaunchKernelGGL((ovl3c_fwd_fast<T, LMAXB>), dim3(grid), dim3(OVL_TILE), 0, ctx->stream, sh,
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<const T *>(c.posC),
                        static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
                        reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
                        reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
                        reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int64_t *>(ws + w.grp_off),
-                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), G);
+                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), w.maxtiles);
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
