@@ -25,6 +25,12 @@ sys.path.insert(0, os.path.join(ROOT, "atomicorbitalkernels.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of the C1 kernels: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
+# runs of this bench (KiB per dispatch, mean of the timed launches), each scaled by what
+# the same counter reads for a 2 GiB device copy in the same runs (FETCH_SIZE reads
+# 1048612 KiB for 2097152 KiB moved: x2.0; WRITE_SIZE reads 2097152 KiB: x1.0).
+PMC_KIB = {"ovl2c_bwd_fast": (139216.3, 11319.0), "ovl2c_fwd_fast": (24611.8, 126700.5)}
+PMC_SCALE = (2.0, 1.0)
 METRIC = "2-ctr overlap pairs/s + (ζ,D)-grad pairs/s, FP32@1e-6 vs plain FP64; AO GB/s"
 
 
@@ -151,6 +157,8 @@ def main():
     dom_ms = max(fwd_ms, bwd_ms)
     dom_flops = flops_bwd if dom == "ovl2c_bwd_fast" else flops_fwd
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    fk, wk = PMC_KIB[dom]
+    traffic = (fk * PMC_SCALE[0] + wk * PMC_SCALE[1]) * 1024.0
     out = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -159,8 +167,9 @@ def main():
                                "(N=15, K=9), zeta log-uniform [0.1,7000], FP32", "global_batch": B,
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": dom,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_flop_per_launch": dom_flops,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "note": "VALU FP32 kernel priced against the published FP32 peak"},
         "fwd_kernel_ms": fwd_ms, "bwd_kernel_ms": bwd_ms,
         "fwd_kernel_pairs_per_s": B / (fwd_ms * 1e-3), "grad_kernel_pairs_per_s": B / (bwd_ms * 1e-3),
@@ -211,58 +220,6 @@ def main():
             "fwd": tp["fwd"][1] / fwd_ms, "bwd": tp["bwd"][1] / bwd_ms,
             "fwd+bwd": (tp["fwd"][1] + tp["bwd"][1]) / (fwd_ms + bwd_ms),
             "step_wall": (tp["fwd"][0] + tp["bwd"][0]) / ms_step}
-        # C2: uneven batch, 6 species (union spec N = 50, K = 10, l <= 3), Zipf species pairs
-        log("C2 B=2^18 rrule step")
-        d2 = configs.c2(2, B=2 ** 18)
-        cb2 = d2["basis"]
-        spec2, _, _ = cb2.ao_maps()
-
-        def orbital2(dtype):
-            R2 = aok.GaussianTypeRadials(aok.fortran_from_numpy(cb2.zeta, dtype, dev),
-                                         aok.fortran_from_numpy(cb2.D, dtype, dev), spec2, None, tuple(range(1, 7)))
-            return aok.AtomicOrbitals(R2, max(cb2.ls), length_unit="bohr")
-
-        B2 = d2["posA"].shape[0]
-        o2 = orbital2(torch.float32)
-        p2 = o2.static_params()
-        X2A = (torch.as_tensor(d2["posA"], dtype=torch.float32, device=dev).contiguous(),
-               torch.as_tensor(d2["sidxA"], dtype=torch.int32, device=dev))
-        X2B = (torch.as_tensor(d2["posB"], dtype=torch.float32, device=dev).contiguous(),
-               torch.as_tensor(d2["sidxB"], dtype=torch.int32, device=dev))
-        g2 = torch.Generator(device=dev).manual_seed(9)
-        dS2 = torch.randn((B2, cb2.N, cb2.N), generator=g2, device=dev, dtype=torch.float32).permute(2, 1, 0)
-        for _ in range(2):
-            S2, pb2 = aok.rrule_batch_overlap(o2, X2A, X2B, p2, ctx=ctx)
-            pb2(dS2)
-        torch.cuda.synchronize()
-        fk, bk = [], []
-        ts = time.perf_counter()
-        for _ in range(5):
-            S2, pb2 = aok.rrule_batch_overlap(o2, X2A, X2B, p2, ctx=ctx)
-            fk.append(ctx.last_kernel_ms())
-            pb2(dS2)
-            bk.append(ctx.last_kernel_ms())
-        torch.cuda.synchronize()
-        wall2 = (time.perf_counter() - ts) * 1e3 / 5
-        f2f, f2b = algorithmic_flops(cb2, d2["sidxA"], d2["sidxB"])
-        o2d = orbital2(torch.float64)
-        S2d = aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
-                                {"Rnl": {"zeta": aok.as_fortran(o2.Rnl.zeta.double()),
-                                         "D": aok.as_fortran(o2.Rnl.D.double())}, "Ylm": {}}, ctx=ctx)
-        num = (S2.double() - S2d).abs().amax(dim=(0, 1))
-        den = S2d.abs().amax(dim=(0, 1)).clamp_min(1e-300)
-        err2 = float((num / den).max())
-        del S2d
-        aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
-                          o2d.static_params(), mode="plain", ctx=ctx)
-        plain2 = ctx.last_kernel_ms()
-        out["ovl2c_c2"] = {
-            "B": B2, "N": cb2.N, "NZ": cb2.NZ, "step_ms": wall2, "pairs_per_s": B2 / (wall2 * 1e-3),
-            "fwd_kernel_ms": float(np.mean(fk)), "bwd_kernel_ms": float(np.mean(bk)),
-            "fwd_TFLOPs": f2f / (np.mean(fk) * 1e-3) / 1e12, "bwd_TFLOPs": f2b / (np.mean(bk) * 1e-3) / 1e12,
-            "fp32_blockwise_max_rel_err": err2, "plain_f64_fwd_kernel_ms": plain2,
-            "fwd_speedup_vs_plain_f64": plain2 / float(np.mean(fk))}
-        del S2, dS2, pb2
         # HBM ceilings measured with torch's own copy and fill kernels (2 GiB buffers): the
         # copy kernel is also the scale reference for the FETCH_SIZE / WRITE_SIZE counters
         log("HBM copy / fill ceilings")
@@ -339,6 +296,58 @@ def main():
                          "frac_of_hbm_peak": nbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "algorithmic_bytes": nbytes}
         del P, X, dPin
+        # C2: uneven batch, 6 species (union spec N = 50, K = 10, l <= 3), Zipf species pairs
+        log("C2 B=2^18 rrule step")
+        d2 = configs.c2(2, B=2 ** 18)
+        cb2 = d2["basis"]
+        spec2, _, _ = cb2.ao_maps()
+
+        def orbital2(dtype):
+            R2 = aok.GaussianTypeRadials(aok.fortran_from_numpy(cb2.zeta, dtype, dev),
+                                         aok.fortran_from_numpy(cb2.D, dtype, dev), spec2, None, tuple(range(1, 7)))
+            return aok.AtomicOrbitals(R2, max(cb2.ls), length_unit="bohr")
+
+        B2 = d2["posA"].shape[0]
+        o2 = orbital2(torch.float32)
+        p2 = o2.static_params()
+        X2A = (torch.as_tensor(d2["posA"], dtype=torch.float32, device=dev).contiguous(),
+               torch.as_tensor(d2["sidxA"], dtype=torch.int32, device=dev))
+        X2B = (torch.as_tensor(d2["posB"], dtype=torch.float32, device=dev).contiguous(),
+               torch.as_tensor(d2["sidxB"], dtype=torch.int32, device=dev))
+        g2 = torch.Generator(device=dev).manual_seed(9)
+        dS2 = torch.randn((B2, cb2.N, cb2.N), generator=g2, device=dev, dtype=torch.float32).permute(2, 1, 0)
+        for _ in range(2):
+            S2, pb2 = aok.rrule_batch_overlap(o2, X2A, X2B, p2, ctx=ctx)
+            pb2(dS2)
+        torch.cuda.synchronize()
+        fk, bk = [], []
+        ts = time.perf_counter()
+        for _ in range(5):
+            S2, pb2 = aok.rrule_batch_overlap(o2, X2A, X2B, p2, ctx=ctx)
+            fk.append(ctx.last_kernel_ms())
+            pb2(dS2)
+            bk.append(ctx.last_kernel_ms())
+        torch.cuda.synchronize()
+        wall2 = (time.perf_counter() - ts) * 1e3 / 5
+        f2f, f2b = algorithmic_flops(cb2, d2["sidxA"], d2["sidxB"])
+        o2d = orbital2(torch.float64)
+        S2d = aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
+                                {"Rnl": {"zeta": aok.as_fortran(o2.Rnl.zeta.double()),
+                                         "D": aok.as_fortran(o2.Rnl.D.double())}, "Ylm": {}}, ctx=ctx)
+        num = (S2.double() - S2d).abs().amax(dim=(0, 1))
+        den = S2d.abs().amax(dim=(0, 1)).clamp_min(1e-300)
+        err2 = float((num / den).max())
+        del S2d
+        aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
+                          o2d.static_params(), mode="plain", ctx=ctx)
+        plain2 = ctx.last_kernel_ms()
+        out["ovl2c_c2"] = {
+            "B": B2, "N": cb2.N, "NZ": cb2.NZ, "step_ms": wall2, "pairs_per_s": B2 / (wall2 * 1e-3),
+            "fwd_kernel_ms": float(np.mean(fk)), "bwd_kernel_ms": float(np.mean(bk)),
+            "fwd_TFLOPs": f2f / (np.mean(fk) * 1e-3) / 1e12, "bwd_TFLOPs": f2b / (np.mean(bk) * 1e-3) / 1e12,
+            "fp32_blockwise_max_rel_err": err2, "plain_f64_fwd_kernel_ms": plain2,
+            "fwd_speedup_vs_plain_f64": plain2 / float(np.mean(fk))}
+        del S2, dS2, pb2
         # 3-centre overlaps, B = 2^14 triples over the same 3 species, FP32
         log("3c B=2^14")
         c4 = configs.c4a()
