@@ -151,11 +151,18 @@ def load():
 
 
 class Context:
-    """One libaok context: a HIP stream (private, or torch's current stream when
-    ``use_torch_stream``), workspace and error state."""
+    """One libaok context: its HIP stream (by default torch's default stream of the
+    device, or a blocking stream ordered with it when that is the NULL stream; or the
+    stream handle given), workspace and error state."""
 
     def __init__(self, device: int = 0, stream: int | None = None):
         self.lib = load()
+        if stream is None:
+            # torch's default stream: shared when it is a real stream; when it is the legacy
+            # NULL stream the library creates a blocking stream, which is ordered with it
+            import torch
+
+            stream = torch.cuda.default_stream(device).cuda_stream
         h = _vp()
         rc = self.lib.aok_create(ctypes.byref(h), int(device), _vp(stream) if stream else None)
         if rc != AOK_OK:
@@ -181,6 +188,7 @@ class Context:
         raise _ERRORS.get(rc, AOKError)(f"{what}: {msg} (status {rc})")
 
     def call(self, name: str, *args):
+        _order_inputs(self.device)
         rc = getattr(self.lib, name)(self.handle, *args)
         self.check(rc, name)
 
@@ -205,6 +213,16 @@ class Context:
 
     def last_kernel_name(self) -> str:
         return self.lib.aok_last_kernel_name(self.handle).decode()
+
+
+def _order_inputs(device: int):
+    """The library's own stream is ordered after torch's default stream; if torch is
+    currently queueing on another stream, wait for it so the inputs are complete."""
+    import torch
+
+    cur = torch.cuda.current_stream(device)
+    if cur != torch.cuda.default_stream(device):
+        cur.synchronize()
 
 
 _ctx = {}

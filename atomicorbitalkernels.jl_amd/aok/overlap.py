@@ -192,6 +192,7 @@ class Ovl2cPlan:
         b, _ = _sidx_ptr(sB)
         ns, K, NZ = p1.shape
         h = _lib.c_void_p()
+        _lib._order_inputs(self.ctx.device)
         rc = self.ctx.lib.aok_ovl2c_plan_create(self.ctx.handle, _dtype_code(dtype), _flags(mode), a, b, it, B, ns, K,
                                                 NZ, _lib.i64_array(ls), params, p1.data_ptr(), p2.data_ptr(),
                                                 float(lengthscale), _lib.byref(h))
@@ -199,6 +200,7 @@ class Ovl2cPlan:
         self.handle = h
 
     def forward(self, pA, pB, out):
+        _lib._order_inputs(self.ctx.device)
         rc = self.ctx.lib.aok_ovl2c_plan_forward(self.handle, pA.data_ptr(), pB.data_ptr(), out.data_ptr())
         self.ctx.check(rc, "aok_ovl2c_plan_forward")
         return out
@@ -206,6 +208,7 @@ class Ovl2cPlan:
     def pullback(self, dS, pA, pB):
         d1 = fortran_empty(self.shape, self.dtype, dS.device)
         d2 = fortran_empty(self.shape, self.dtype, dS.device)
+        _lib._order_inputs(self.ctx.device)
         rc = self.ctx.lib.aok_ovl2c_plan_pullback(self.handle, dS.data_ptr(), pA.data_ptr(), pB.data_ptr(),
                                                   d1.data_ptr(), d2.data_ptr())
         self.ctx.check(rc, "aok_ovl2c_plan_pullback")

@@ -82,7 +82,9 @@ aok_status aok_create(aok_ctx *out, int device, void *stream)
     if (stream) {
         c->stream = static_cast<hipStream_t>(stream);
     } else {
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        // a blocking stream: ordered after work the caller queued on the legacy default
+        // stream (inputs produced there are complete before the library reads them)
+        if (hipStreamCreate(&c->stream) != hipSuccess) {
             delete c;
             return AOK_ERR_DEVICE;
         }

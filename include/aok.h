@@ -30,7 +30,9 @@
  *    except the per-element species range check, which the kernels perform and the
  *    call reports (AOK_ERR_SPECIES) after its closing synchronisation.
  *  - A context owns one HIP stream (or borrows the caller's), its workspace and its
- *    autotune table.  One context must not be used by two host threads at once;
+ *    autotune table.  An owned stream is a blocking stream: its work is ordered after
+ *    work queued earlier on the legacy default (NULL) stream.  Callers that produce
+ *    inputs on another stream pass that stream to aok_create or synchronise it first.  One context must not be used by two host threads at once;
  *    separate contexts are independent.
  */
 #ifndef AOK_H

@@ -312,3 +312,25 @@ def test_c2_uneven_six_species(dev, dtype):
     tol = 1e-10 if dtype == torch.float64 else 1e-5
     assert rel_max(host(dps["Rnl"]["zeta"]), dzr) < tol
     assert rel_max(host(dps["Rnl"]["D"]), dDr) < tol
+
+
+def test_inputs_produced_by_torch_just_before_the_call(dev):
+    """Cotangents computed by torch kernels immediately before the pullback (no host
+    synchronisation in between) are complete when the library reads them: the result
+    equals the one computed after an explicit synchronisation, bit for bit."""
+    d = _c1_small(65536)
+    cb = d["basis"]
+    log("2c stream ordering B=65536")
+    orb = _orb(cb, torch.float32, dev)
+    ps = orb.static_params()
+    pA, pB = dev_rows(d["posA"], torch.float32, dev), dev_rows(d["posB"], torch.float32, dev)
+    sA = torch.as_tensor(d["sidxA"], device=dev)
+    sB = torch.as_tensor(d["sidxB"], device=dev)
+    S, pb = aok.rrule_batch_overlap(orb, (pA, sA), (pB, sB), ps, None)
+    for seed in range(3):
+        g = torch.Generator(device=dev).manual_seed(seed)
+        dS = 2.0 * (S - torch.randn(S.shape, generator=g, device=dev))  # queued, not waited for
+        a = pb(dS)[4]["Rnl"]["zeta"]
+        torch.cuda.synchronize()
+        b = pb(dS)[4]["Rnl"]["zeta"]
+        assert torch.equal(a, b), seed
