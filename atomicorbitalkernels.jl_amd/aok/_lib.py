@@ -225,6 +225,22 @@ def _order_inputs(device: int):
         cur.synchronize()
 
 
+def require_device(*tensors):
+    """Every tensor handed to a kernel must live on one HIP device; libaok has no CPU path."""
+    dev = None
+    for t in tensors:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise AOKArgumentError("inputs and outputs must be device (HIP) tensors; libaok has no CPU path "
+                                   "(move a basis with adapt_basis / .to(device))")
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise AOKArgumentError(f"all tensors must be on one device; got {dev} and {t.device}")
+    return dev
+
+
 _ctx = {}
 
 

@@ -110,8 +110,7 @@ def evaluate(basis: AtomicOrbitals, X, ps=None, st=None, *, sidx=None, grad=Fals
     if sidx is None:
         sidx = species_indices(basis, X)
     Rs, zeta, D, dt, nRad, K, NZ, poly, iR, iY, nOrb, L = _prep(basis, Rs, ps, st)
-    if not Rs.is_cuda:
-        raise _lib.AOKArgumentError("positions must be a device (HIP) tensor; libaok has no CPU path")
+    _lib.require_device(Rs, zeta, D, sidx if isinstance(sidx, torch.Tensor) else None)
     ctx = ctx or _lib.context(Rs.device.index or 0)
     nX = Rs.shape[0]
     P = fortran_empty((nX, nOrb), dt, Rs.device)
@@ -138,6 +137,7 @@ def _pullback(dP, basis, X, ps, st, sidx, want_x, want_ps, ctx=None):
     if tuple(dP.shape) != (nX, nOrb):
         raise _lib.AOKArgumentError(f"cotangent shape {tuple(dP.shape)} != {(nX, nOrb)}")
     dP = as_fortran(dP.to(dt))
+    _lib.require_device(Rs, zeta, D, dP, sidx if isinstance(sidx, torch.Tensor) else None)
     ctx = ctx or _lib.context(Rs.device.index or 0)
     dX = torch.empty((nX, 3), dtype=dt, device=Rs.device) if want_x else None
     dz = fortran_empty((nRad, K, NZ), dt, Rs.device) if want_ps else None

@@ -18,6 +18,8 @@ import torch
 from . import _lib
 from .basis import (AtomicOrbitals, CartesianGTOBasis, _dtype_code, as_fortran, compile_basis, compile_struct,
                     fortran_empty)
+from . import compiled as _cb
+from .compiled import CompiledBasis
 from .orbitals import NO_TANGENT, PStates
 
 
@@ -67,9 +69,12 @@ def _pair_inputs(XA, XB, zlist):
     return pA, sA, pB, sB
 
 
-def batch_overlap_(out, basis: CartesianGTOBasis, XA, XB, *, mode="fast", ctx=None):
+def batch_overlap_(out, basis, XA, XB, *, mode="fast", ctx=None):
     """batch_overlap!(out, basis, XA, XB) (overlap_2c.jl:171-197); ``out`` is (N, N, B),
-    column-major, and sets the kernel precision."""
+    column-major, and sets the kernel precision.  A CompiledBasis takes the geometry-in
+    form (kernels_2c.jl:158-168) with unit-carrying (3, B) positions."""
+    if isinstance(basis, CompiledBasis):
+        return _cb.batch_overlap_(out, basis, XA, XB, mode=mode, ctx=ctx)
     pA, sA, pB, sB = _pair_inputs(XA, XB, basis.zlist)
     B = pA.shape[0]
     N = basis.nbf_total
@@ -81,6 +86,7 @@ def batch_overlap_(out, basis: CartesianGTOBasis, XA, XB, *, mode="fast", ctx=No
     pA, pB = pA.to(dt), pB.to(dt)
     coef = as_fortran(basis.coef.to(dt))
     zeta = as_fortran(basis.zeta.to(dt))
+    _lib.require_device(out, pA, pB, sA, sB, coef, zeta)
     ctx = ctx or _lib.context(out.device.index or 0)
     a, it = _sidx_ptr(sA)
     b, _ = _sidx_ptr(sB)
@@ -96,7 +102,10 @@ def _out_tensor(N, B, dtype, device):
 
 def batch_overlap(basis, XA, XB, ps=None, st=None, *, mode="fast", dtype=None, ctx=None):
     """batch_overlap(basis::CartesianGTOBasis, XA, XB) or batch_overlap(orb, XA, XB[, ps, st]).
-    The output precision follows the positions (promoted with (zeta, D) in the Lux form)."""
+    The output precision follows the positions (promoted with (zeta, D) in the Lux form).
+    A CompiledBasis takes the geometry-in form (kernels_2c.jl:170-184), FP64 by default."""
+    if isinstance(basis, CompiledBasis):
+        return _cb.batch_overlap(basis, XA, XB, dtype=dtype or torch.float64, mode=mode, ctx=ctx)
     if isinstance(basis, CartesianGTOBasis):
         pA = XA.positions if isinstance(XA, PStates) else XA[0]
         dt = dtype or pA.dtype
@@ -119,6 +128,7 @@ def batch_overlap(basis, XA, XB, ps=None, st=None, *, mode="fast", dtype=None, c
     zeta, D = as_fortran(zeta.to(dt)), as_fortran(D.to(dt))
     pA, pB = pA.to(dt), pB.to(dt)
     B = pA.shape[0]
+    _lib.require_device(pA, pB, sA, sB, zeta, D)
     out = _out_tensor(off[-1], B, dt, pA.device)
     ctx = ctx or _lib.context(pA.device.index or 0)
     a, it = _sidx_ptr(sA)
@@ -143,6 +153,7 @@ def overlap_pb_coef_alpha(dS, basis: CartesianGTOBasis, XA, XB, *, mode="fast", 
     zeta = as_fortran(basis.zeta.to(dt))
     ns, K, NZ = zeta.shape
     dc = fortran_empty(zeta.shape, dt, dS.device)
+    _lib.require_device(dS, pA, pB, sA, sB, coef, zeta)
     da = fortran_empty(zeta.shape, dt, dS.device)
     ctx = ctx or _lib.context(dS.device.index or 0)
     a, it = _sidx_ptr(sA)
@@ -167,6 +178,7 @@ def pullback_ps(dS, orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", ct
     pA, pB = pA.to(dt), pB.to(dt)
     ns, K, NZ = zeta.shape
     dz = fortran_empty(zeta.shape, dt, dS.device)
+    _lib.require_device(dS, pA, pB, sA, sB, zeta, D)
     dD = fortran_empty(zeta.shape, dt, dS.device)
     ctx = ctx or _lib.context(dS.device.index or 0)
     a, it = _sidx_ptr(sA)
@@ -183,6 +195,7 @@ class Ovl2cPlan:
     Holds the species-index and parameter tensors it was built from."""
 
     def __init__(self, ls, sA, sB, B, p1, p2, params, lengthscale, dtype, mode="fast", ctx=None):
+        _lib.require_device(sA, sB, p1, p2)
         self.ctx = ctx or _lib.context(p1.device.index or 0)
         self._keep = (sA, sB, p1, p2)
         self.dtype = dtype
@@ -200,12 +213,14 @@ class Ovl2cPlan:
         self.handle = h
 
     def forward(self, pA, pB, out):
+        _lib.require_device(pA, pB, out)
         _lib._order_inputs(self.ctx.device)
         rc = self.ctx.lib.aok_ovl2c_plan_forward(self.handle, pA.data_ptr(), pB.data_ptr(), out.data_ptr())
         self.ctx.check(rc, "aok_ovl2c_plan_forward")
         return out
 
     def pullback(self, dS, pA, pB):
+        _lib.require_device(dS, pA, pB)
         d1 = fortran_empty(self.shape, self.dtype, dS.device)
         d2 = fortran_empty(self.shape, self.dtype, dS.device)
         _lib._order_inputs(self.ctx.device)
@@ -254,10 +269,13 @@ def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast"
     return S, pullback
 
 
-def batch_overlap_3c_(out, basis: CartesianGTOBasis, XA, XB, XC, *, mode="fast", ctx=None):
+def batch_overlap_3c_(out, basis, XA, XB, XC, *, mode="fast", ctx=None):
     """batch_overlap_3c!(out, basis, posA, posB, posC) (src/integrals/kernels_3c.jl:183-200),
     species-aware: XA/XB/XC are PStates or (positions, sidx) tuples, positions in the
-    basis' length unit.  ``out`` is (N, N, N, B), column-major."""
+    basis' length unit.  ``out`` is (N, N, N, B), column-major.  A CompiledBasis takes the
+    reference's own single-basis form with unit-carrying (3, B) positions."""
+    if isinstance(basis, CompiledBasis):
+        return _cb.batch_overlap_3c_(out, basis, XA, XB, XC, mode=mode, ctx=ctx)
     pA, sA = _unpack(XA, basis.zlist, "XA")
     pB, sB = _unpack(XB, basis.zlist, "XB")
     pC, sC = _unpack(XC, basis.zlist, "XC")
@@ -272,6 +290,7 @@ def batch_overlap_3c_(out, basis: CartesianGTOBasis, XA, XB, XC, *, mode="fast",
     coef = as_fortran(basis.coef.to(dt))
     zeta = as_fortran(basis.zeta.to(dt))
     ns, K, NZ = zeta.shape
+    _lib.require_device(out, pA, pB, pC, sA, sB, sC, coef, zeta)
     ctx = ctx or _lib.context(out.device.index or 0)
     a, it = _sidx_ptr(sA)
     b, _ = _sidx_ptr(sB)
@@ -282,8 +301,10 @@ def batch_overlap_3c_(out, basis: CartesianGTOBasis, XA, XB, XC, *, mode="fast",
     return out
 
 
-def batch_overlap_3c(basis: CartesianGTOBasis, XA, XB, XC, *, mode="fast", dtype=None, ctx=None):
+def batch_overlap_3c(basis, XA, XB, XC, *, mode="fast", dtype=None, ctx=None):
     """batch_overlap_3c(basis, posA, posB, posC) -> (N, N, N, B) (kernels_3c.jl:213-222)."""
+    if isinstance(basis, CompiledBasis):
+        return _cb.batch_overlap_3c(basis, XA, XB, XC, dtype=dtype or torch.float64, mode=mode, ctx=ctx)
     pA = XA.positions if isinstance(XA, PStates) else XA[0]
     dt = dtype or pA.dtype
     N = basis.nbf_total
