@@ -12,20 +12,21 @@ from ._lib import (AOKArgumentError, AOKDeviceError, AOKError, AOKLibraryError, 
 from .basis import (ANG2BOHR, AtomicOrbitals, CartesianGTOBasis, GaussianTypeRadials, SlaterTypeRadials, as_fortran,
                     compile_basis, compile_coef, compile_coef_pb, fortran_from_numpy, fortran_zeros,
                     gaussian_orbitals, length_factor, slater_orbitals)
+from .basisset import BasisSet, Shell, atomic_number, basis_set, normalise_shell, read_gbs
 from .compiled import UNITFUL_HINT, CompiledBasis, Length, adapt_basis, compile_shells, to_bohr
 from .orbitals import (NO_TANGENT, NoTangent, PStates, evaluate, evaluate_ed, pullback_ps, pullback_x,
                        rrule_evaluate)
-from .overlap import (Ovl2cPlan, batch_overlap, batch_overlap_, batch_overlap_3c, batch_overlap_3c_, overlap_pb_coef_alpha,
-                      rrule_batch_overlap)
+from .overlap import (Ovl2cPlan, batch_overlap, batch_overlap_, batch_overlap_3c, batch_overlap_3c_,
+                      overlap_pb_coef_alpha, rrule_batch_overlap)
 from .overlap import pullback_ps as overlap_pullback_ps
 
 __all__ = [
-    "ANG2BOHR", "AOKArgumentError", "CompiledBasis", "Length", "UNITFUL_HINT", "adapt_basis", "compile_shells",
-    "to_bohr", "AOKDeviceError", "AOKError", "AOKLibraryError", "AOKLmaxError",
-    "AOKSpeciesError", "AtomicOrbitals", "CartesianGTOBasis", "Context", "GaussianTypeRadials", "NO_TANGENT",
-    "NoTangent", "Ovl2cPlan", "PStates", "SlaterTypeRadials", "as_fortran", "batch_overlap", "batch_overlap_", "batch_overlap_3c",
-    "batch_overlap_3c_", "compile_basis",
-    "compile_coef", "compile_coef_pb", "context", "evaluate", "evaluate_ed", "fortran_from_numpy", "fortran_zeros",
-    "gaussian_orbitals", "length_factor", "overlap_pb_coef_alpha", "overlap_pullback_ps", "pullback_ps",
-    "pullback_x", "rrule_batch_overlap", "rrule_evaluate", "slater_orbitals",
+    "ANG2BOHR", "AOKArgumentError", "AOKDeviceError", "AOKError", "AOKLibraryError", "AOKLmaxError",
+    "AOKSpeciesError", "AtomicOrbitals", "BasisSet", "CartesianGTOBasis", "CompiledBasis", "Context",
+    "GaussianTypeRadials", "Length", "NO_TANGENT", "NoTangent", "Ovl2cPlan", "PStates", "Shell", "SlaterTypeRadials",
+    "UNITFUL_HINT", "adapt_basis", "as_fortran", "atomic_number", "basis_set", "batch_overlap", "batch_overlap_",
+    "batch_overlap_3c", "batch_overlap_3c_", "compile_basis", "compile_coef", "compile_coef_pb", "compile_shells",
+    "context", "evaluate", "evaluate_ed", "fortran_from_numpy", "fortran_zeros", "gaussian_orbitals",
+    "length_factor", "normalise_shell", "overlap_pb_coef_alpha", "overlap_pullback_ps", "pullback_ps", "pullback_x",
+    "read_gbs", "rrule_batch_overlap", "rrule_evaluate", "slater_orbitals", "to_bohr",
 ]

@@ -196,8 +196,20 @@ def _example_radial(TR, N1, N2, K=1, dtype=torch.float64, zlist=(1,), device="cp
     return TR(fortran_from_numpy(zeta, dtype, device), fortran_from_numpy(D, dtype, device), spec, nnspec, zlist)
 
 
-def gaussian_orbitals(N1=4, N2=3, *, length_unit, K=1, dtype=torch.float64, nspecies=1, zlist=None, device="cpu"):
-    """gaussian_orbitals (src/orbitals/utils.jl:47-51)."""
+def gaussian_orbitals(N1=4, N2=3, *, length_unit, K=1, dtype=torch.float64, nspecies=1, zlist=None, device="cpu",
+                      path=None):
+    """gaussian_orbitals (src/orbitals/utils.jl:47-51): the synthetic (N1, N2) basis.
+    Also the basis-set forms of src/orbitals/gaussianbasis.jl: ``gaussian_orbitals(bs)``
+    for a BasisSet, ``gaussian_orbitals(name, elements)`` and
+    ``gaussian_orbitals([(element, name), ...])``."""
+    from . import basisset as _bs
+
+    if isinstance(N1, _bs.BasisSet):
+        return _bs.orbitals_from_basis_set(N1, length_unit=length_unit, dtype=dtype, device=device)
+    if isinstance(N1, str):
+        return _bs.orbitals_from_name(N1, N2, length_unit=length_unit, dtype=dtype, device=device, path=path)
+    if isinstance(N1, (list, tuple)):
+        return _bs.orbitals_from_pairs(N1, length_unit=length_unit, dtype=dtype, device=device, path=path)
     zl = tuple(zlist) if zlist is not None else tuple(range(1, nspecies + 1))
     return AtomicOrbitals(_example_radial(GaussianTypeRadials, N1, N2, K, dtype, zl, device), N1 - 1,
                           length_unit=length_unit)
