@@ -86,6 +86,9 @@ _SIGS = {
     "aok_create": [ctypes.POINTER(_vp), _int, _vp],
     "aok_destroy": [_vp],
     "aok_set_timing": [_vp, _int],
+    "aok_set_blocking": [_vp, _int],
+    "aok_sync": [_vp],
+    "aok_kernel_times": [_vp, _i64, ctypes.POINTER(_dbl), ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_i64)],
     "aok_tune": [_vp, ctypes.c_char_p, _i64],
     "aok_tune_get": [_vp, ctypes.c_char_p, ctypes.POINTER(_i64)],
     "aok_ao_eval": [_vp, _int, _vp, _vp, _int, _i64, _vp, _vp, _i64, _i64, _i64, _i64p, _i64p, _i64p, _i64,
@@ -112,7 +115,7 @@ _SIGS = {
 
 # Every symbol include/aok.h declares (checked by the CPU test suite).
 EXPORTED = sorted(list(_SIGS) + ["aok_last_error", "aok_stream", "aok_version", "aok_last_kernel_ms",
-                                 "aok_last_kernel_name"])
+                                 "aok_last_kernel_name", "aok_get_blocking"])
 
 _lib = None
 _lock = threading.Lock()
@@ -146,6 +149,8 @@ def load():
         L.aok_last_kernel_ms.restype = _dbl
         L.aok_last_kernel_name.argtypes = [_vp]
         L.aok_last_kernel_name.restype = ctypes.c_char_p
+        L.aok_get_blocking.argtypes = [_vp]
+        L.aok_get_blocking.restype = _int
         _lib = L
     return _lib
 
@@ -198,6 +203,29 @@ class Context:
 
     def set_timing(self, on: bool):
         self.lib.aok_set_timing(self.handle, 1 if on else 0)
+
+    @property
+    def blocking(self) -> bool:
+        return self.lib.aok_get_blocking(self.handle) == 1
+
+    def set_blocking(self, on: bool):
+        """Blocking (default, the reference's semantics) or stream-ordered execution
+        (aok_set_blocking): in stream-ordered mode calls return once enqueued on the
+        context stream, torch's default stream, and device-detected errors surface at
+        ``sync()``."""
+        self.check(self.lib.aok_set_blocking(self.handle, 1 if on else 0), "aok_set_blocking")
+
+    def sync(self):
+        """Wait for this context's work and raise any latched device-side error."""
+        self.check(self.lib.aok_sync(self.handle), "aok_sync")
+
+    def kernel_times(self, cap: int = 65536):
+        """[(kernel name, ms), ...] of the timed launches not yet returned, oldest first."""
+        ms = (_dbl * cap)()
+        names = (ctypes.c_char_p * cap)()
+        n = _i64(0)
+        self.check(self.lib.aok_kernel_times(self.handle, cap, ms, names, ctypes.byref(n)), "aok_kernel_times")
+        return [(names[i].decode(), ms[i]) for i in range(n.value)]
 
     def tune(self, key: str, value: int):
         """Set a launch-shape knob (see aok_tune in include/aok.h)."""

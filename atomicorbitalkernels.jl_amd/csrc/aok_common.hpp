@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -32,10 +33,22 @@ struct aok_context {
     // Error word in host-mapped pinned memory: h_err (host view), d_err (device view).
     unsigned *d_err = nullptr;
     unsigned *h_err = nullptr;
-    // Optional timing of the dominant kernel of a call.
+    // Blocking (default): every entry point waits for its work and reports device-side
+    // errors.  Stream-ordered: entry points return once the work is enqueued; device-side
+    // error flags accumulate in the error word until aok_sync.
+    bool blocking = true;
+    // Optional timing of the dominant kernel of each call: an event pair per timed launch.
+    // Pairs wait in `pend` until read (at the closing synchronisation of a blocking call,
+    // or at aok_sync / aok_kernel_times / aok_last_kernel_ms); read times go to `done`.
+    struct Timed {
+        const char *name;
+        hipEvent_t a, b;
+    };
     bool timing = false;
-    bool timed = false;  // the current call recorded ev0/ev1
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool open = false;  // time_begin recorded pend.back().a, time_end not yet
+    std::vector<Timed> pend;
+    std::vector<hipEvent_t> evpool;
+    std::deque<std::pair<const char *, double>> done;
     double last_ms = 0.0;
     const char *last_kernel = "";
     // Tuning knobs (0 = built-in default), set by aok_tune or the autotuner.
@@ -76,18 +89,10 @@ aok_status clear_err(aok_ctx ctx);
 aok_status finish(aok_ctx ctx, const char *what);
 
 // Event-bracketed launch of the dominant kernel (only when timing is enabled).
-inline void time_begin(aok_ctx ctx, const char *name)
-{
-    if (ctx->timing) {
-        (void)hipEventRecord(ctx->ev0, ctx->stream);
-        ctx->last_kernel = name;
-        ctx->timed = true;
-    }
-}
-inline void time_end(aok_ctx ctx)
-{
-    if (ctx->timing) (void)hipEventRecord(ctx->ev1, ctx->stream);
-}
+void time_begin(aok_ctx ctx, const char *name);
+void time_end(aok_ctx ctx);
+// Wait for every pending timed launch and move its time to ctx->done.
+void collect_times(aok_ctx ctx);
 
 // Device helpers ----------------------------------------------------------------
 // 1-based species index (int64 or int32) -> 0-based sigma, -1 for NULL input.

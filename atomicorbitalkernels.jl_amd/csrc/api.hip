@@ -27,28 +27,89 @@ aok_status ensure_ws(aok_ctx ctx, size_t bytes)
     return AOK_OK;
 }
 
-// The error word lives in host-mapped pinned memory: the host clears it before the
-// call's first launch, kernels store flags into it (plain stores, the same value from
-// every writer), and the host reads it after the call's closing synchronisation.
+// The error word lives in host-mapped pinned memory: kernels store flags into it (plain
+// stores, the same value from every writer) and the host reads it after a
+// synchronisation.  Blocking calls clear it before their first launch and read it after
+// their closing synchronisation; in stream-ordered mode it accumulates the flags of
+// every call until aok_sync reads and clears it.
 aok_status clear_err(aok_ctx ctx)
 {
-    *reinterpret_cast<volatile unsigned *>(ctx->h_err) = 0u;
-    ctx->timed = false;
-    ctx->last_ms = 0.0;
-    ctx->last_kernel = "";
+    if (ctx->blocking) {
+        *reinterpret_cast<volatile unsigned *>(ctx->h_err) = 0u;
+        // a blocking call reports its own timed kernel (none: 0)
+        ctx->last_ms = 0.0;
+        ctx->last_kernel = "";
+    }
     return AOK_OK;
+}
+
+static hipEvent_t take_event(aok_ctx ctx)
+{
+    if (!ctx->evpool.empty()) {
+        hipEvent_t e = ctx->evpool.back();
+        ctx->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+void time_begin(aok_ctx ctx, const char *name)
+{
+    ctx->open = false;
+    if (!ctx->timing) return;
+    hipEvent_t a = take_event(ctx), b = a ? take_event(ctx) : nullptr;
+    if (!a || !b || hipEventRecord(a, ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        if (a) ctx->evpool.push_back(a);
+        if (b) ctx->evpool.push_back(b);
+        return;  // this launch goes untimed
+    }
+    ctx->pend.push_back({name, a, b});
+    ctx->open = true;
+}
+
+void time_end(aok_ctx ctx)
+{
+    if (!ctx->open) return;
+    ctx->open = false;
+    if (hipEventRecord(ctx->pend.back().b, ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->evpool.push_back(ctx->pend.back().a);
+        ctx->evpool.push_back(ctx->pend.back().b);
+        ctx->pend.pop_back();
+    }
+}
+
+void collect_times(aok_ctx ctx)
+{
+    constexpr size_t DONE_CAP = 1 << 16;
+    for (const auto &t : ctx->pend) {
+        float ms = 0.f;
+        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            ctx->last_ms = ms;
+            ctx->last_kernel = t.name;
+            ctx->done.emplace_back(t.name, static_cast<double>(ms));
+            if (ctx->done.size() > DONE_CAP) ctx->done.pop_front();
+        } else {
+            (void)hipGetLastError();  // do not leave a timing failure for the caller's next HIP call
+        }
+        ctx->evpool.push_back(t.a);
+        ctx->evpool.push_back(t.b);
+    }
+    ctx->pend.clear();
 }
 
 aok_status finish(aok_ctx ctx, const char *what)
 {
+    // stream-ordered: return with the work enqueued (timed launches are read later)
+    if (!ctx->blocking) return AOK_OK;
     AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->timing && ctx->timed) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess)
-            ctx->last_ms = ms;
-        else
-            (void)hipGetLastError();  // do not leave a timing failure for the caller's next HIP call
-    }
+    collect_times(ctx);
     if (*reinterpret_cast<volatile unsigned *>(ctx->h_err) & ERR_SPECIES)
         return fail(ctx, AOK_ERR_SPECIES, std::string(what) + ": species index outside 1..NZ");
     return AOK_OK;
@@ -91,11 +152,11 @@ aok_status aok_create(aok_ctx *out, int device, void *stream)
         c->own_stream = true;
     }
     if (hipHostMalloc(&c->h_err, sizeof(unsigned), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
-        hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_err), c->h_err, 0) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_err), c->h_err, 0) != hipSuccess) {
         aok_destroy(c);
         return AOK_ERR_DEVICE;
     }
+    *reinterpret_cast<volatile unsigned *>(c->h_err) = 0u;  // pinned memory comes uncleared
     *out = c;
     return AOK_OK;
 }
@@ -111,8 +172,11 @@ aok_status aok_destroy(aok_ctx ctx)
         p->buf = nullptr, p->bytes = 0, p->ctx = nullptr;
     }
     if (ctx->h_err) (void)hipHostFree(ctx->h_err);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (const auto &t : ctx->pend) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return AOK_OK;
@@ -124,9 +188,36 @@ void *aok_stream(aok_ctx ctx) { return ctx ? static_cast<void *>(ctx->stream) : 
 
 const char *aok_version(void) { return "aok-mi355x 0.1.0 (gfx950)"; }
 
-double aok_last_kernel_ms(aok_ctx ctx) { return ctx ? ctx->last_ms : 0.0; }
+double aok_last_kernel_ms(aok_ctx ctx)
+{
+    if (!ctx) return 0.0;
+    collect_times(ctx);
+    return ctx->last_ms;
+}
 
-const char *aok_last_kernel_name(aok_ctx ctx) { return ctx ? ctx->last_kernel : ""; }
+const char *aok_last_kernel_name(aok_ctx ctx)
+{
+    if (!ctx) return "";
+    collect_times(ctx);
+    return ctx->last_kernel;
+}
+
+aok_status aok_kernel_times(aok_ctx ctx, int64_t cap, double *ms, const char **names, int64_t *count)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, cap >= 0 && count != nullptr && (cap == 0 || (ms != nullptr && names != nullptr)),
+                "aok_kernel_times: bad arguments");
+    collect_times(ctx);
+    int64_t n = 0;
+    while (n < cap && !ctx->done.empty()) {
+        names[n] = ctx->done.front().first;
+        ms[n] = ctx->done.front().second;
+        ctx->done.pop_front();
+        ++n;
+    }
+    *count = n;
+    return AOK_OK;
+}
 
 aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
 {
@@ -166,6 +257,32 @@ aok_status aok_set_timing(aok_ctx ctx, int enable)
     ctx->timing = enable != 0;
     return AOK_OK;
 }
+
+aok_status aok_sync(aok_ctx ctx)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    collect_times(ctx);
+    volatile unsigned *e = reinterpret_cast<volatile unsigned *>(ctx->h_err);
+    const unsigned flags = *e;
+    *e = 0u;
+    if (flags & ERR_SPECIES) return fail(ctx, AOK_ERR_SPECIES, "species index outside 1..NZ (an earlier call)");
+    return AOK_OK;
+}
+
+aok_status aok_set_blocking(aok_ctx ctx, int blocking)
+{
+    AOK_CHECK_CTX(ctx);
+    // leaving stream-ordered mode reports what the enqueued work flagged; entering it
+    // starts from a clear error word (blocking calls have reported their own flags)
+    const aok_status st = ctx->blocking ? AOK_OK : aok_sync(ctx);
+    if (ctx->blocking && !blocking) *reinterpret_cast<volatile unsigned *>(ctx->h_err) = 0u;
+    ctx->blocking = blocking != 0;
+    return st;
+}
+
+int aok_get_blocking(aok_ctx ctx) { return ctx ? (ctx->blocking ? 1 : 0) : -1; }
 
 // ---- H1 --------------------------------------------------------------------------------
 static aok_status ao_common(aok_ctx ctx, aok_dtype dtype, const void *X, aok_itype itype, int64_t nX,
@@ -447,14 +564,18 @@ aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan)
         (void)hipFree(buf);
         return AOK_OK;
     }
-    // Keep the larger of the two buffers for the next plan; every call is blocking, so
-    // no work on the context stream still reads either.
+    // Keep the larger of the two buffers for the next plan (whose work is ordered after
+    // this plan's on the context stream).  The other is freed once the stream has drained:
+    // in stream-ordered mode enqueued work may still read it.
     (void)hipSetDevice(ctx->device);
+    void *drop = buf;
     if (bytes > ctx->plan_cache_bytes) {
-        if (ctx->plan_cache) (void)hipFree(ctx->plan_cache);
+        drop = ctx->plan_cache;
         ctx->plan_cache = buf, ctx->plan_cache_bytes = bytes;
-    } else {
-        (void)hipFree(buf);
+    }
+    if (drop) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(drop);
     }
     return AOK_OK;
 }

@@ -229,12 +229,33 @@ aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value);
 
 /* ---- measurement hooks ----------------------------------------------------------------- */
 /* Device time (ms) of the dominant kernel of the most recent call on this context, taken
- * with HIP events on the context stream around that kernel's launch (0 if none). */
+ * with HIP events on the context stream around that kernel's launch (0 if none; in
+ * stream-ordered mode the most recent timed launch, waited for). */
 double aok_last_kernel_ms(aok_ctx ctx);
 /* Name of that kernel. */
 const char *aok_last_kernel_name(aok_ctx ctx);
-/* Enable/disable the per-call kernel timing events (default off). */
+/* Every timed launch not yet returned, oldest first: waits for them, writes up to `cap`
+ * (name, ms) pairs and removes those from the record; *count = pairs written.  Names
+ * are static strings.  The record keeps the latest 65536 launches. */
+aok_status aok_kernel_times(aok_ctx ctx, int64_t cap, double *ms, const char **names, int64_t *count);
+/* Enable/disable the per-call kernel timing events (default off).  A timed call waits
+ * for its work in either execution mode. */
 aok_status aok_set_timing(aok_ctx ctx, int enable);
+
+/* ---- execution mode ---------------------------------------------------------------------- */
+/* No reference counterpart.  Blocking (the default, and the reference's semantics: every
+ * wrapper ends in KA.synchronize) -- each entry point returns after its work has finished
+ * and reports device-detected errors (species index out of range) itself.
+ * Stream-ordered (blocking = 0) -- each entry point returns once its work is enqueued on
+ * the context stream; argument errors are still reported at once, device-detected errors
+ * are latched and reported by aok_sync (or when blocking mode is switched back on).
+ * Outputs may be read only after the stream has reached them; inputs must stay valid and
+ * unchanged until then.  Leaving stream-ordered mode synchronises. */
+aok_status aok_set_blocking(aok_ctx ctx, int blocking);
+/* 1 blocking, 0 stream-ordered, -1 for a NULL context. */
+int aok_get_blocking(aok_ctx ctx);
+/* Wait for all work enqueued by this context; report and clear latched device errors. */
+aok_status aok_sync(aok_ctx ctx);
 
 #ifdef __cplusplus
 }
