@@ -104,6 +104,9 @@ _SIGS = {
                            _dbl, _vp, _vp],
     "aok_ovl2c_pullback_ps": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp,
                               _vp, _dbl, _vp, _vp],
+    "aok_ovl2c_pullback_x": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp,
+                             _vp, _dbl, _vp, _vp],
+    "aok_ovl2c_plan_pullback_x": [_vp, _vp, _vp, _vp, _vp, _vp],
     "aok_ovl2c_plan_create": [_vp, _int, _uint, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _int, _vp, _vp,
                               _dbl, ctypes.POINTER(_vp)],
     "aok_ovl2c_plan_forward": [_vp, _vp, _vp, _vp],

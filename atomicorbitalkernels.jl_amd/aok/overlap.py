@@ -164,6 +164,33 @@ def overlap_pb_coef_alpha(dS, basis: CartesianGTOBasis, XA, XB, *, mode="fast", 
     return dc, da
 
 
+def overlap_pullback_x(dS, basis: CartesianGTOBasis, XA, XB, *, ctx=None):
+    """Position cotangents of batch_overlap(basis, XA, XB) (aok_ovl2c_pullback_x; no
+    reference counterpart) -> (dXA, dXB), each (B, 3), in the basis' length unit;
+    dXB = -dXA."""
+    pA, sA, pB, sB = _pair_inputs(XA, XB, basis.zlist)
+    dt = torch.promote_types(dS.dtype, basis.coef.dtype)
+    dS = as_fortran(dS.to(dt))
+    B = pA.shape[0]
+    N = basis.nbf_total
+    if tuple(dS.shape) != (N, N, B):
+        raise _lib.AOKArgumentError(f"dS must be {(N, N, B)}")
+    pA, pB = pA.to(dt), pB.to(dt)
+    coef = as_fortran(basis.coef.to(dt))
+    zeta = as_fortran(basis.zeta.to(dt))
+    ns, K, NZ = zeta.shape
+    _lib.require_device(dS, pA, pB, sA, sB, coef, zeta)
+    dA = torch.empty((B, 3), dtype=dt, device=dS.device)
+    dB = torch.empty((B, 3), dtype=dt, device=dS.device)
+    ctx = ctx or _lib.context(dS.device.index or 0)
+    a, it = _sidx_ptr(sA)
+    b, _ = _sidx_ptr(sB)
+    ctx.call("aok_ovl2c_pullback_x", _dtype_code(dt), _lib.AOK_FAST, dS.data_ptr(), pA.data_ptr(), pB.data_ptr(), a,
+             b, it, B, ns, K, NZ, _lib.i64_array(basis.ls), coef.data_ptr(), zeta.data_ptr(),
+             float(basis.lengthscale), dA.data_ptr(), dB.data_ptr())
+    return dA, dB
+
+
 def pullback_ps(dS, orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", ctx=None):
     """pullback_ps(dS, orb, XA, XB, ps, st) (overlap_2c_grad.jl:296-304)."""
     ls, _, off = compile_struct(orb)
@@ -229,6 +256,17 @@ class Ovl2cPlan:
         self.ctx.check(rc, "aok_ovl2c_plan_pullback")
         return d1, d2
 
+    def pullback_x(self, dS, pA, pB):
+        """Position cotangents (dA, dB), each (B, 3), of dS (aok_ovl2c_plan_pullback_x)."""
+        _lib.require_device(dS, pA, pB)
+        dA = torch.empty((self.B, 3), dtype=self.dtype, device=dS.device)
+        dB = torch.empty((self.B, 3), dtype=self.dtype, device=dS.device)
+        _lib._order_inputs(self.ctx.device)
+        rc = self.ctx.lib.aok_ovl2c_plan_pullback_x(self.handle, dS.data_ptr(), pA.data_ptr(), pB.data_ptr(),
+                                                    dA.data_ptr(), dB.data_ptr())
+        self.ctx.check(rc, "aok_ovl2c_plan_pullback_x")
+        return dA, dB
+
     def close(self):
         if getattr(self, "handle", None):
             self.ctx.lib.aok_ovl2c_plan_destroy(self.handle)
@@ -241,10 +279,14 @@ class Ovl2cPlan:
             pass
 
 
-def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", dtype=None, ctx=None):
+def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast", dtype=None, ctx=None,
+                        wrt_positions=False):
     """rrule(batch_overlap, orb, XA, XB, ps, st) (overlap_2c_grad.jl:309-318) -> (S, pullback);
     pullback(dS) -> (NoTangent, NoTangent, NoTangent, NoTangent, dps, NoTangent).  Forward
-    and pullback share one plan (coefficient compilation, bucketing, pair records)."""
+    and pullback share one plan (coefficient compilation, bucketing, pair records).
+    With ``wrt_positions`` (no reference counterpart: its rrule has NoTangent there) the
+    third and fourth slots hold the position cotangents dXA, dXB as (B, 3) tensors in the
+    input length unit (aok_ovl2c_plan_pullback_x)."""
     if orb.Rnl.decay != _lib.AOK_DECAY_GAUSSIAN:
         raise _lib.AOKArgumentError("the overlap kernels integrate plain Gaussians")
     ls, _, off = compile_struct(orb)
@@ -262,8 +304,12 @@ def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast"
     def pullback(dS):
         if tuple(dS.shape) != (N, N, B):
             raise _lib.AOKArgumentError(f"dS must be {(N, N, B)}")
-        dz, dD = plan.pullback(as_fortran(dS.to(dt)), pA, pB)
+        g = as_fortran(dS.to(dt))
+        dz, dD = plan.pullback(g, pA, pB)
         dps = {"Rnl": {"zeta": dz, "D": dD}, "Ylm": {}}
+        if wrt_positions:
+            dXA, dXB = plan.pullback_x(g, pA, pB)
+            return NO_TANGENT, NO_TANGENT, dXA, dXB, dps, NO_TANGENT
         return NO_TANGENT, NO_TANGENT, NO_TANGENT, NO_TANGENT, dps, NO_TANGENT
 
     return S, pullback

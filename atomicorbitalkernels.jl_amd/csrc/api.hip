@@ -375,8 +375,10 @@ aok_status aok_compile_coef_pb(aok_ctx ctx, aok_dtype dtype, const void *dcoef, 
 }
 
 // ---- H2 2-centre ------------------------------------------------------------------------
-static aok_status ovl2c_entry(aok_ctx ctx, aok_dtype dtype, Ovl2cCall &c, bool pullback, const char *what)
+// op: 0 forward, 1 parameter pullback, 2 position cotangents
+static aok_status ovl2c_entry(aok_ctx ctx, aok_dtype dtype, Ovl2cCall &c, int op, const char *what)
 {
+    const bool pullback = op != 0;
     AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
     AOK_REQUIRE(ctx, !bad_itype(static_cast<aok_itype>(c.itype)), "itype must be AOK_I64 or AOK_I32");
     AOK_REQUIRE(ctx, (c.flags & ~AOK_PLAIN) == 0u, "unknown flags");
@@ -390,7 +392,10 @@ static aok_status ovl2c_entry(aok_ctx ctx, aok_dtype dtype, Ovl2cCall &c, bool p
     AOK_HIP(ctx, hipSetDevice(ctx->device));
     aok_status st = clear_err(ctx);
     if (st != AOK_OK) return st;
-    st = dtype == AOK_F32 ? ovl2c_run<float>(ctx, c, pullback) : ovl2c_run<double>(ctx, c, pullback);
+    if (op == 2)
+        st = dtype == AOK_F32 ? ovl2c_dx_run<float>(ctx, c) : ovl2c_dx_run<double>(ctx, c);
+    else
+        st = dtype == AOK_F32 ? ovl2c_run<float>(ctx, c, pullback) : ovl2c_run<double>(ctx, c, pullback);
     if (st != AOK_OK) return st;
     return finish(ctx, what);
 }
@@ -406,7 +411,7 @@ aok_status aok_ovl2c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *p
     c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
     c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.coef = coef, c.alpha = alpha, c.mode = 0, c.scale = lengthscale;
     c.out = out;
-    return ovl2c_entry(ctx, dtype, c, false, "aok_ovl2c");
+    return ovl2c_entry(ctx, dtype, c, 0, "aok_ovl2c");
 }
 
 aok_status aok_ovl2c_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *posA, const void *posB,
@@ -420,7 +425,7 @@ aok_status aok_ovl2c_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void
     c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
     c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.alpha = zeta, c.D = D, c.mode = 1, c.scale = lengthscale;
     c.out = out;
-    return ovl2c_entry(ctx, dtype, c, false, "aok_ovl2c_ps");
+    return ovl2c_entry(ctx, dtype, c, 0, "aok_ovl2c_ps");
 }
 
 aok_status aok_ovl2c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dS, const void *posA,
@@ -435,7 +440,22 @@ aok_status aok_ovl2c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, cons
     c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
     c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.coef = coef, c.alpha = alpha, c.mode = 0, c.scale = lengthscale;
     c.dS = dS, c.o1 = dcoef, c.o2 = dalpha, c.vjp = false;
-    return ovl2c_entry(ctx, dtype, c, true, "aok_ovl2c_pullback");
+    return ovl2c_entry(ctx, dtype, c, 1, "aok_ovl2c_pullback");
+}
+
+aok_status aok_ovl2c_pullback_x(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dS, const void *posA,
+                                const void *posB, const void *sidxA, const void *sidxB, aok_itype itype, int64_t B,
+                                int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, const void *coef,
+                                const void *alpha, double lengthscale, void *dA, void *dB)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, B == 0 || dA != nullptr, "dA is NULL");
+    Ovl2cCall c{};
+    c.flags = flags;
+    c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
+    c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.coef = coef, c.alpha = alpha, c.mode = 0, c.scale = lengthscale;
+    c.dS = dS, c.dA = dA, c.dB = dB;
+    return ovl2c_entry(ctx, dtype, c, 2, "aok_ovl2c_pullback_x");
 }
 
 aok_status aok_ovl2c_pullback_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dS, const void *posA,
@@ -450,7 +470,7 @@ aok_status aok_ovl2c_pullback_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, c
     c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
     c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.alpha = zeta, c.D = D, c.mode = 1, c.scale = lengthscale;
     c.dS = dS, c.o1 = dzeta, c.o2 = dD, c.vjp = true;
-    return ovl2c_entry(ctx, dtype, c, true, "aok_ovl2c_pullback_ps");
+    return ovl2c_entry(ctx, dtype, c, 1, "aok_ovl2c_pullback_ps");
 }
 
 // ---- H2 2-centre plans -------------------------------------------------------------------
@@ -542,6 +562,25 @@ aok_status aok_ovl2c_plan_pullback(aok_ovl2c_plan plan, const void *dS, const vo
     Ovl2cCall io{};
     io.posA = posA, io.posB = posB, io.dS = dS, io.o1 = d1, io.o2 = d2, io.scale = plan->st.call.scale;
     return plan_exec(plan, io, true, "aok_ovl2c_plan_pullback");
+}
+
+aok_status aok_ovl2c_plan_pullback_x(aok_ovl2c_plan plan, const void *dS, const void *posA, const void *posB,
+                                     void *dA, void *dB)
+{
+    if (!plan || !plan->st.ctx) return AOK_ERR_ARG;
+    aok_ctx ctx = plan->st.ctx;
+    const int64_t B = plan->st.call.B;
+    AOK_REQUIRE(ctx, B == 0 || (posA && posB), "positions are NULL");
+    AOK_REQUIRE(ctx, B == 0 || dS != nullptr, "dS is NULL");
+    AOK_REQUIRE(ctx, B == 0 || dA != nullptr, "dA is NULL");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    Ovl2cCall io{};
+    io.posA = posA, io.posB = posB, io.dS = dS, io.dA = dA, io.dB = dB, io.scale = plan->st.call.scale;
+    st = plan->st.dtype == AOK_F32 ? ovl2c_plan_dx<float>(ctx, plan->st, io) : ovl2c_plan_dx<double>(ctx, plan->st, io);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_ovl2c_plan_pullback_x");
 }
 
 aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan)

@@ -466,6 +466,56 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
     }
 }
 
+// Position derivative of one shell pair for the lane's b, accumulated in FP64:
+// acc += sum over real primitive pairs of G * sum_lin g[lin] * d/dA (s_x s_y s_z), with
+// d/dA_x [x_A^i e^{-alpha x_A^2}] = 2 alpha x_A^{i+1} e - i x_A^{i-1} e, i.e. the 1-D
+// tables raised by one on the bra: d_x(i, j) = 2 alpha s_x(i+1, j) - i s_x(i-1, j).
+template <typename T, int LA, int LB>
+__device__ __forceinline__ void pair_dx(const typename RecOf<T>::type *__restrict__ recs, int nra, int nrb, int K,
+                                        double R2, const T (&ab)[3], const T *g, int nrows, double (&acc)[3])
+{
+    constexpr int NBA = ShellC<LA>::nbf, N1 = 2 * LA + 1;
+    for (int i = 0; i < nra; ++i)
+        for (int j = 0; j < nrb; ++j) {
+            const typename RecOf<T>::type r = recs[i * K + j];
+            const T G = r.Kab * pair_exp(r, R2);
+            T sx[LA + 2][LB + 1], sy[LA + 2][LB + 1], sz[LA + 2][LB + 1];
+            os1d<T, LA + 1, LB>(-r.bop * ab[0], r.aop * ab[0], r.i2p, sx);
+            os1d<T, LA + 1, LB>(-r.bop * ab[1], r.aop * ab[1], r.i2p, sy);
+            os1d<T, LA + 1, LB>(-r.bop * ab[2], r.aop * ab[2], r.i2p, sz);
+            const T a2 = r.aop / r.i2p;  // 2 alpha = (alpha / p) / (1 / 2p)
+            T hx = T(0), hy = T(0), hz = T(0);
+            int index1 = 0;
+#pragma unroll
+            for (int ll1 = LA; ll1 >= 0; --ll1)
+#pragma unroll
+                for (int n1 = 0; n1 <= LA - ll1; ++n1) {
+                    const int m1 = LA - ll1 - n1;
+                    int index2 = 0;
+#pragma unroll
+                    for (int ll2 = LB; ll2 >= 0; --ll2)
+#pragma unroll
+                        for (int n2 = 0; n2 <= LB - ll2; ++n2) {
+                            const int m2 = LB - ll2 - n2;
+                            const int lin = index1 + N1 * index2;
+                            const T gv = g[lin % NBA + nrows * (lin / NBA)];
+                            const T x0 = sx[ll1][ll2], y0 = sy[m1][m2], z0 = sz[n1][n2];
+                            const T dxv = a2 * sx[ll1 + 1][ll2] - (ll1 > 0 ? T(ll1) * sx[ll1 > 0 ? ll1 - 1 : 0][ll2] : T(0));
+                            const T dyv = a2 * sy[m1 + 1][m2] - (m1 > 0 ? T(m1) * sy[m1 > 0 ? m1 - 1 : 0][m2] : T(0));
+                            const T dzv = a2 * sz[n1 + 1][n2] - (n1 > 0 ? T(n1) * sz[n1 > 0 ? n1 - 1 : 0][n2] : T(0));
+                            hx += gv * (dxv * y0 * z0);
+                            hy += gv * (x0 * dyv * z0);
+                            hz += gv * (x0 * y0 * dzv);
+                            ++index2;
+                        }
+                    ++index1;
+                }
+            acc[0] += static_cast<double>(G * hx);
+            acc[1] += static_cast<double>(G * hy);
+            acc[2] += static_cast<double>(G * hz);
+        }
+}
+
 // (la, lb) dispatch over the compile-time cases allowed by LMAXB
 #define AOK_LL_CASES(X)                                                                                       \
     X(0, 0) X(0, 1) X(0, 2) X(0, 3) X(0, 4) X(1, 0) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(2, 0) X(2, 1) X(2, 2) \
@@ -630,6 +680,106 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     __syncthreads();
     double *pt = partial + (static_cast<int64_t>(tile) * sh.nchunk + chunk) * pstride;
     for (int q = lane; q < pstride; q += OVL_TILE) pt[q] = accA[q];
+}
+
+// ---------------------------------------------------------------------------------
+// Position cotangents: grid (tile, chunk) as the pullback; each lane sums its pair's
+// A-derivative over the chunk's shell pairs and writes one FP64 partial per (chunk,
+// bucket position): partial[(chunk * B + pos) * 3 + c], pos = the lane's place in the
+// species-pair bucket order (contiguous per tile, so the stores coalesce).
+// ---------------------------------------------------------------------------------
+template <typename T, int LMAXB>
+__global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B, const T *__restrict__ dS,
+                                                    const T *__restrict__ posA, const T *__restrict__ posB, T scale,
+                                                    const int *__restrict__ perm, const int *__restrict__ tile_key,
+                                                    const int *__restrict__ boff, const int *__restrict__ bcnt,
+                                                    const int *__restrict__ toff,
+                                                    const typename RecOf<T>::type *__restrict__ rec,
+                                                    const int *__restrict__ nreal, double *__restrict__ partial)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    int *bl_b = reinterpret_cast<int *>(smem_raw);
+    T *stripe = reinterpret_cast<T *>(smem_raw + 256);
+    TileInfo ti;
+    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
+    if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
+    const int lane = threadIdx.x;
+    const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
+    const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
+    bl_b[lane] = b;
+    const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
+    const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
+    const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);
+    const double dy = static_cast<double>(Ay) - static_cast<double>(By);
+    const double dz = static_cast<double>(Az) - static_cast<double>(Bz);
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    const T ab[3] = {static_cast<T>(dx), static_cast<T>(dy), static_cast<T>(dz)};
+    T *mine = stripe + lane * sh.str;
+    const int64_t NN = static_cast<int64_t>(N) * N;
+    __syncthreads();
+    const OvlChunk ch = unpack_chunk(sh.ch[chunk]);
+    const int sb = ch.sb, lb = sh.ls[sb];
+    const int nbb = nbf_of(lb), nrows = ch.nrows;
+    const int len = nrows * nbb, total = OVL_TILE * len;
+    const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
+    const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
+    for (int e = lane; e < total; e += OVL_TILE) {
+        const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
+        const int q = e - bl * len;
+        const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
+        const int i = q - j * nrows;
+        stripe[bl * sh.str + q] =
+            bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i] : T(0);
+    }
+    __syncthreads();
+    const int nrb = nreal[sgB * ns + sb];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
+        const int la = sh.ls[sa];
+        const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
+        const typename RecOf<T>::type *recs = rec + g * K * K;
+        const int nra = nreal[sgA * ns + sa];
+        const T *gsl = mine + (sh.off[sa] - ch.row0);
+#define AOK_DX_CASE(A, B_)                                                       \
+    if constexpr (A <= LMAXB && B_ <= LMAXB)                                     \
+        if (la == A && lb == B_) {                                               \
+            pair_dx<T, A, B_>(recs, nra, nrb, K, R2, ab, gsl, nrows, acc);       \
+            continue;                                                            \
+        }
+        AOK_LL_CASES(AOK_DX_CASE)
+#undef AOK_DX_CASE
+    }
+    if (lane < ti.nact) {
+        double *pp = partial + (static_cast<int64_t>(chunk) * B + ti.first + lane) * 3;
+        pp[0] = acc[0];
+        pp[1] = acc[1];
+        pp[2] = acc[2];
+    }
+}
+
+// Sum of the chunk partials of each pair in chunk order (deterministic), chain rule to
+// input length units, dB = -dA.
+template <typename T>
+__global__ __launch_bounds__(256) void ovl2c_dx_reduce(int64_t B, int nchunk, const int *__restrict__ perm,
+                                                       const double *__restrict__ partial, double scale,
+                                                       T *__restrict__ dA, T *__restrict__ dB)
+{
+    const int64_t pos = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (pos >= B) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < nchunk; ++c) {
+        const double *pp = partial + (static_cast<int64_t>(c) * B + pos) * 3;
+        s[0] += pp[0];
+        s[1] += pp[1];
+        s[2] += pp[2];
+    }
+    const int64_t b = perm[pos];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dA[3 * b + k] = static_cast<T>(scale * s[k]);
+        if (dB) dB[3 * b + k] = static_cast<T>(-scale * s[k]);
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -1150,6 +1300,53 @@ static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh)
     return AOK_OK;
 }
 
+template <typename T, int LMAXB>
+static aok_status fast_dx_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws,
+                                 double *partial)
+{
+    using R = typename RecOf<T>::type;
+    const size_t shm = 256 + sizeof(T) * OVL_TILE * sh.str;
+    auto kern = &ovl2c_dx_fast<T, LMAXB>;
+    if (shm > 64 * 1024)
+        AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
+    time_begin(ctx, "ovl2c_dx_fast");
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh, c.B,
+                       static_cast<const T *>(c.dS), static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
+                       static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
+                       reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
+                       reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
+                       reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int *>(ws + w.nreal),
+                       partial);
+    time_end(ctx);
+    AOK_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL((ovl2c_dx_reduce<T>), dim3(static_cast<unsigned>((c.B + 255) / 256)), dim3(256), 0,
+                       ctx->stream, c.B, sh.nchunk, reinterpret_cast<const int *>(ws + w.perm), partial, c.scale,
+                       static_cast<T *>(c.dA), static_cast<T *>(c.dB));
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+// Position cotangents on a prepared workspace; `partial` holds nchunk * B * 3 doubles.
+template <typename T>
+static aok_status fast_dx(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws,
+                          double *partial)
+{
+    if (c.B == 0) return AOK_OK;
+    switch (lmax_of(c.ls, c.ns)) {
+        case 0: return fast_dx_launch<T, 0>(ctx, c, sh, w, ws, partial);
+        case 1: return fast_dx_launch<T, 1>(ctx, c, sh, w, ws, partial);
+        case 2: return fast_dx_launch<T, 2>(ctx, c, sh, w, ws, partial);
+        case 3: return fast_dx_launch<T, 3>(ctx, c, sh, w, ws, partial);
+        default: return fast_dx_launch<T, 4>(ctx, c, sh, w, ws, partial);
+    }
+}
+
+static size_t dx_partial_bytes(const OvlShape &sh, int64_t B)
+{
+    return align256(sizeof(double) * 3 * static_cast<size_t>(sh.nchunk) * static_cast<size_t>(B > 0 ? B : 1));
+}
+
 template <typename T>
 static aok_status fast_forward(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
@@ -1294,6 +1491,47 @@ aok_status compile_coef_pb_run(aok_ctx ctx, const void *dcoef, const void *zeta,
     return AOK_OK;
 }
 
+// Position cotangents, one-shot: the fast path's prep, then the partials after it in the
+// context workspace.
+template <typename T>
+aok_status ovl2c_dx_run(aok_ctx ctx, const Ovl2cCall &c)
+{
+    OvlShape sh;
+    aok_status st = shape_of(ctx, c, sh);
+    if (st != AOK_OK) return st;
+    if (c.flags & AOK_PLAIN) return fail(ctx, AOK_ERR_ARG, "position cotangents are computed by the fast path only");
+    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), false);
+    st = ensure_ws(ctx, w.total + dx_partial_bytes(sh, c.B));
+    if (st != AOK_OK) return st;
+    char *ws = static_cast<char *>(ctx->ws);
+    if (c.B == 0) return AOK_OK;
+    if ((st = fast_prep<T>(ctx, c, sh, w, ws)) != AOK_OK) return st;
+    return fast_dx<T>(ctx, c, sh, w, ws, reinterpret_cast<double *>(ws + w.total));
+}
+
+// Position cotangents through a plan: its prep, the partials in the context workspace.
+template <typename T>
+aok_status ovl2c_plan_dx(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall &io)
+{
+    if (p.call.flags & AOK_PLAIN)
+        return fail(ctx, AOK_ERR_ARG, "position cotangents are computed by the fast path only");
+    if (p.call.B == 0) return AOK_OK;
+    aok_status st = ensure_ws(ctx, dx_partial_bytes(p.sh, p.call.B));
+    if (st != AOK_OK) return st;
+    Ovl2cCall c = p.call;
+    c.posA = io.posA;
+    c.posB = io.posB;
+    c.scale = io.scale;
+    c.dS = io.dS;
+    c.dA = io.dA;
+    c.dB = io.dB;
+    return fast_dx<T>(ctx, c, p.sh, p.w, static_cast<char *>(p.buf), static_cast<double *>(ctx->ws));
+}
+
+template aok_status ovl2c_dx_run<float>(aok_ctx, const Ovl2cCall &);
+template aok_status ovl2c_dx_run<double>(aok_ctx, const Ovl2cCall &);
+template aok_status ovl2c_plan_dx<float>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &);
+template aok_status ovl2c_plan_dx<double>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &);
 template aok_status ovl2c_run<float>(aok_ctx, const Ovl2cCall &, bool);
 template aok_status ovl2c_run<double>(aok_ctx, const Ovl2cCall &, bool);
 template aok_status compile_coef_run<float>(aok_ctx, const void *, const void *, int64_t, int64_t, int64_t,

@@ -189,6 +189,7 @@ struct Ovl2cCall {
     const void *dS;
     void *o1, *o2;  // pullback outputs: (dcoef, dalpha) or (dzeta, dD)
     bool vjp;
+    void *dA, *dB;  // position cotangents [3*B] (dB may be NULL)
 };
 
 // One 3-centre call.
@@ -222,6 +223,10 @@ template <typename T>
 aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p);
 template <typename T>
 aok_status ovl2c_plan_exec(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall &io, bool pullback);
+template <typename T>
+aok_status ovl2c_dx_run(aok_ctx ctx, const Ovl2cCall &c);
+template <typename T>
+aok_status ovl2c_plan_dx(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall &io);
 template <typename T>
 aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c);
 template <typename T>

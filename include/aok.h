@@ -171,6 +171,20 @@ aok_status aok_ovl2c_pullback_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, c
                                  const void *zeta, const void *D, double lengthscale,
                                  void *dzeta, void *dD);
 
+/* No reference counterpart: the reference's rrule returns NoTangent for the positions
+ * (overlap_2c_grad.jl:309-318).  Position cotangents of the 2-centre overlap, for callers
+ * that differentiate through geometry (forces, position-dependent losses):
+ *   dA[3b+c] = sum_ij dS[i,j,b] dS_ij/dA_bc  (per unit of the input length unit),
+ *   dB = -dA (S depends on A - B only); dB may be NULL.
+ * The derivative is of the forward as computed (same alias map, zero coefficients
+ * skipped); per-pair sums in FP64, fixed order.  Fast path only (AOK_PLAIN is rejected). */
+aok_status aok_ovl2c_pullback_x(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dS,
+                                const void *posA, const void *posB, const void *sidxA,
+                                const void *sidxB, aok_itype itype, int64_t B,
+                                int64_t ns, int64_t K, int64_t NZ, const int64_t *ls,
+                                const void *coef, const void *alpha, double lengthscale,
+                                void *dA, void *dB);
+
 /* ---- H2: 2-centre plans (forward + pullback sharing one preparation) ----------------- */
 /* rrule(batch_overlap, orb, XA, XB, ps, st) (overlap_2c_grad.jl:309-318) runs the forward
  * and, later, pullback_ps on the same batch and parameters; the reference recompiles the
@@ -201,6 +215,9 @@ aok_status aok_ovl2c_plan_forward(aok_ovl2c_plan plan, const void *posA, const v
 /* Pullback of dS [N*N*B]: (d1, d2) [ns*K*NZ] as described above. */
 aok_status aok_ovl2c_plan_pullback(aok_ovl2c_plan plan, const void *dS, const void *posA,
                                    const void *posB, void *d1, void *d2);
+/* Position cotangents of dS through the plan: (dA, dB) [3*B] as aok_ovl2c_pullback_x. */
+aok_status aok_ovl2c_plan_pullback_x(aok_ovl2c_plan plan, const void *dS, const void *posA,
+                                     const void *posB, void *dA, void *dB);
 aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan);
 
 /* ---- H2: 3-centre overlap ------------------------------------------------------------ */

@@ -527,6 +527,84 @@ void orc_ovl2c_pb(const double *dS, const double *posA, const double *posB,
     free(nbf);
 }
 
+/* Position cotangents of the 2-centre overlap.  No reference counterpart (the
+ * reference's rrule returns NoTangent for the positions, overlap_2c_grad.jl:309-318).
+ * Differentiates the forward exactly as computed (same alias map, same skip of zero
+ * coefficients): d/dA_x [x_A^i e^{-a x_A^2}] = 2a x_A^{i+1} e - i x_A^{i-1} e, so each
+ * term's A-derivative is prefac (2a E^{i+1,j}_0 - i E^{i-1,j}_0) E^y E^z (and likewise
+ * for y, z).  dA[3b+c] = sum_ij dS[i,j,b] dS_ij/dA_c; dB = -dA (S depends on A - B). */
+void orc_ovl2c_dx(const double *dS, const double *posA, const double *posB,
+                  const int64_t *sidxA, const int64_t *sidxB, int64_t B, int64_t ns, int64_t K,
+                  int64_t NZ, const int64_t *ls, const double *coef, const double *alpha,
+                  double *dA, double *dB)
+{
+    int64_t *nbf = (int64_t *)malloc(sizeof(int64_t) * (size_t)(2 * ns + 1));
+    int64_t *off = nbf + ns;
+    const int64_t N = shell_layout(ns, ls, nbf, off);
+    (void)NZ;
+    double *Ex = (double *)malloc(sizeof(double) * 3 * ORC_T * (ORC_LE + 1) * (ORC_LE + 1));
+    double *Ey = Ex + ORC_T * (ORC_LE + 1) * (ORC_LE + 1), *Ez = Ey + ORC_T * (ORC_LE + 1) * (ORC_LE + 1);
+    for (int64_t b = 0; b < B; ++b) {
+        const int64_t sgA = sidxA ? sidxA[b] - 1 : 0, sgB = sidxB ? sidxB[b] - 1 : 0;
+        const double *A = posA + 3 * b, *Bv = posB + 3 * b;
+        const double *g_b = dS + N * N * b;
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int64_t sb = 0; sb < ns; ++sb)
+            for (int64_t sa = 0; sa < ns; ++sa) {
+                const int la = (int)ls[sa], lb = (int)ls[sb];
+                const int64_t nba = nbf[sa];
+                const int N1 = 2 * la + 1, LA = la + 1, LB = lb;
+                for (int64_t ip = 0; ip < K; ++ip)
+                    for (int64_t jp = 0; jp < K; ++jp) {
+                        const int64_t qa = sa + ns * (ip + K * sgA), qb = sb + ns * (jp + K * sgB);
+                        const double ca = coef[qa], cb = coef[qb];
+                        if (ca == 0.0 || cb == 0.0) continue;
+                        const double aa = alpha[qa], ab = alpha[qb];
+                        const double p = aa + ab, mu = aa * ab / p, oo2p = 0.5 / p;
+                        const double prefac = ca * cb * pow(ORC_PI / p, 1.5);
+                        double *Es[3] = {Ex, Ey, Ez};
+                        for (int ax = 0; ax < 3; ++ax) {
+                            const double P = (aa * A[ax] + ab * Bv[ax]) / p;
+                            const double AB = A[ax] - Bv[ax];
+                            md_etab(Es[ax], LA, LB, P - A[ax], P - Bv[ax], oo2p, exp(-mu * AB * AB));
+                        }
+#define ORC_DE(E, i, j) (2.0 * aa * e0((E), LA, LB, (i) + 1, (j)) - ((i) > 0 ? (i) * e0((E), LA, LB, (i) - 1, (j)) : 0.0))
+                        double hx = 0.0, hy = 0.0, hz = 0.0;
+                        int index1 = 0;
+                        for (int ll1 = la; ll1 >= 0; --ll1)
+                            for (int n1 = 0; n1 <= la - ll1; ++n1) {
+                                const int m1 = la - ll1 - n1;
+                                int index2 = 0;
+                                for (int ll2 = lb; ll2 >= 0; --ll2)
+                                    for (int n2 = 0; n2 <= lb - ll2; ++n2) {
+                                        const int m2 = lb - ll2 - n2;
+                                        const int lin = index1 + N1 * index2;
+                                        const int64_t i = lin % nba, j = lin / nba;
+                                        const double g = g_b[(off[sa] + i) + N * (off[sb] + j)];
+                                        const double ex = e0(Ex, LA, LB, ll1, ll2), ey = e0(Ey, LA, LB, m1, m2),
+                                                     ez = e0(Ez, LA, LB, n1, n2);
+                                        hx += g * ORC_DE(Ex, ll1, ll2) * ey * ez;
+                                        hy += g * ex * ORC_DE(Ey, m1, m2) * ez;
+                                        hz += g * ex * ey * ORC_DE(Ez, n1, n2);
+                                        ++index2;
+                                    }
+                                ++index1;
+                            }
+#undef ORC_DE
+                        acc[0] += prefac * hx;
+                        acc[1] += prefac * hy;
+                        acc[2] += prefac * hz;
+                    }
+            }
+        for (int c = 0; c < 3; ++c) {
+            dA[3 * b + c] = acc[c];
+            if (dB) dB[3 * b + c] = -acc[c];
+        }
+    }
+    free(Ex);
+    free(nbf);
+}
+
 /* ------------------------------------------------------------------------- */
 /* 3-centre overlap: E3 recursion about Q (reference/ecoeffs.jl:70-107, as in  */
 /* kernels_3c.jl:54-131), contraction with the N1, N1*N2 strides and the       */

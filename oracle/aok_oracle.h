@@ -75,6 +75,13 @@ void orc_ovl2c_pb(const double *dS, const double *posA, const double *posB,
                   int64_t K, int64_t NZ, const int64_t *ls, const double *coef,
                   const double *alpha, double *dcoef, double *dalpha);
 
+/* Position cotangents of the 2-centre overlap (no reference counterpart): dA, dB
+ * [3*B] in Bohr, dB = -dA; dB may be NULL. */
+void orc_ovl2c_dx(const double *dS, const double *posA, const double *posB,
+                  const int64_t *sidxA, const int64_t *sidxB, int64_t B, int64_t ns,
+                  int64_t K, int64_t NZ, const int64_t *ls, const double *coef,
+                  const double *alpha, double *dA, double *dB);
+
 /* Species-aware 3-centre overlap following batch_V_kernel! (kernels_3c.jl:12-162)
  * with the per-species primitive slices of the orbital path; sidx* may be NULL
  * (species 1).  Returns N. */

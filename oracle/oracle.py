@@ -55,6 +55,8 @@ def lib():
             f.restype = ctypes.c_int64
         L.orc_ovl2c_pb.argtypes = [_D, _D, _D, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                    ctypes.c_int64, _I, _D, _D, _D, _D]
+        L.orc_ovl2c_dx.argtypes = [_D, _D, _D, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_int64, _I, _D, _D, _D, _D]
         L.orc_ovl3c.argtypes = [_D, _D, _D, _I, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                 ctypes.c_int64, _I, _D, _D, _D]
         L.orc_ovl3c.restype = ctypes.c_int64
@@ -212,6 +214,26 @@ def ovl2c_pb(dS, posA, posB, sidxA, sidxB, ls, coef, alpha):
                        da.ctypes.data_as(_D))
     shp = (ns, K, NZ)
     return dc.reshape(shp, order="F"), da.reshape(shp, order="F")
+
+
+def ovl2c_dx(dS, posA, posB, sidxA, sidxB, ls, coef, alpha):
+    """Position cotangents (dA, dB), each (B, 3), of the 2-centre overlap (Bohr)."""
+    posA = np.asarray(posA, np.float64)
+    B = posA.shape[0]
+    ns, K, NZ = coef.shape
+    ga, gp = _d(_fortran_flat(dS))
+    Aa, Ap = _d(posA.reshape(-1))
+    Ba, Bp = _d(np.asarray(posB, np.float64).reshape(-1))
+    sa, sap = _i(sidxA)
+    sb, sbp = _i(sidxB)
+    la, lp = _i(ls)
+    ca, cp = _d(_fortran_flat(coef))
+    aa, ap = _d(_fortran_flat(alpha))
+    dA = np.zeros(3 * B)
+    dB = np.zeros(3 * B)
+    lib().orc_ovl2c_dx(gp, Ap, Bp, sap, sbp, B, ns, K, NZ, lp, cp, ap, dA.ctypes.data_as(_D),
+                       dB.ctypes.data_as(_D))
+    return dA.reshape(B, 3), dB.reshape(B, 3)
 
 
 def ovl3c(posA, posB, posC, sidxA, sidxB, sidxC, ls, coef, alpha):

@@ -154,6 +154,47 @@ def test_overlap_param_pullback_vs_fd():
     assert abs(np.sum(dD * W) - fd_D) <= 1e-6 * max(1.0, abs(fd_D))
 
 
+def test_position_cotangent_vs_fd():
+    """orc_ovl2c_dx: central differences of <dS, S> in every coordinate of A and of B,
+    shells up to l = 4 (the alias map included), two species."""
+    rng = np.random.default_rng(31)
+    ls = [0, 1, 2, 3, 4, 2]
+    K, NZ, B = 3, 2, 4
+    zeta = np.exp(rng.uniform(-1, 1.5, size=(len(ls), K, NZ)))
+    coef = rng.standard_normal((len(ls), K, NZ))
+    coef[2, 2, 0] = 0.0  # a zero coefficient (skipped in the forward)
+    posA = rng.normal(0, 0.5, size=(B, 3))
+    posB = posA + rng.normal(0, 0.9, size=(B, 3))
+    sA, sB = rng.integers(1, NZ + 1, B), rng.integers(1, NZ + 1, B)
+    S = O.ovl2c(posA, posB, sA, sB, ls, coef, zeta)
+    dS = rng.standard_normal(S.shape)
+    dA, dB = O.ovl2c_dx(dS, posA, posB, sA, sB, ls, coef, zeta)
+    h = 1e-6
+    for b in range(B):
+        for c in range(3):
+            for which, d in ((0, dA), (1, dB)):
+                P = [posA.copy(), posB.copy()]
+                P[which][b, c] += h
+                fp = np.sum(dS * O.ovl2c(P[0], P[1], sA, sB, ls, coef, zeta))
+                P[which][b, c] -= 2 * h
+                fm = np.sum(dS * O.ovl2c(P[0], P[1], sA, sB, ls, coef, zeta))
+                fd = (fp - fm) / (2 * h)
+                assert abs(d[b, c] - fd) <= 1e-7 * max(1.0, np.abs(d).max()), (b, c, which, d[b, c], fd)
+    assert np.array_equal(dB, -dA)
+
+
+def test_position_cotangent_ss_closed_form():
+    """s-s primitives: S = c_a c_b (pi/p)^(3/2) exp(-mu R^2), dS/dA = -2 mu (A - B) S."""
+    ls, coef, zeta = _two_shell(0, 0, [1.3], [0.7], [0.45], [1.1])
+    A, B = np.array([[0.2, -0.4, 0.9]]), np.array([[-0.5, 0.3, 0.1]])
+    S = O.ovl2c(A, B, None, None, ls, coef, zeta)
+    g = np.zeros_like(S)
+    g[0, 1, 0] = 1.0  # the (a, b) entry only
+    dA, _ = O.ovl2c_dx(g, A, B, None, None, ls, coef, zeta)
+    mu = 1.3 * 0.45 / 1.75
+    np.testing.assert_allclose(dA[0], -2 * mu * (A[0] - B[0]) * S[0, 1, 0], rtol=1e-13)
+
+
 def test_normalisation_vjp_vs_fd():
     rng = np.random.default_rng(2)
     ls = [0, 1, 2]
