@@ -409,12 +409,19 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
 // Pullback of one shell pair: per-pair contributions reduced over the wave (FP64) and
 // added to the wave's FP64 LDS accumulators of the bra side (accA, list index i) and
 // the ket side (accB, list index j); [2*li + 0] coefficient, [2*li + 1] exponent.
+// Bra sums run over j in registers and are wave-reduced once per i; ket sums run over i
+// in the lane's own LDS slots kac[(2*j + c) * 64 + lane] and are wave-reduced once per j
+// after the i loop (both: T-precision sums of at most K terms per lane, FP64 across lanes).
 template <typename T, int LA, int LB>
 __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restrict__ recs, int nra, int nla, int nrb,
                                          int nlb, int K, double R2, const T (&ab)[3], const T *g, int nrows,
-                                         double *accA, double *accB, int lane)
+                                         double *accA, double *accB, T *kac, int lane)
 {
     using R = typename RecOf<T>::type;
+    for (int j = 0; j < nrb; ++j) {
+        kac[2 * j * OVL_TILE + lane] = T(0);
+        kac[(2 * j + 1) * OVL_TILE + lane] = T(0);
+    }
     // real x real: all four cotangents
     for (int i = 0; i < nra; ++i) {
         T cA = T(0), aA = T(0);
@@ -426,18 +433,22 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             const T eH = e * H;
             cA += r.w1 * eH;
             aA -= r.Kab * e * Ha;
-            const double cB = wave_sum(static_cast<double>(r.w2 * eH));
-            const double aB = wave_sum(static_cast<double>(-r.Kab * e * Hb));
-            if (lane == 0) {
-                accB[2 * j] += cB;
-                accB[2 * j + 1] += aB;
-            }
+            kac[2 * j * OVL_TILE + lane] += r.w2 * eH;
+            kac[(2 * j + 1) * OVL_TILE + lane] -= r.Kab * e * Hb;
         }
         const double sc = wave_sum(static_cast<double>(cA));
         const double sa = wave_sum(static_cast<double>(aA));
         if (lane == 0) {
             accA[2 * i] += sc;
             accA[2 * i + 1] += sa;
+        }
+    }
+    for (int j = 0; j < nrb; ++j) {
+        const double cB = wave_sum(static_cast<double>(kac[2 * j * OVL_TILE + lane]));
+        const double aB = wave_sum(static_cast<double>(kac[(2 * j + 1) * OVL_TILE + lane]));
+        if (lane == 0) {
+            accB[2 * j] += cB;
+            accB[2 * j + 1] += aB;
         }
     }
     // padded bra representative x real ket: coefficient cotangent of the bra slot only
@@ -625,6 +636,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     int *bl_b = reinterpret_cast<int *>(smem_raw);
     double *accA = reinterpret_cast<double *>(smem_raw + 256);  // [sa - sa0][K][2], then the ket side at ns*K*2
     T *stripe = reinterpret_cast<T *>(accA + pstride);
+    T *kac = stripe + OVL_TILE * sh.str;  // per-lane ket sums [2 * K][64]
     TileInfo ti;
     const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
@@ -671,7 +683,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
 #define AOK_BWD_CASE(A, B_)                                                                      \
     if constexpr (A <= LMAXB && B_ <= LMAXB)                                                     \
         if (la == A && lb == B_) {                                                               \
-            pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gsl, nrows, aA, aB, lane);   \
+            pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gsl, nrows, aA, aB, kac, lane); \
             continue;                                                                            \
         }
         AOK_LL_CASES(AOK_BWD_CASE)
@@ -1190,7 +1202,8 @@ template <typename T, int LMAXB>
 static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
     using R = typename RecOf<T>::type;
-    const size_t shm = 256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 + sizeof(T) * OVL_TILE * sh.str;
+    const size_t shm =
+        256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 + sizeof(T) * OVL_TILE * (sh.str + 2 * sh.K);
     auto kern = &ovl2c_bwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
