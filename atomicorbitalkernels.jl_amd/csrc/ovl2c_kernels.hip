@@ -306,6 +306,16 @@ __device__ __forceinline__ void os1d(T PA, T PB, T i2p, T (&s)[IA + 1][IB + 1])
         }
 }
 
+// Unroll factors of the primitive-pair loops (launch-shape tuning, fixed at build time).
+#ifndef AOK_FWD_UNROLL
+#define AOK_FWD_UNROLL 1
+#endif
+#ifndef AOK_PB_UNROLL
+#define AOK_PB_UNROLL 1
+#endif
+#define AOK_PRAGMA_(x) _Pragma(#x)
+#define AOK_PRAGMA(x) AOK_PRAGMA_(x)
+
 template <int L>
 struct ShellC {
     static constexpr int nbf = (L + 1) * (L + 2) / 2;
@@ -324,6 +334,7 @@ __device__ __forceinline__ void pair_fwd(const typename RecOf<T>::type *__restri
     for (int q = 0; q < NLIN; ++q) acc[q] = T(0);
     for (int i = 0; i < nra; ++i) {
         const typename RecOf<T>::type *row = recs + i * K;
+        AOK_PRAGMA(unroll AOK_FWD_UNROLL)
         for (int j = 0; j < nrb; ++j) {
             const typename RecOf<T>::type r = row[j];
             const T G = r.Kab * pair_exp(r, R2);
@@ -425,6 +436,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
     // real x real: all four cotangents
     for (int i = 0; i < nra; ++i) {
         T cA = T(0), aA = T(0);
+        AOK_PRAGMA(unroll AOK_PB_UNROLL)
         for (int j = 0; j < nrb; ++j) {
             const R r = recs[i * K + j];
             const T e = pair_exp(r, R2);
@@ -1143,7 +1155,7 @@ static int lmax_of(const int64_t *ls, int64_t ns)
     return m;
 }
 
-constexpr int OVL_STRIPE_CAP = 32;
+constexpr int OVL_STRIPE_CAP = 40;
 
 template <typename T>
 static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
