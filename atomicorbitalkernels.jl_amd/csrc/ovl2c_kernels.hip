@@ -377,9 +377,23 @@ __device__ __forceinline__ void pair_fwd(const typename RecOf<T>::type *__restri
 // Contraction of one primitive pair against the aliased cotangent block g (LDS stripe
 // slice of the lane, column stride nrows).  H = sum g s_x s_y s_z; with RAISED also the
 // bra- and ket-raised sums Ha, Hb (d/dalpha x^l e^{-alpha x^2} = -x^{l+2} e^{-alpha x^2}).
-template <typename T, int LA, int LB, bool RAISED>
-__device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, const T (&ab)[3], const T *g,
-                                              int nrows, T &H, T &Ha, T &Hb)
+// Cotangent accessors for pair_contract: the lane's slice of the LDS stripe (column stride
+// nrows), or the shell pair's block held in registers (column stride NBA).
+template <typename T>
+struct GStripe {
+    const T *g;
+    int nrows;
+    __device__ __forceinline__ T operator()(int i, int j) const { return g[i + nrows * j]; }
+};
+template <typename T, int NBA, int NBB>
+struct GRegs {
+    T v[NBA * NBB];
+    __device__ __forceinline__ T operator()(int i, int j) const { return v[i + NBA * j]; }
+};
+
+template <typename T, int LA, int LB, bool RAISED, typename G>
+__device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, const T (&ab)[3], const G &g, T &H,
+                                              T &Ha, T &Hb)
 {
     constexpr int NBA = ShellC<LA>::nbf, N1 = 2 * LA + 1;
     constexpr int RA = RAISED ? LA + 2 : LA, RB = RAISED ? LB + 2 : LB;
@@ -403,8 +417,7 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
                 for (int n2 = 0; n2 <= LB - ll2; ++n2) {
                     const int m2 = LB - ll2 - n2;
                     const int lin = index1 + N1 * index2;
-                    const int i = lin % NBA, j = lin / NBA;
-                    const T gv = g[i + nrows * j];
+                    const T gv = g(lin % NBA, lin / NBA);
                     const T x0 = sx[ll1][ll2], y0 = sy[m1][m2], z0 = sz[n1][n2];
                     H += gv * (x0 * y0 * z0);
                     if constexpr (RAISED) {
@@ -423,10 +436,10 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
 // Bra sums run over j in registers and are wave-reduced once per i; ket sums run over i
 // in the lane's own LDS slots kac[(2*j + c) * 64 + lane] and are wave-reduced once per j
 // after the i loop (both: T-precision sums of at most K terms per lane, FP64 across lanes).
-template <typename T, int LA, int LB>
+template <typename T, int LA, int LB, typename G>
 __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restrict__ recs, int nra, int nla, int nrb,
-                                         int nlb, int K, double R2, const T (&ab)[3], const T *g, int nrows,
-                                         double *accA, double *accB, T *kac, int lane)
+                                         int nlb, int K, double R2, const T (&ab)[3], const G &g, double *accA,
+                                         double *accB, T *kac, int lane)
 {
     using R = typename RecOf<T>::type;
     for (int j = 0; j < nrb; ++j) {
@@ -441,7 +454,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             const R r = recs[i * K + j];
             const T e = pair_exp(r, R2);
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, true>(r, ab, g, nrows, H, Ha, Hb);
+            pair_contract<T, LA, LB, true>(r, ab, g, H, Ha, Hb);
             const T eH = e * H;
             cA += r.w1 * eH;
             aA -= r.Kab * e * Ha;
@@ -469,7 +482,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
         for (int j = 0; j < nrb; ++j) {
             const R r = recs[i * K + j];
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, false>(r, ab, g, nrows, H, Ha, Hb);
+            pair_contract<T, LA, LB, false>(r, ab, g, H, Ha, Hb);
             cA += r.w1 * pair_exp(r, R2) * H;
         }
         const double sc = wave_sum(static_cast<double>(cA));
@@ -481,7 +494,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
         for (int i = 0; i < nra; ++i) {
             const R r = recs[i * K + j];
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, false>(r, ab, g, nrows, H, Ha, Hb);
+            pair_contract<T, LA, LB, false>(r, ab, g, H, Ha, Hb);
             cB += r.w2 * pair_exp(r, R2) * H;
         }
         const double sc = wave_sum(static_cast<double>(cB));
@@ -538,6 +551,13 @@ __device__ __forceinline__ void pair_dx(const typename RecOf<T>::type *__restric
             acc[2] += static_cast<double>(G * hz);
         }
 }
+
+template <typename T, int LMAXB>
+__host__ __device__ constexpr bool bwd_direct()
+{
+    return LMAXB <= 2 && std::is_same<T, float>::value;
+}
+
 
 // (la, lb) dispatch over the compile-time cases allowed by LMAXB
 #define AOK_LL_CASES(X)                                                                                       \
@@ -646,9 +666,13 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     const int pstride = (ns + 1) * K * 2;
     int *bl_b = reinterpret_cast<int *>(smem_raw);
+    // FP32 with bases up to d shells: each shell pair's cotangent block is read straight
+    // into registers (no LDS stripe, so more waves fit per CU); larger blocks, and FP64
+    // (twice the registers), stage it through the stripe.
+    constexpr bool DIRECT = bwd_direct<T, LMAXB>();
     double *accA = reinterpret_cast<double *>(smem_raw + 256);  // [sa - sa0][K][2], then the ket side at ns*K*2
     T *stripe = reinterpret_cast<T *>(accA + pstride);
-    T *kac = stripe + OVL_TILE * sh.str;  // per-lane ket sums [2 * K][64]
+    T *kac = DIRECT ? stripe : stripe + OVL_TILE * sh.str;  // per-lane ket sums [2 * K][64]
     TileInfo ti;
     const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
@@ -674,15 +698,20 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
     const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
     // stage the cotangent stripe (zeros for inactive lanes)
-    for (int e = lane; e < total; e += OVL_TILE) {
-        const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
-        const int q = e - bl * len;
-        const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
-        const int i = q - j * nrows;
-        stripe[bl * sh.str + q] =
-            bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i] : T(0);
+    if constexpr (!DIRECT) {
+        for (int e = lane; e < total; e += OVL_TILE) {
+            const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
+            const int q = e - bl * len;
+            const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
+            const int i = q - j * nrows;
+            stripe[bl * sh.str + q] =
+                bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i]
+                             : T(0);
+        }
     }
     __syncthreads();
+    const bool active = lane < ti.nact;
+    const T *gblk = dS + static_cast<int64_t>(b) * NN + colbase - ch.row0;  // + off[sa] + i + N * j
     const int nrb = nreal[sgB * ns + sb], nlb = nlist[sgB * ns + sb];
     double *aB = accA + ns * K * 2;
     for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
@@ -692,11 +721,21 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
         const int nra = nreal[sgA * ns + sa], nla = nlist[sgA * ns + sa];
         const T *gsl = mine + (sh.off[sa] - ch.row0);
         double *aA = accA + 2 * (sa - ch.sa0) * K;
-#define AOK_BWD_CASE(A, B_)                                                                      \
-    if constexpr (A <= LMAXB && B_ <= LMAXB)                                                     \
-        if (la == A && lb == B_) {                                                               \
-            pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gsl, nrows, aA, aB, kac, lane); \
-            continue;                                                                            \
+#define AOK_BWD_CASE(A, B_)                                                                        \
+    if constexpr (A <= LMAXB && B_ <= LMAXB)                                                       \
+        if (la == A && lb == B_) {                                                                 \
+            if constexpr (DIRECT) {                                                                \
+                GRegs<T, ShellC<A>::nbf, ShellC<B_>::nbf> gr;                                      \
+                const T *src = gblk + sh.off[sa];                                                  \
+                _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
+                    _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
+                        gr.v[i + ShellC<A>::nbf * j] = active ? src[i + static_cast<int64_t>(N) * j] : T(0); \
+                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, kac, lane);    \
+            } else {                                                                               \
+                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, GStripe<T>{gsl, nrows}, aA, aB, \
+                                   kac, lane);                                                     \
+            }                                                                                      \
+            continue;                                                                              \
         }
         AOK_LL_CASES(AOK_BWD_CASE)
 #undef AOK_BWD_CASE
@@ -1214,8 +1253,8 @@ template <typename T, int LMAXB>
 static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
     using R = typename RecOf<T>::type;
-    const size_t shm =
-        256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 + sizeof(T) * OVL_TILE * (sh.str + 2 * sh.K);
+    const size_t shm = 256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 +
+                       sizeof(T) * OVL_TILE * ((bwd_direct<T, LMAXB>() ? 0 : sh.str) + 2 * sh.K);
     auto kern = &ovl2c_bwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
