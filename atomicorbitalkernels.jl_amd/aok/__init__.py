@@ -17,7 +17,8 @@ from .compiled import UNITFUL_HINT, CompiledBasis, Length, adapt_basis, compile_
 from .orbitals import (NO_TANGENT, NoTangent, PStates, evaluate, evaluate_ed, pullback_ps, pullback_x,
                        rrule_evaluate)
 from .overlap import (Ovl2cPlan, batch_overlap, batch_overlap_, batch_overlap_3c, batch_overlap_3c_,
-                      overlap_pb_coef_alpha, overlap_pullback_x, rrule_batch_overlap)
+                      overlap_3c_pb_coef_alpha, overlap_pb_coef_alpha, overlap_pullback_x, rrule_batch_overlap,
+                      rrule_batch_overlap_3c)
 from .overlap import pullback_ps as overlap_pullback_ps
 
 __all__ = [
@@ -27,6 +28,6 @@ __all__ = [
     "UNITFUL_HINT", "adapt_basis", "as_fortran", "atomic_number", "basis_set", "batch_overlap", "batch_overlap_",
     "batch_overlap_3c", "batch_overlap_3c_", "compile_basis", "compile_coef", "compile_coef_pb", "compile_shells",
     "context", "evaluate", "evaluate_ed", "fortran_from_numpy", "fortran_zeros", "gaussian_orbitals",
-    "length_factor", "normalise_shell", "overlap_pb_coef_alpha", "overlap_pullback_ps", "overlap_pullback_x",
-    "pullback_ps", "pullback_x", "read_gbs", "rrule_batch_overlap", "rrule_evaluate", "slater_orbitals", "to_bohr",
+    "length_factor", "normalise_shell", "overlap_3c_pb_coef_alpha", "overlap_pb_coef_alpha", "overlap_pullback_ps", "overlap_pullback_x",
+    "pullback_ps", "pullback_x", "read_gbs", "rrule_batch_overlap", "rrule_batch_overlap_3c", "rrule_evaluate", "slater_orbitals", "to_bohr",
 ]

@@ -112,6 +112,8 @@ _SIGS = {
     "aok_ovl2c_plan_forward": [_vp, _vp, _vp, _vp],
     "aok_ovl2c_plan_pullback": [_vp, _vp, _vp, _vp, _vp, _vp],
     "aok_ovl2c_plan_destroy": [_vp],
+    "aok_ovl3c_pullback": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p,
+                           _vp, _vp, _dbl, _vp, _vp],
     "aok_ovl3c": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp,
                   _dbl, _vp],
 }

@@ -16,8 +16,8 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from .basis import (AtomicOrbitals, CartesianGTOBasis, _dtype_code, as_fortran, compile_basis, compile_struct,
-                    fortran_empty)
+from .basis import (AtomicOrbitals, CartesianGTOBasis, _dtype_code, as_fortran, compile_basis, compile_coef,
+                    compile_coef_pb, compile_struct, fortran_empty)
 from . import compiled as _cb
 from .compiled import CompiledBasis
 from .orbitals import NO_TANGENT, PStates
@@ -345,6 +345,65 @@ def batch_overlap_3c_(out, basis, XA, XB, XC, *, mode="fast", ctx=None):
              B, ns, K, NZ, _lib.i64_array(basis.ls), coef.data_ptr(), zeta.data_ptr(), float(basis.lengthscale),
              out.data_ptr())
     return out
+
+
+def overlap_3c_pb_coef_alpha(dT, basis: CartesianGTOBasis, XA, XB, XC, *, ctx=None):
+    """(dcoef, dalpha), each (ns, K, NZ), of batch_overlap_3c(basis, XA, XB, XC) for the
+    cotangent dT (N, N, N, B) (aok_ovl3c_pullback; no reference counterpart).  Every
+    primitive slot receives its coefficient cotangent, padded slots included, as in the
+    2-centre pullback; shells up to l = 2."""
+    pA, sA = _unpack(XA, basis.zlist, "XA")
+    pB, sB = _unpack(XB, basis.zlist, "XB")
+    pC, sC = _unpack(XC, basis.zlist, "XC")
+    B = pA.shape[0]
+    if pB.shape[0] != B or pC.shape[0] != B:
+        raise _lib.AOKArgumentError("XA, XB and XC must have equal length")
+    N = basis.nbf_total
+    dt = torch.promote_types(dT.dtype, basis.coef.dtype)
+    dT = as_fortran(dT.to(dt))
+    if tuple(dT.shape) != (N, N, N, B):
+        raise _lib.AOKArgumentError(f"dT must be {(N, N, N, B)}")
+    pA, pB, pC = pA.to(dt), pB.to(dt), pC.to(dt)
+    coef = as_fortran(basis.coef.to(dt))
+    zeta = as_fortran(basis.zeta.to(dt))
+    ns, K, NZ = zeta.shape
+    _lib.require_device(dT, pA, pB, pC, sA, sB, sC, coef, zeta)
+    dc = fortran_empty(zeta.shape, dt, dT.device)
+    da = fortran_empty(zeta.shape, dt, dT.device)
+    ctx = ctx or _lib.context(dT.device.index or 0)
+    a, it = _sidx_ptr(sA)
+    b, _ = _sidx_ptr(sB)
+    c, _ = _sidx_ptr(sC)
+    ctx.call("aok_ovl3c_pullback", _dtype_code(dt), _lib.AOK_FAST, dT.data_ptr(), pA.data_ptr(), pB.data_ptr(),
+             pC.data_ptr(), a, b, c, it, B, ns, K, NZ, _lib.i64_array(basis.ls), coef.data_ptr(), zeta.data_ptr(),
+             float(basis.lengthscale), dc.data_ptr(), da.data_ptr())
+    return dc, da
+
+
+def rrule_batch_overlap_3c(orb: AtomicOrbitals, XA, XB, XC, ps, st=None, *, dtype=None, ctx=None):
+    """3-centre overlap of an orbital basis with (zeta, D) from ps, and its pullback
+    (no reference counterpart) -> (T, pullback); pullback(dT) -> (NoTangent, NoTangent,
+    NoTangent, NoTangent, NoTangent, dps, NoTangent) with dps = (Rnl = (zeta, D), Ylm = ()):
+    dzeta = dalpha + dzeta_norm, dD through the FP64 normalisation VJP."""
+    if orb.Rnl.decay != _lib.AOK_DECAY_GAUSSIAN:
+        raise _lib.AOKArgumentError("the overlap kernels integrate plain Gaussians")
+    ls, nbf, off = compile_struct(orb)
+    zeta, D = ps["Rnl"]["zeta"], ps["Rnl"]["D"]
+    pA = XA.positions if isinstance(XA, PStates) else XA[0]
+    dt = dtype or torch.promote_types(pA.dtype, torch.promote_types(zeta.dtype, D.dtype))
+    zeta, D = as_fortran(zeta.to(dt)), as_fortran(D.to(dt))
+    coef = compile_coef(zeta, D, ls, ctx=ctx)
+    basis = CartesianGTOBasis(max(ls), len(ls), zeta.shape[1], list(ls), nbf, off, off[-1], zeta, coef,
+                              orb.Rnl.zlist, orb.lengthscale)
+    T = batch_overlap_3c(basis, XA, XB, XC, dtype=dt, ctx=ctx)
+
+    def pullback(dT):
+        dc, da = overlap_3c_pb_coef_alpha(dT, basis, XA, XB, XC, ctx=ctx)
+        dD, dzn = compile_coef_pb(dc, zeta, D, ls, ctx=ctx)
+        dps = {"Rnl": {"zeta": da + dzn, "D": dD}, "Ylm": {}}
+        return NO_TANGENT, NO_TANGENT, NO_TANGENT, NO_TANGENT, NO_TANGENT, dps, NO_TANGENT
+
+    return T, pullback
 
 
 def batch_overlap_3c(basis, XA, XB, XC, *, mode="fast", dtype=None, ctx=None):

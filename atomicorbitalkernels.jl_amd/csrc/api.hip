@@ -640,4 +640,28 @@ aok_status aok_ovl3c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *p
     return finish(ctx, "aok_ovl3c");
 }
 
+aok_status aok_ovl3c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dT, const void *posA,
+                              const void *posB, const void *posC, const void *sidxA, const void *sidxB,
+                              const void *sidxC, aok_itype itype, int64_t B, int64_t ns, int64_t K, int64_t NZ,
+                              const int64_t *ls, const void *coef, const void *alpha, double lengthscale,
+                              void *dcoef, void *dalpha)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
+    AOK_REQUIRE(ctx, !bad_itype(itype), "itype must be AOK_I64 or AOK_I32");
+    AOK_REQUIRE(ctx, (flags & ~AOK_PLAIN) == 0u, "unknown flags");
+    AOK_REQUIRE(ctx, ls && coef && alpha, "NULL basis argument");
+    AOK_REQUIRE(ctx, dcoef && dalpha, "dcoef/dalpha are NULL");
+    AOK_REQUIRE(ctx, B >= 0, "B must be >= 0");
+    AOK_REQUIRE(ctx, B == 0 || (posA && posB && posC && dT), "positions/dT are NULL");
+    Ovl3cCall c{flags, posA, posB, posC, sidxA, sidxB, sidxC, itype, B, ls, ns, K, NZ, coef, alpha, lengthscale,
+                nullptr, dT, dcoef, dalpha};
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    st = dtype == AOK_F32 ? ovl3c_pullback_run<float>(ctx, c) : ovl3c_pullback_run<double>(ctx, c);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_ovl3c_pullback");
+}
+
 }  // extern "C"

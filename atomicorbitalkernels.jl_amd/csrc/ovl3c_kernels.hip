@@ -517,6 +517,243 @@ __global__ __launch_bounds__(128) void ovl3c_fwd_plain(const OvlShape sh, int64_
 }
 
 // ---------------------------------------------------------------------------------
+// Pullback (coef, alpha) of the 3-centre overlap.  No reference counterpart; the same
+// rules as the 2-centre pullback: every primitive slot receives its coefficient
+// cotangent (a padded slot's is the derivative at coef = 0), exponent cotangents by the
+// raised-momentum identity d/dalpha x^l e^{-alpha x^2} = -x^{l+2} e^{-alpha x^2}.
+// Per (species, shell) a list holds the real slots, then one representative per
+// distinct padded exponent (pcls maps every slot to its list entry).  One thread per
+// batch element walks every shell triple and every list triple with at most one padded
+// entry (two padded entries contribute nothing), accumulating into its own FP64 row
+// [side][shell][list entry][coef, exponent]; ovl3_bwd_reduce then sums the rows of each
+// slot over the batch in a fixed order.  Deterministic, no atomics.
+// ---------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void widen_pair(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
+                                                  double *__restrict__ x64, double *__restrict__ y64)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    x64[i] = static_cast<double>(x[i]);
+    y64[i] = static_cast<double>(y[i]);
+}
+
+__global__ __launch_bounds__(64) void ovl3_pb_lists(const OvlShape sh, const double *__restrict__ coef64,
+                                                    const double *__restrict__ alpha64, int *__restrict__ nreal,
+                                                    int *__restrict__ nlist, int *__restrict__ lidx,
+                                                    int *__restrict__ pcls)
+{
+    const int ns = sh.ns, K = sh.K;
+    const int idx = static_cast<int>(blockIdx.x) * 64 + threadIdx.x;  // sigma * ns + s
+    if (idx >= sh.NZ * ns) return;
+    const int sg = idx / ns, s = idx - sg * ns;
+    auto q = [&](int k) { return s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg); };
+    int n = 0;
+    for (int k = 0; k < K; ++k)
+        if (coef64[q(k)] != 0.0) {
+            pcls[static_cast<int64_t>(idx) * K + k] = n;
+            lidx[static_cast<int64_t>(idx) * K + n++] = k;
+        }
+    nreal[idx] = n;
+    for (int k = 0; k < K; ++k) {
+        if (coef64[q(k)] != 0.0) continue;
+        int rep = -1;
+        for (int r = nreal[idx]; r < n; ++r)
+            if (alpha64[q(lidx[static_cast<int64_t>(idx) * K + r])] == alpha64[q(k)]) rep = r;
+        if (rep < 0) {
+            rep = n;
+            lidx[static_cast<int64_t>(idx) * K + n++] = k;
+        }
+        pcls[static_cast<int64_t>(idx) * K + k] = rep;
+    }
+    nlist[idx] = n;
+}
+
+template <typename T, int LA, int LB, int LC>
+__device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, int sc, const int (&sg)[3],
+                                           const double *__restrict__ coef64, const double *__restrict__ alpha64,
+                                           const int *__restrict__ nreal, const int *__restrict__ nlist,
+                                           const int *__restrict__ lidx, const Geo3 &geo, const T (&ab)[3],
+                                           const T (&ac)[3], const T *__restrict__ g, double *__restrict__ row)
+{
+    constexpr int NBA = Sh3<LA>::nbf, NBB = Sh3<LB>::nbf;
+    constexpr int N1 = 2 * LA + 1, N2 = 2 * LB + 1;
+    const int ns = sh.ns, K = sh.K, N = sh.N;
+    const int ia = sg[0] * ns + sa, ib = sg[1] * ns + sb, ic = sg[2] * ns + sc;
+    const int nra = nreal[ia], nrb = nreal[ib], nrc = nreal[ic];
+    const int nla = nlist[ia], nlb = nlist[ib], nlc = nlist[ic];
+    double *rA = row + 2 * (static_cast<int64_t>(0 * ns + sa) * K);
+    double *rB = row + 2 * (static_cast<int64_t>(1 * ns + sb) * K);
+    double *rC = row + 2 * (static_cast<int64_t>(2 * ns + sc) * K);
+    for (int i = 0; i < nla; ++i)
+        for (int j = 0; j < nlb; ++j)
+            for (int k = 0; k < nlc; ++k) {
+                if ((i >= nra) + (j >= nrb) + (k >= nrc) > 1) continue;
+                const int64_t qa = sa + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ia) * K + i] +
+                                                                    static_cast<int64_t>(K) * sg[0]);
+                const int64_t qb = sb + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ib) * K + j] +
+                                                                    static_cast<int64_t>(K) * sg[1]);
+                const int64_t qc = sc + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ic) * K + k] +
+                                                                    static_cast<int64_t>(K) * sg[2]);
+                const double ca = coef64[qa], cb = coef64[qb], cc = coef64[qc];
+                const double al = alpha64[qa], be = alpha64[qb], ga = alpha64[qc];
+                const double p = al + be, q = p + ga, iq = 1.0 / q, ip = 1.0 / p;
+                const double w = 5.568327996831707845 * iq * sqrt(iq) *
+                                 exp(-(al * be * geo.rab + al * ga * geo.rac + be * ga * geo.rbc) * iq);
+                const T aop = static_cast<T>(al * ip), bop = static_cast<T>(be * ip), gq = static_cast<T>(ga * iq);
+                const T pq = static_cast<T>(p * iq), i2q = static_cast<T>(0.5 * iq);
+                T sx[LA + 3][LB + 3][LC + 3], sy[LA + 3][LB + 3][LC + 3], sz[LA + 3][LB + 3][LC + 3];
+                {
+                    const T PA = -bop * ab[0], PB = aop * ab[0], PC = PA + ac[0];
+                    os3<T, LA + 2, LB + 2, LC + 2>(PA - gq * PC, PB - gq * PC, pq * PC, i2q, sx);
+                }
+                {
+                    const T PA = -bop * ab[1], PB = aop * ab[1], PC = PA + ac[1];
+                    os3<T, LA + 2, LB + 2, LC + 2>(PA - gq * PC, PB - gq * PC, pq * PC, i2q, sy);
+                }
+                {
+                    const T PA = -bop * ab[2], PB = aop * ab[2], PC = PA + ac[2];
+                    os3<T, LA + 2, LB + 2, LC + 2>(PA - gq * PC, PB - gq * PC, pq * PC, i2q, sz);
+                }
+                T H = T(0), Ha = T(0), Hb = T(0), Hc = T(0);
+                int index1 = 0;
+#pragma unroll
+                for (int ll1 = LA; ll1 >= 0; --ll1)
+#pragma unroll
+                    for (int n1 = 0; n1 <= LA - ll1; ++n1) {
+                        const int m1 = LA - ll1 - n1;
+                        int index2 = 0;
+#pragma unroll
+                        for (int ll2 = LB; ll2 >= 0; --ll2)
+#pragma unroll
+                            for (int n2 = 0; n2 <= LB - ll2; ++n2) {
+                                const int m2 = LB - ll2 - n2;
+                                int index3 = 0;
+#pragma unroll
+                                for (int ll3 = LC; ll3 >= 0; --ll3)
+#pragma unroll
+                                    for (int n3 = 0; n3 <= LC - ll3; ++n3) {
+                                        const int m3 = LC - ll3 - n3;
+                                        const int lin = index1 + N1 * index2 + N1 * N2 * index3;
+                                        const int ii = lin % NBA, jj = (lin / NBA) % NBB, kk = lin / (NBA * NBB);
+                                        const T gv = g[(sh.off[sa] + ii) +
+                                                       static_cast<int64_t>(N) *
+                                                           ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
+                                        const T x = sx[ll1][ll2][ll3], y = sy[m1][m2][m3], z = sz[n1][n2][n3];
+                                        H += gv * (x * y * z);
+                                        Ha += gv * (sx[ll1 + 2][ll2][ll3] * y * z + x * sy[m1 + 2][m2][m3] * z +
+                                                    x * y * sz[n1 + 2][n2][n3]);
+                                        Hb += gv * (sx[ll1][ll2 + 2][ll3] * y * z + x * sy[m1][m2 + 2][m3] * z +
+                                                    x * y * sz[n1][n2 + 2][n3]);
+                                        Hc += gv * (sx[ll1][ll2][ll3 + 2] * y * z + x * sy[m1][m2][m3 + 2] * z +
+                                                    x * y * sz[n1][n2][n3 + 2]);
+                                        ++index3;
+                                    }
+                                ++index2;
+                            }
+                        ++index1;
+                    }
+                const double wH = w * static_cast<double>(H);
+                const double cw = ca * cb * cc * w;
+                rA[2 * i] += cb * cc * wH;
+                rB[2 * j] += ca * cc * wH;
+                rC[2 * k] += ca * cb * wH;
+                rA[2 * i + 1] -= cw * static_cast<double>(Ha);
+                rB[2 * j + 1] -= cw * static_cast<double>(Hb);
+                rC[2 * k + 1] -= cw * static_cast<double>(Hc);
+            }
+}
+
+template <typename T, int LMAXB>
+__global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t B, const T *__restrict__ dT,
+                                                     const T *__restrict__ posA, const T *__restrict__ posB,
+                                                     const T *__restrict__ posC, T scale, const void *__restrict__ sA,
+                                                     const void *__restrict__ sB, const void *__restrict__ sC,
+                                                     int itype, const double *__restrict__ coef64,
+                                                     const double *__restrict__ alpha64,
+                                                     const int *__restrict__ nreal, const int *__restrict__ nlist,
+                                                     const int *__restrict__ lidx, double *__restrict__ rows,
+                                                     unsigned *__restrict__ err)
+{
+    const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
+    if (b >= B) return;
+    const int64_t rlen = 6 * static_cast<int64_t>(ns) * K;
+    double *row = rows + b * rlen;
+    for (int64_t q = 0; q < rlen; ++q) row[q] = 0.0;
+    int sg[3] = {load_sigma(sA, itype, b), load_sigma(sB, itype, b), load_sigma(sC, itype, b)};
+    for (int c = 0; c < 3; ++c)
+        if (sg[c] < 0 || sg[c] >= NZ) {
+            *err = ERR_SPECIES;
+            return;
+        }
+    T A[3], Bv[3], C[3];
+    Geo3 geo{0.0, 0.0, 0.0};
+    T ab[3], ac[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        A[ax] = scale * posA[3 * b + ax];
+        Bv[ax] = scale * posB[3 * b + ax];
+        C[ax] = scale * posC[3 * b + ax];
+        const double dab = static_cast<double>(A[ax]) - static_cast<double>(Bv[ax]);
+        const double dac = static_cast<double>(A[ax]) - static_cast<double>(C[ax]);
+        const double dbc = static_cast<double>(Bv[ax]) - static_cast<double>(C[ax]);
+        geo.rab += dab * dab;
+        geo.rac += dac * dac;
+        geo.rbc += dbc * dbc;
+        ab[ax] = static_cast<T>(dab);
+        ac[ax] = static_cast<T>(dac);
+    }
+    const T *g = dT + static_cast<int64_t>(N) * N * N * b;
+    for (int sc = 0; sc < ns; ++sc)
+        for (int sb = 0; sb < ns; ++sb)
+            for (int sa = 0; sa < ns; ++sa) {
+                const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
+#define AOK_B3_CASE(A_, B_, C_)                                                                                  \
+    if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB)                                                     \
+        if (la == A_ && lb == B_ && lc == C_) {                                                                  \
+            triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, coef64, alpha64, nreal, nlist, lidx, geo, ab, ac, g,  \
+                                      row);                                                                      \
+            continue;                                                                                            \
+        }
+                AOK_L3_CASES(AOK_B3_CASE)
+#undef AOK_B3_CASE
+            }
+}
+
+// Block per slot (sigma, s, k): sum over the batch of the rows whose species on that
+// side is sigma, sides A, B, C; thread-strided partial sums, then a block tree.
+template <typename T>
+__global__ __launch_bounds__(256) void ovl3_bwd_reduce(const OvlShape sh, int64_t B, const void *__restrict__ sA,
+                                                       const void *__restrict__ sB, const void *__restrict__ sC,
+                                                       int itype, const int *__restrict__ pcls,
+                                                       const double *__restrict__ rows, T *__restrict__ dcoef,
+                                                       T *__restrict__ dalpha)
+{
+    __shared__ double red[256];
+    const int ns = sh.ns, K = sh.K;
+    const int q = blockIdx.x;  // [s + ns * (k + K * sg)], the Julia index
+    const int s = q % ns, k = (q / ns) % K, sg = q / (ns * K);
+    const int li = pcls[static_cast<int64_t>(sg * ns + s) * K + k];
+    const int64_t rlen = 6 * static_cast<int64_t>(ns) * K;
+    const void *sides[3] = {sA, sB, sC};
+    double c = 0.0, a = 0.0;
+    for (int64_t b = threadIdx.x; b < B; b += 256)
+        for (int side = 0; side < 3; ++side) {
+            if (load_sigma(sides[side], itype, b) != sg) continue;
+            const double *r = rows + b * rlen + 2 * (static_cast<int64_t>(side * ns + s) * K + li);
+            c += r[0];
+            a += r[1];
+        }
+    c = block_sum256(c, red);
+    a = block_sum256(a, red);
+    if (threadIdx.x == 0) {
+        dcoef[q] = static_cast<T>(c);
+        dalpha[q] = static_cast<T>(a);
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Host orchestration
 // ---------------------------------------------------------------------------------
 template <typename T, int LMAXB>
@@ -607,6 +844,70 @@ aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c)
     }
 }
 
+// (coef, alpha) pullback: lists, per-element rows, fixed-order reduction.
+template <typename T, int LMAXB>
+static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &sh)
+{
+    const size_t npar = static_cast<size_t>(sh.ns) * sh.K * sh.NZ;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    const size_t o_c = take(sizeof(double) * npar), o_a = take(sizeof(double) * npar);
+    const size_t o_nr = take(sizeof(int) * sh.NZ * sh.ns), o_nl = take(sizeof(int) * sh.NZ * sh.ns);
+    const size_t o_li = take(sizeof(int) * npar), o_pc = take(sizeof(int) * npar);
+    const size_t o_rows = take(sizeof(double) * 6 * static_cast<size_t>(sh.ns) * sh.K * static_cast<size_t>(c.B > 0 ? c.B : 1));
+    aok_status st = ensure_ws(ctx, o);
+    if (st != AOK_OK) return st;
+    char *ws = static_cast<char *>(ctx->ws);
+    double *c64 = reinterpret_cast<double *>(ws + o_c), *a64 = reinterpret_cast<double *>(ws + o_a);
+    hipLaunchKernelGGL((widen_pair<T>), dim3(static_cast<unsigned>((npar + 255) / 256)), dim3(256), 0, ctx->stream,
+                       static_cast<int64_t>(npar), static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha), c64,
+                       a64);
+    AOK_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(ovl3_pb_lists, dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh, c64, a64,
+                       reinterpret_cast<int *>(ws + o_nr), reinterpret_cast<int *>(ws + o_nl),
+                       reinterpret_cast<int *>(ws + o_li), reinterpret_cast<int *>(ws + o_pc));
+    AOK_HIP(ctx, hipGetLastError());
+    if (c.B > 0) {
+        time_begin(ctx, "ovl3c_bwd_rows");
+        hipLaunchKernelGGL((ovl3c_bwd_rows<T, LMAXB>), dim3(static_cast<unsigned>((c.B + 63) / 64)), dim3(64), 0,
+                           ctx->stream, sh, c.B, static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
+                           static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
+                           c.sidxA, c.sidxB, c.sidxC, c.itype, c64, a64, reinterpret_cast<const int *>(ws + o_nr),
+                           reinterpret_cast<const int *>(ws + o_nl), reinterpret_cast<const int *>(ws + o_li),
+                           reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
+        time_end(ctx);
+        AOK_HIP(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL((ovl3_bwd_reduce<T>), dim3(static_cast<unsigned>(npar)), dim3(256), 0, ctx->stream, sh, c.B,
+                       c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const int *>(ws + o_pc),
+                       reinterpret_cast<const double *>(ws + o_rows), static_cast<T *>(c.o1),
+                       static_cast<T *>(c.o2));
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+template <typename T>
+aok_status ovl3c_pullback_run(aok_ctx ctx, const Ovl3cCall &c)
+{
+    OvlShape sh;
+    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, OVL3_FAST_LMAX, 1 << 20, sh);
+    if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
+    if (c.B < 0 || c.B > (int64_t(1) << 28)) return fail(ctx, AOK_ERR_ARG, "B out of range");
+    int lmax = 0;
+    for (int q = 0; q < sh.ns; ++q) lmax = std::max<int>(lmax, sh.ls[q]);
+    switch (lmax) {
+        case 0: return bwd3_launch<T, 0>(ctx, c, sh);
+        case 1: return bwd3_launch<T, 1>(ctx, c, sh);
+        default: return bwd3_launch<T, 2>(ctx, c, sh);
+    }
+}
+
+template aok_status ovl3c_pullback_run<float>(aok_ctx, const Ovl3cCall &);
+template aok_status ovl3c_pullback_run<double>(aok_ctx, const Ovl3cCall &);
 template aok_status ovl3c_run<float>(aok_ctx, const Ovl3cCall &);
 template aok_status ovl3c_run<double>(aok_ctx, const Ovl3cCall &);
 

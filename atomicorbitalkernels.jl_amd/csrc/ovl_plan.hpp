@@ -203,6 +203,8 @@ struct Ovl3cCall {
     const void *coef, *alpha;
     double scale;
     void *out;
+    const void *dT;  // pullback: cotangent [N*N*N*B]
+    void *o1, *o2;   // pullback outputs (dcoef, dalpha)
 };
 
 // A reusable 2-centre plan (owned device workspace + the call description).
@@ -229,6 +231,8 @@ template <typename T>
 aok_status ovl2c_plan_dx(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall &io);
 template <typename T>
 aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c);
+template <typename T>
+aok_status ovl3c_pullback_run(aok_ctx ctx, const Ovl3cCall &c);
 template <typename T>
 aok_status compile_coef_run(aok_ctx ctx, const void *zeta, const void *D, int64_t ns, int64_t K, int64_t NZ,
                             const int64_t *ls, void *coef);

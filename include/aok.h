@@ -234,6 +234,20 @@ aok_status aok_ovl3c(aok_ctx ctx, aok_dtype dtype, unsigned flags,
                      int64_t ns, int64_t K, int64_t NZ, const int64_t *ls,
                      const void *coef, const void *alpha, double lengthscale, void *out);
 
+/* No reference counterpart (the reference has no 3-centre pullback).  Parameter
+ * cotangents of aok_ovl3c: dcoef, dalpha [ns*K*NZ] (dtype) from dT [N*N*N*B], with the
+ * rules of aok_ovl2c_pullback (every primitive slot receives its coefficient cotangent,
+ * padded slots included; exponent cotangents by the raised-momentum identity; FP64 sums
+ * in a fixed order).  Shells up to l = 2 (AOK_ERR_LMAX beyond); flags are ignored (one
+ * implementation).  The (zeta, D) cotangents of an orbital basis follow with
+ * aok_compile_coef_pb: dzeta = dalpha + dzeta_norm. */
+aok_status aok_ovl3c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *dT,
+                              const void *posA, const void *posB, const void *posC,
+                              const void *sidxA, const void *sidxB, const void *sidxC,
+                              aok_itype itype, int64_t B, int64_t ns, int64_t K, int64_t NZ,
+                              const int64_t *ls, const void *coef, const void *alpha,
+                              double lengthscale, void *dcoef, void *dalpha);
+
 /* ---- tuning -------------------------------------------------------------------------------- */
 /* No reference counterpart (the reference has no tunables): set / read a per-context
  * launch-shape knob used by the kernels and by the autotuner.  Keys:

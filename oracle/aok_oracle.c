@@ -724,3 +724,105 @@ int64_t orc_ovl3c(const double *posA, const double *posB, const double *posC,
     free(nbf);
     return N;
 }
+
+/* Parameter cotangents of the 3-centre overlap.  No reference counterpart (the reference
+ * has no 3-centre pullback).  As the 2-centre pullback (overlap_2c_grad.jl:71-214): every
+ * K^3 primitive triple, padded slots included (a padded slot's coefficient cotangent is
+ * the derivative at coef = 0); E3 tables to l+2 on every centre; the cotangent read back
+ * through the 3-centre alias map.  d/dalpha x^l e^{-alpha x^2} = -x^{l+2} e^{-alpha x^2},
+ * so dV/dalpha_a is minus the A-raised integral summed over the three axes. */
+void orc_ovl3c_pb(const double *dT, const double *posA, const double *posB, const double *posC,
+                  const int64_t *sidxA, const int64_t *sidxB, const int64_t *sidxC, int64_t B,
+                  int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, const double *coef,
+                  const double *alpha, double *dcoef, double *dalpha)
+{
+    int64_t *nbf = (int64_t *)malloc(sizeof(int64_t) * (size_t)(2 * ns + 1));
+    int64_t *off = nbf + ns;
+    const int64_t N = shell_layout(ns, ls, nbf, off);
+    memset(dcoef, 0, sizeof(double) * (size_t)(ns * K * NZ));
+    memset(dalpha, 0, sizeof(double) * (size_t)(ns * K * NZ));
+    const size_t e3n = (size_t)(3 * (ORC_LMAX + 2) + 2) * (ORC_LMAX + 3) * (ORC_LMAX + 3) * (ORC_LMAX + 3);
+    double *E3 = (double *)malloc(sizeof(double) * 3 * e3n);
+    for (int64_t b = 0; b < B; ++b) {
+        const int64_t sA = sidxA ? sidxA[b] - 1 : 0, sB = sidxB ? sidxB[b] - 1 : 0,
+                      sC = sidxC ? sidxC[b] - 1 : 0;
+        const double *A = posA + 3 * b, *Bv = posB + 3 * b, *C = posC + 3 * b;
+        const double *g_b = dT + N * N * N * b;
+        for (int64_t sc = 0; sc < ns; ++sc)
+            for (int64_t sb = 0; sb < ns; ++sb)
+                for (int64_t sa = 0; sa < ns; ++sa) {
+                    const int la = (int)ls[sa], lb = (int)ls[sb], lc = (int)ls[sc];
+                    const int LA = la + 2, LB = lb + 2, LC = lc + 2;
+                    const int64_t na = nbf[sa], nb = nbf[sb];
+                    const int N1 = 2 * la + 1, N2 = 2 * lb + 1;
+                    for (int64_t ip = 0; ip < K; ++ip)
+                        for (int64_t jp = 0; jp < K; ++jp)
+                            for (int64_t kp = 0; kp < K; ++kp) {
+                                const int64_t qa = sa + ns * (ip + K * sA), qb = sb + ns * (jp + K * sB),
+                                              qc = sc + ns * (kp + K * sC);
+                                const double ca = coef[qa], cb = coef[qb], cc = coef[qc];
+                                /* two zero coefficients make every cotangent of the triple 0 */
+                                if ((ca == 0.0) + (cb == 0.0) + (cc == 0.0) > 1) continue;
+                                const double aa = alpha[qa], ab = alpha[qb], ac = alpha[qc];
+                                const double p = aa + ab, q = p + ac;
+                                const double mu = aa * ab / p, nu = p * ac / q, oo2q = 0.5 / q;
+                                const double pref0 = pow(ORC_PI / q, 1.5);
+                                for (int ax = 0; ax < 3; ++ax) {
+                                    const double P = (aa * A[ax] + ab * Bv[ax]) / p;
+                                    const double Qc = (p * P + ac * C[ax]) / q;
+                                    const double AB = A[ax] - Bv[ax], PC = P - C[ax];
+                                    md_e3tab(E3 + e3n * (size_t)ax, LA, LB, LC, Qc - A[ax], Qc - Bv[ax], Qc - C[ax],
+                                             oo2q, exp(-mu * AB * AB - nu * PC * PC));
+                                }
+                                const double *Ex = E3, *Ey = E3 + e3n, *Ez = E3 + 2 * e3n;
+#define ORC_E(E, i, j, k) e30((E), LA, LB, LC, (i), (j), (k))
+                                double H = 0.0, Ha = 0.0, Hb = 0.0, Hc = 0.0;
+                                int index1 = 0;
+                                for (int ll1 = la; ll1 >= 0; --ll1)
+                                    for (int n1 = 0; n1 <= la - ll1; ++n1) {
+                                        const int m1 = la - ll1 - n1;
+                                        int index2 = 0;
+                                        for (int ll2 = lb; ll2 >= 0; --ll2)
+                                            for (int n2 = 0; n2 <= lb - ll2; ++n2) {
+                                                const int m2 = lb - ll2 - n2;
+                                                int index3 = 0;
+                                                for (int ll3 = lc; ll3 >= 0; --ll3)
+                                                    for (int n3 = 0; n3 <= lc - ll3; ++n3) {
+                                                        const int m3 = lc - ll3 - n3;
+                                                        const int64_t lin = index1 + N1 * index2 + N1 * N2 * index3;
+                                                        const int64_t ii = lin % na, jj = (lin / na) % nb,
+                                                                      kk = lin / (na * nb);
+                                                        const double g =
+                                                            g_b[(off[sa] + ii) + N * ((off[sb] + jj) + N * (off[sc] + kk))];
+                                                        const double x = ORC_E(Ex, ll1, ll2, ll3),
+                                                                     y = ORC_E(Ey, m1, m2, m3),
+                                                                     z = ORC_E(Ez, n1, n2, n3);
+                                                        H += g * x * y * z;
+                                                        Ha -= g * (ORC_E(Ex, ll1 + 2, ll2, ll3) * y * z +
+                                                                   x * ORC_E(Ey, m1 + 2, m2, m3) * z +
+                                                                   x * y * ORC_E(Ez, n1 + 2, n2, n3));
+                                                        Hb -= g * (ORC_E(Ex, ll1, ll2 + 2, ll3) * y * z +
+                                                                   x * ORC_E(Ey, m1, m2 + 2, m3) * z +
+                                                                   x * y * ORC_E(Ez, n1, n2 + 2, n3));
+                                                        Hc -= g * (ORC_E(Ex, ll1, ll2, ll3 + 2) * y * z +
+                                                                   x * ORC_E(Ey, m1, m2, m3 + 2) * z +
+                                                                   x * y * ORC_E(Ez, n1, n2, n3 + 2));
+                                                        ++index3;
+                                                    }
+                                                ++index2;
+                                            }
+                                        ++index1;
+                                    }
+#undef ORC_E
+                                dcoef[qa] += cb * cc * pref0 * H;
+                                dcoef[qb] += ca * cc * pref0 * H;
+                                dcoef[qc] += ca * cb * pref0 * H;
+                                dalpha[qa] += ca * cb * cc * pref0 * Ha;
+                                dalpha[qb] += ca * cb * cc * pref0 * Hb;
+                                dalpha[qc] += ca * cb * cc * pref0 * Hc;
+                            }
+                }
+    }
+    free(E3);
+    free(nbf);
+}

@@ -90,6 +90,13 @@ int64_t orc_ovl3c(const double *posA, const double *posB, const double *posC,
                   int64_t B, int64_t ns, int64_t K, int64_t NZ, const int64_t *ls,
                   const double *coef, const double *alpha, double *out);
 
+/* Parameter cotangents of the 3-centre overlap (no reference counterpart): dcoef,
+ * dalpha [ns*K*NZ] from dT [N*N*N*B]; every primitive slot, padded ones included. */
+void orc_ovl3c_pb(const double *dT, const double *posA, const double *posB, const double *posC,
+                  const int64_t *sidxA, const int64_t *sidxB, const int64_t *sidxC, int64_t B,
+                  int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, const double *coef,
+                  const double *alpha, double *dcoef, double *dalpha);
+
 /* OpenMP variants used only for the CPU baseline timing (same arithmetic). */
 int64_t orc_ovl2c_omp(const double *posA, const double *posB, const int64_t *sidxA,
                       const int64_t *sidxB, int64_t B, int64_t ns, int64_t K, int64_t NZ,

@@ -60,6 +60,8 @@ def lib():
         L.orc_ovl3c.argtypes = [_D, _D, _D, _I, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                 ctypes.c_int64, _I, _D, _D, _D]
         L.orc_ovl3c.restype = ctypes.c_int64
+        L.orc_ovl3c_pb.argtypes = [_D, _D, _D, _D, _I, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_int64, _I, _D, _D, _D, _D]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -255,6 +257,37 @@ def ovl3c(posA, posB, posC, sidxA, sidxB, sidxC, ls, coef, alpha):
     if rc < 0:
         raise ValueError(f"oracle ovl3c rc={rc}")
     return out.reshape((N, N, N, B), order="F")
+
+
+def ovl3c_pb(dT, posA, posB, posC, sidxA, sidxB, sidxC, ls, coef, alpha):
+    """(dcoef, dalpha), each (ns, K, NZ), of the 3-centre overlap (Bohr positions)."""
+    posA = np.asarray(posA, np.float64)
+    B = posA.shape[0]
+    ns, K, NZ = coef.shape
+    ga, gp = _d(_fortran_flat(dT))
+    Aa, Ap = _d(posA.reshape(-1))
+    Ba, Bp = _d(np.asarray(posB, np.float64).reshape(-1))
+    Ca, Cp = _d(np.asarray(posC, np.float64).reshape(-1))
+    sa, sap = _i(sidxA)
+    sb, sbp = _i(sidxB)
+    sc, scp = _i(sidxC)
+    la, lp = _i(ls)
+    ca, cp = _d(_fortran_flat(coef))
+    aa, ap = _d(_fortran_flat(alpha))
+    dc = np.zeros(ns * K * NZ)
+    da = np.zeros(ns * K * NZ)
+    lib().orc_ovl3c_pb(gp, Ap, Bp, Cp, sap, sbp, scp, B, ns, K, NZ, lp, cp, ap, dc.ctypes.data_as(_D),
+                       da.ctypes.data_as(_D))
+    shp = (ns, K, NZ)
+    return dc.reshape(shp, order="F"), da.reshape(shp, order="F")
+
+
+def pullback_ps_3c(dT, posA, posB, posC, sidxA, sidxB, sidxC, ls, zeta, D):
+    """(dzeta, dD) of the 3-centre overlap of an orbital basis (coef compiled from (zeta, D))."""
+    coef = compile_coef(zeta, D, ls)
+    dc, da = ovl3c_pb(dT, posA, posB, posC, sidxA, sidxB, sidxC, ls, coef, zeta)
+    dD, dzn = compile_coef_pb(dc, zeta, D, ls)
+    return da + dzn, dD
 
 
 def pullback_ps_2c(dS, posA, posB, sidxA, sidxB, ls, zeta, D):

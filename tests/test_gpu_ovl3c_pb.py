@@ -1,0 +1,97 @@
+"""GPU parity tests of the 3-centre parameter pullback (aok_ovl3c_pullback and the Python
+rrule_batch_overlap_3c; no reference counterpart, pinned through the oracle's
+orc_ovl3c_pb, itself pinned by finite differences and the 3c -> 2c reduction in
+tests/test_oracle.py).
+
+Tolerances (relative to the largest |cotangent|): FP64 1e-10; FP32 vs the FP64 oracle on
+the same FP32-rounded inputs 1e-5 (FP32 terms, FP64 sums over the batch)."""
+import numpy as np
+import pytest
+import torch
+
+import aok
+import oracle as O
+from aok import configs
+from helpers import dev_f, dev_rows, host, log, rel_max, rounded
+
+pytestmark = pytest.mark.gpu
+
+
+def _basis(ls, zeta, coef, dtype, dev):
+    nbf = [(l + 1) * (l + 2) // 2 for l in ls]
+    off = [0]
+    for n in nbf:
+        off.append(off[-1] + n)
+    return aok.CartesianGTOBasis(max(ls), len(ls), zeta.shape[1], list(ls), nbf, off, off[-1],
+                                 dev_f(zeta, dtype, dev), dev_f(coef, dtype, dev),
+                                 tuple(range(1, zeta.shape[2] + 1)), 1.0)
+
+
+def _inputs(d, dtype, dev):
+    return [(dev_rows(d[f"pos{c}"], dtype, dev), torch.as_tensor(d[f"sidx{c}"], device=dev)) for c in "ABC"]
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
+def test_c4a_coef_alpha_pullback_vs_oracle(dev, dtype, tol):
+    B = 60  # the FP64 oracle walks every primitive triple: seconds per 10 elements
+    d = configs.c4a(B=B)
+    cb = d["basis"]
+    log(f"3c pullback C4a B={B} dtype={dtype}")
+    coef = rounded(O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls), dtype)
+    basis = _basis(cb.ls, cb.zeta, coef, dtype, dev)
+    rng = np.random.default_rng(9)
+    dT = rng.standard_normal((cb.N, cb.N, cb.N, B))
+    dc, da = aok.overlap_3c_pb_coef_alpha(dev_f(dT, dtype, dev), basis, *_inputs(d, dtype, dev))
+    rc, ra = O.ovl3c_pb(rounded(dT, dtype), *(rounded(d[f"pos{c}"], dtype) for c in "ABC"), d["sidxA"], d["sidxB"],
+                        d["sidxC"], cb.ls, coef, rounded(cb.zeta, dtype))
+    assert rel_max(host(dc), rc) < tol
+    assert rel_max(host(da), ra) < tol
+    # padded slots: a coefficient cotangent, no exponent cotangent (coef = 0)
+    pad = cb.D == 0
+    assert np.all(host(da)[pad] == 0.0)
+    assert np.any(host(dc)[pad] != 0.0)
+
+
+def test_orbital_rrule_vs_oracle(dev):
+    """(zeta, D) cotangents through the FP64 normalisation VJP, FP64."""
+    B = 40
+    d = configs.c4a(B=B)
+    cb = d["basis"]
+    log(f"3c rrule (zeta, D) C4a B={B}")
+    spec, _, _ = cb.ao_maps()
+    R = aok.GaussianTypeRadials(dev_f(cb.zeta, torch.float64, dev), dev_f(cb.D, torch.float64, dev), spec, None,
+                                tuple(range(1, cb.NZ + 1)))
+    orb = aok.AtomicOrbitals(R, max(cb.ls), length_unit="bohr")
+    X = _inputs(d, torch.float64, dev)
+    T, pb = aok.rrule_batch_overlap_3c(orb, *X, orb.static_params())
+    Tr = O.ovl3c(*(d[f"pos{c}"] for c in "ABC"), d["sidxA"], d["sidxB"], d["sidxC"], cb.ls,
+                 O.compile_coef(cb.zeta, cb.D, cb.ls), cb.zeta)
+    assert rel_max(host(T), Tr) < 1e-12
+    rng = np.random.default_rng(10)
+    dT = rng.standard_normal(Tr.shape)
+    dps = pb(dev_f(dT, torch.float64, dev))[5]
+    rz, rD = O.pullback_ps_3c(dT, *(d[f"pos{c}"] for c in "ABC"), d["sidxA"], d["sidxB"], d["sidxC"], cb.ls,
+                              cb.zeta, cb.D)
+    assert rel_max(host(dps["Rnl"]["zeta"]), rz) < 1e-10
+    assert rel_max(host(dps["Rnl"]["D"]), rD) < 1e-10
+
+
+def test_edges(dev):
+    d = configs.c4a(B=4)
+    cb = d["basis"]
+    coef = O.compile_coef(cb.zeta, cb.D, cb.ls)
+    basis = _basis(cb.ls, cb.zeta, coef, torch.float64, dev)
+    log("3c pullback edges")
+    z = (dev_rows(np.zeros((0, 3)), torch.float64, dev), None)
+    dT0 = torch.zeros((cb.N, cb.N, cb.N, 0), dtype=torch.float64, device=dev)
+    dc, da = aok.overlap_3c_pb_coef_alpha(dT0, basis, z, z, z)
+    assert torch.count_nonzero(dc) == 0 and torch.count_nonzero(da) == 0
+    # shells beyond d are not supported by the 3-centre pullback
+    ls = [0, 3]
+    zeta = np.ones((2, 1, 1))
+    c1 = np.ones((2, 1, 1))
+    b3 = _basis(ls, zeta, c1, torch.float64, dev)
+    N = 1 + 10
+    P = (dev_rows(np.zeros((1, 3)), torch.float64, dev), None)
+    with pytest.raises(aok.AOKLmaxError):
+        aok.overlap_3c_pb_coef_alpha(torch.zeros((N, N, N, 1), dtype=torch.float64, device=dev), b3, P, P, P)

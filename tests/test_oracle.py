@@ -195,6 +195,73 @@ def test_position_cotangent_ss_closed_form():
     np.testing.assert_allclose(dA[0], -2 * mu * (A[0] - B[0]) * S[0, 1, 0], rtol=1e-13)
 
 
+def _three_centre_case(seed, ls=(0, 1, 2, 1), K=2, NZ=2, B=3):
+    rng = np.random.default_rng(seed)
+    ls = list(ls)
+    zeta = np.exp(rng.uniform(-1, 1.2, size=(len(ls), K, NZ)))
+    D = rng.standard_normal((len(ls), K, NZ))
+    D[3, 1, :] = 0.0  # a padded slot (zeta 1)
+    zeta[3, 1, :] = 1.0
+    P = [rng.normal(0, 0.6, size=(B, 3)) for _ in range(3)]
+    S = [rng.integers(1, NZ + 1, B) for _ in range(3)]
+    return ls, zeta, D, P, S
+
+
+def test_3c_param_pullback_vs_fd():
+    """orc_ovl3c_pb (coef, alpha) and its (zeta, D) chain vs central differences of
+    <dT, T>, padded slot included, shells up to l = 2, two species per centre."""
+    ls, zeta, D, P, S = _three_centre_case(3)
+    coef = O.compile_coef(zeta, D, ls)
+    T = O.ovl3c(*P, *S, ls, coef, zeta)
+    rng = np.random.default_rng(4)
+    dT = rng.standard_normal(T.shape)
+    dc, da = O.ovl3c_pb(dT, *P, *S, ls, coef, zeta)
+    h = 1e-6
+    f = lambda c, a: np.sum(dT * O.ovl3c(*P, *S, ls, c, a))
+    V = rng.standard_normal(coef.shape)
+    fd = (f(coef + h * V, zeta) - f(coef - h * V, zeta)) / (2 * h)
+    assert abs(np.sum(dc * V) - fd) <= 1e-7 * max(1.0, abs(fd))
+    W = rng.standard_normal(zeta.shape) * 0.1
+    fd = (f(coef, zeta + h * W) - f(coef, zeta - h * W)) / (2 * h)
+    assert abs(np.sum(da * W) - fd) <= 1e-6 * max(1.0, abs(fd))
+    # the padded slot (coef 0) has a coefficient cotangent and no exponent cotangent
+    assert dc[3, 1, 0] != 0.0 and da[3, 1, 0] == 0.0
+    # (zeta, D) through the normalisation
+    dz, dD = O.pullback_ps_3c(dT, *P, *S, ls, zeta, D)
+    g = lambda z, d: np.sum(dT * O.ovl3c(*P, *S, ls, O.compile_coef(z, d, ls), z))
+    fd = (g(zeta + h * W, D) - g(zeta - h * W, D)) / (2 * h)
+    assert abs(np.sum(dz * W) - fd) <= 1e-6 * max(1.0, abs(fd))
+    fd = (g(zeta, D + h * V) - g(zeta, D - h * V)) / (2 * h)
+    assert abs(np.sum(dD * V) - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_3c_pullback_reduces_to_2c():
+    """A third centre with an s shell of exponent ~0 (1e-14: exactly 0 would make p = 0 in
+    the s3 x s3 pairs, whose cotangent is zero anyway) and coefficient 1 leaves the
+    2-centre overlap (q -> p, Q -> P): the 3c (coef, alpha) cotangents of the A and B shells
+    equal the 2-centre pullback's (orc_ovl2c_pb, an independent code path)."""
+    rng = np.random.default_rng(12)
+    ls2 = [0, 1, 2]
+    K, B = 2, 3
+    zeta2 = np.exp(rng.uniform(-0.5, 1.0, size=(3, K, 1)))
+    coef2 = rng.standard_normal((3, K, 1))
+    A, Bp = rng.normal(0, 0.5, (B, 3)), rng.normal(0, 0.5, (B, 3))
+    S = O.ovl2c(A, Bp, None, None, ls2, coef2, zeta2)
+    dS = rng.standard_normal(S.shape)
+    dc2, da2 = O.ovl2c_pb(dS, A, Bp, None, None, ls2, coef2, zeta2)
+    ls3 = ls2 + [0]
+    zeta3 = np.concatenate([zeta2, np.array([[[1e-14], [1.0]]])], axis=0)
+    coef3 = np.concatenate([coef2, np.array([[[1.0], [0.0]]])], axis=0)
+    N2, N3 = S.shape[0], S.shape[0] + 1
+    dT = np.zeros((N3, N3, N3, B))
+    dT[:N2, :N2, N2, :] = dS
+    C = rng.normal(0, 0.5, (B, 3))
+    dc3, da3 = O.ovl3c_pb(dT, A, Bp, C, None, None, None, ls3, coef3, zeta3)
+    # A-side and B-side contributions both land on the shared (ns, K) slots: compare sums
+    np.testing.assert_allclose(dc3[:3], dc2, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(da3[:3], da2, rtol=1e-10, atol=1e-12)
+
+
 def test_normalisation_vjp_vs_fd():
     rng = np.random.default_rng(2)
     ls = [0, 1, 2]
