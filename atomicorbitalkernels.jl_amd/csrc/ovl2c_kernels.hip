@@ -430,22 +430,24 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
         }
 }
 
-// Pullback of one shell pair: per-pair contributions reduced over the wave (FP64) and
-// added to the wave's FP64 LDS accumulators of the bra side (accA, list index i) and
-// the ket side (accB, list index j); [2*li + 0] coefficient, [2*li + 1] exponent.
-// Bra sums run over j in registers and are wave-reduced once per i; ket sums run over i
-// in the lane's own LDS slots kac[(2*j + c) * 64 + lane] and are wave-reduced once per j
-// after the i loop (both: T-precision sums of at most K terms per lane, FP64 across lanes).
+// Pullback of one shell pair.  Every per-pair contribution is first summed per lane in T
+// into the lane's own LDS rows (stride LROW, odd, so the transposed read below is free of
+// bank conflicts): bra rows 2*i + c for list entry i (c = 0 coefficient, 1 exponent), ket
+// rows 2*nla + 2*j + c.  Bra sums run over j in registers, ket sums over i in the rows.
+// After the shell pair, lane v sums row v over the 64 lanes in FP64 (a transposed
+// reduction: one pass for all rows instead of one cross-lane reduction per row) and adds
+// it to the wave's FP64 accumulators accA[2*i + c] / accB[2*j + c].
+constexpr int LROW = OVL_TILE + 1;
+
 template <typename T, int LA, int LB, typename G>
 __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restrict__ recs, int nra, int nla, int nrb,
                                          int nlb, int K, double R2, const T (&ab)[3], const G &g, double *accA,
-                                         double *accB, T *kac, int lane)
+                                         double *accB, T *lac, int lane)
 {
     using R = typename RecOf<T>::type;
-    for (int j = 0; j < nrb; ++j) {
-        kac[2 * j * OVL_TILE + lane] = T(0);
-        kac[(2 * j + 1) * OVL_TILE + lane] = T(0);
-    }
+    const int nrow = 2 * (nla + nlb);
+    T *ket = lac + 2 * nla * LROW;
+    for (int v = 0; v < nrow; ++v) lac[v * LROW + lane] = T(0);
     // real x real: all four cotangents
     for (int i = 0; i < nra; ++i) {
         T cA = T(0), aA = T(0);
@@ -458,23 +460,11 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             const T eH = e * H;
             cA += r.w1 * eH;
             aA -= r.Kab * e * Ha;
-            kac[2 * j * OVL_TILE + lane] += r.w2 * eH;
-            kac[(2 * j + 1) * OVL_TILE + lane] -= r.Kab * e * Hb;
+            ket[2 * j * LROW + lane] += r.w2 * eH;
+            ket[(2 * j + 1) * LROW + lane] -= r.Kab * e * Hb;
         }
-        const double sc = wave_sum(static_cast<double>(cA));
-        const double sa = wave_sum(static_cast<double>(aA));
-        if (lane == 0) {
-            accA[2 * i] += sc;
-            accA[2 * i + 1] += sa;
-        }
-    }
-    for (int j = 0; j < nrb; ++j) {
-        const double cB = wave_sum(static_cast<double>(kac[2 * j * OVL_TILE + lane]));
-        const double aB = wave_sum(static_cast<double>(kac[(2 * j + 1) * OVL_TILE + lane]));
-        if (lane == 0) {
-            accB[2 * j] += cB;
-            accB[2 * j + 1] += aB;
-        }
+        lac[2 * i * LROW + lane] = cA;
+        lac[(2 * i + 1) * LROW + lane] = aA;
     }
     // padded bra representative x real ket: coefficient cotangent of the bra slot only
     for (int i = nra; i < nla; ++i) {
@@ -485,8 +475,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             pair_contract<T, LA, LB, false>(r, ab, g, H, Ha, Hb);
             cA += r.w1 * pair_exp(r, R2) * H;
         }
-        const double sc = wave_sum(static_cast<double>(cA));
-        if (lane == 0) accA[2 * i] += sc;
+        lac[2 * i * LROW + lane] = cA;
     }
     // real bra x padded ket representative: coefficient cotangent of the ket slot only
     for (int j = nrb; j < nlb; ++j) {
@@ -497,9 +486,20 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             pair_contract<T, LA, LB, false>(r, ab, g, H, Ha, Hb);
             cB += r.w2 * pair_exp(r, R2) * H;
         }
-        const double sc = wave_sum(static_cast<double>(cB));
-        if (lane == 0) accB[2 * j] += sc;
+        ket[2 * j * LROW + lane] = cB;
     }
+    __syncthreads();
+    for (int v = lane; v < nrow; v += OVL_TILE) {
+        const T *row = lac + v * LROW;
+        double sum = 0.0;
+#pragma unroll 8
+        for (int l = 0; l < OVL_TILE; ++l) sum += static_cast<double>(row[l]);
+        if (v < 2 * nla)
+            accA[v] += sum;
+        else
+            accB[v - 2 * nla] += sum;
+    }
+    __syncthreads();
 }
 
 // Position derivative of one shell pair for the lane's b, accumulated in FP64:
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     constexpr bool DIRECT = bwd_direct<T, LMAXB>();
     double *accA = reinterpret_cast<double *>(smem_raw + 256);  // [sa - sa0][K][2], then the ket side at ns*K*2
     T *stripe = reinterpret_cast<T *>(accA + pstride);
-    T *kac = DIRECT ? stripe : stripe + OVL_TILE * sh.str;  // per-lane ket sums [2 * K][64]
+    T *lac = DIRECT ? stripe : stripe + OVL_TILE * sh.str;  // per-lane sums [4 * K][LROW]
     TileInfo ti;
     const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
@@ -730,10 +730,10 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
                 _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
                     _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
                         gr.v[i + ShellC<A>::nbf * j] = active ? src[i + static_cast<int64_t>(N) * j] : T(0); \
-                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, kac, lane);    \
+                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
             } else {                                                                               \
                 pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, GStripe<T>{gsl, nrows}, aA, aB, \
-                                   kac, lane);                                                     \
+                                   lac, lane);                                                     \
             }                                                                                      \
             continue;                                                                              \
         }
@@ -1254,7 +1254,7 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
 {
     using R = typename RecOf<T>::type;
     const size_t shm = 256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 +
-                       sizeof(T) * OVL_TILE * ((bwd_direct<T, LMAXB>() ? 0 : sh.str) + 2 * sh.K);
+                       sizeof(T) * ((bwd_direct<T, LMAXB>() ? 0 : OVL_TILE * sh.str) + 4 * sh.K * LROW);
     auto kern = &ovl2c_bwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
