@@ -10,6 +10,7 @@ module AOKHip
 
 using AMDGPU: ROCArray, ROCVector, ROCMatrix, stream
 using StaticArrays: SVector
+import Unitful
 import AtomicOrbitalKernels as AOK
 import AtomicOrbitalKernels: evaluate, evaluate_ed, batch_overlap!, pullback_ps
 
@@ -196,6 +197,69 @@ function batch_overlap_3c!(out::ROCArray{FT, 4}, ls::Vector{Int}, coef::ROCArray
                AOK_I64, B, ns, K, NZ, ls, pointer(coef), pointer(α), 1.0, pointer(out))
     check(rc, "batch_overlap_3c!")
     return out
+end
+
+# ---- H2 geometry-in 2-centre: batch_overlap!(out, ::CompiledBasis, posA, posB) ------------
+# (kernels_2c.jl:158-168): units stripped on the host by the reference's own to_bohr; the
+# shells become one species whose primitive slots are zero-padded to the longest contraction.
+function _padded(basis::AOK.CompiledBasis, FT)
+    K = maximum(basis.nprim)
+    coef = zeros(FT, basis.nshells, K, 1); α = ones(FT, basis.nshells, K, 1)
+    for s in 1:basis.nshells, k in 1:basis.nprim[s]
+        coef[s, k, 1] = basis.coef[basis.prim_offset[s] + k - 1]
+        α[s, k, 1] = basis.α[basis.prim_offset[s] + k - 1]
+    end
+    return ROCArray(coef), ROCArray(α)
+end
+
+function batch_overlap!(out::ROCArray{FT, 3}, basis::AOK.CompiledBasis,
+                        posA::AbstractMatrix{<:Unitful.Length}, posB::AbstractMatrix{<:Unitful.Length}) where {FT}
+    A = ROCArray(AOK.to_bohr(posA, FT)); Bm = ROCArray(AOK.to_bohr(posB, FT))
+    coef, α = _padded(basis, FT)
+    ns, K, _ = size(coef)
+    rc = ccall((:aok_ovl2c, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Int64, Int64,
+                Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(A), pointer(Bm), C_NULL, C_NULL, AOK_I64, size(posA, 2), ns, K, 1,
+               _ip(Vector{Int}(Array(basis.ls))), pointer(coef), pointer(α), 1.0, pointer(out))
+    check(rc, "batch_overlap!")
+    return out
+end
+
+# ---- no reference counterpart: position cotangents of the 2-centre overlap ---------------
+# (dA, dB) [3 x B] per unit of the basis' length unit, dB = -dA.
+function overlap_pullback_x(∂S::ROCArray{FT, 3}, basis::AOK.CartesianGTOBasis, XA::AbstractVector,
+                            XB::AbstractVector) where {FT}
+    B = length(XA)
+    posA = ROCArray(AOK._cgto_posmat(XA, FT)); posB = ROCArray(AOK._cgto_posmat(XB, FT))
+    sA = ROCArray(AOK._cgto_sidx(basis, XA)); sB = ROCArray(AOK._cgto_sidx(basis, XB))
+    coef = ROCArray(FT.(basis.coef)); α = ROCArray(FT.(basis.ζ))
+    ns, K, NZ = size(basis.coef)
+    dA = similar(posA); dB = similar(posB)
+    rc = ccall((:aok_ovl2c_pullback_x, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64,
+                Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(∂S), pointer(posA), pointer(posB), pointer(sA), pointer(sB),
+               AOK_I64, B, ns, K, NZ, _ip(basis.ls), pointer(coef), pointer(α), Float64(basis.lengthscale),
+               pointer(dA), pointer(dB))
+    check(rc, "overlap_pullback_x")
+    return dA, dB
+end
+
+# ---- no reference counterpart: (coef, α) cotangents of the 3-centre overlap ---------------
+function overlap_3c_pullback(∂T::ROCArray{FT, 4}, ls::Vector{Int}, coef::ROCArray{FT, 3}, α::ROCArray{FT, 3},
+                             posA::ROCMatrix{FT}, posB::ROCMatrix{FT}, posC::ROCMatrix{FT}) where {FT}
+    ns, K, NZ = size(coef)
+    ∂c = similar(coef); ∂α = similar(α)
+    rc = ccall((:aok_ovl3c_pullback, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                Ptr{Cvoid}, Cint, Int64, Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid},
+                Ptr{Cvoid}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(∂T), pointer(posA), pointer(posB), pointer(posC), C_NULL, C_NULL,
+               C_NULL, AOK_I64, size(posA, 2), ns, K, NZ, ls, pointer(coef), pointer(α), 1.0, pointer(∂c),
+               pointer(∂α))
+    check(rc, "overlap_3c_pullback")
+    return ∂c, ∂α
 end
 
 end # module
