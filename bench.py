@@ -29,7 +29,7 @@ FP32_PEAK_TFLOPS = 157.3
 # runs of this bench (KiB per dispatch, mean over the launches), each scaled by what the
 # same counter reads for a 2 GiB device copy in the same runs (FETCH_SIZE reads
 # 1048589 KiB for 2097152 KiB moved: x2.0; WRITE_SIZE reads 2097152 KiB: x1.0).
-PMC_KIB = {"ovl2c_bwd_fast": (246506.0, 10291.2), "ovl2c_fwd_fast": (21362.2, 115992.0)}
+PMC_KIB = {"ovl2c_bwd_fast": (238415.0, 10291.2), "ovl2c_fwd_fast": (21408.6, 115991.0)}
 PMC_SCALE = (2.0, 1.0)
 # FLOPs per pair the C1 kernels execute: rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32 and
 # SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes), per launch / 65536 pairs.
@@ -37,8 +37,8 @@ PMC_SCALE = (2.0, 1.0)
 # (McMurchie-Davidson) count of the reference algorithm (algorithmic_flops below), so the
 # roofline prices these executed FLOPs; the Hermite-count rate is reported beside it.
 # VALU busy: SQ_ACTIVE_INST_VALU * 4 / (1024 SIMDs * GRBM_GUI_ACTIVE / 8 XCDs).
-EXEC_FLOP_PER_PAIR = {"ovl2c_bwd_fast": (3.915e9 + 4.670e8) / 65536, "ovl2c_fwd_fast": (5.112e8 + 1.287e8) / 65536}
-VALU_BUSY = {"ovl2c_bwd_fast": 0.80, "ovl2c_fwd_fast": 0.41}
+EXEC_FLOP_PER_PAIR = {"ovl2c_bwd_fast": (3.532e9 + 3.224e8) / 65536, "ovl2c_fwd_fast": (4.766e8 + 1.287e8) / 65536}
+VALU_BUSY = {"ovl2c_bwd_fast": 0.68, "ovl2c_fwd_fast": 0.41}
 METRIC = "2-ctr overlap pairs/s + (ζ,D)-grad pairs/s, FP32@1e-6 vs plain FP64; AO GB/s"
 
 
