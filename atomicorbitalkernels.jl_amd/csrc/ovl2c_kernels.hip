@@ -595,8 +595,11 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ nreal, T *__restrict__ out)
 {
     extern __shared__ unsigned char smem_raw[];
-    int *bl_b = reinterpret_cast<int *>(smem_raw);
-    T *stripe = reinterpret_cast<T *>(smem_raw + 256);
+    // LDS: the lanes' block offsets b * N * N, the column-segment offsets of the chunk's
+    // stripe elements, then the stripe itself (fwd_lds_bytes)
+    int64_t *bl_off = reinterpret_cast<int64_t *>(smem_raw);
+    int *q_off = reinterpret_cast<int *>(smem_raw + 8 * OVL_TILE);
+    T *stripe = reinterpret_cast<T *>(smem_raw + 8 * OVL_TILE + 4 * ((sh.str + 3) & ~3));
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     TileInfo ti;
     const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
@@ -604,7 +607,7 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
     const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
-    bl_b[lane] = b;
+    bl_off[lane] = static_cast<int64_t>(b) * N * N;
     const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
     const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
     const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);
@@ -613,11 +616,13 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     const double R2 = dx * dx + dy * dy + dz * dz;
     const T ab[3] = {static_cast<T>(dx), static_cast<T>(dy), static_cast<T>(dz)};
     T *mine = stripe + lane * sh.str;
-    const int64_t NN = static_cast<int64_t>(N) * N;
     {
         const OvlChunk ch = unpack_chunk(sh.ch[chunk]);
         const int sb = ch.sb, lb = sh.ls[sb];
         const int nrb = nreal[sgB * ns + sb];
+        const int nbb = nbf_of(lb), nrows = ch.nrows;
+        const int len = nrows * nbb;
+        for (int q = lane; q < len; q += OVL_TILE) q_off[q] = N * (q / nrows) + q % nrows;
         for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
             const int la = sh.ls[sa];
             const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
@@ -634,17 +639,20 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
 #undef AOK_FWD_CASE
         }
         __syncthreads();
-        const int nbb = nbf_of(lb), nrows = ch.nrows;
-        const int len = nrows * nbb, total = ti.nact * len;
-        const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
-        const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
+        // coalesced write-out: element e = bl * len + q of the stripe goes to pair bl's block
+        // at column-segment offset q_off[q]; (bl, q) advance by 64 elements per step
+        const int total = ti.nact * len;
+        T *obase = out + static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
+        const int dbl = OVL_TILE / len, dq = OVL_TILE - dbl * len;
+        int bl = lane / len, q = lane - bl * len;
         for (int e = lane; e < total; e += OVL_TILE) {
-            const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
-            const int q = e - bl * len;
-            const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
-            const int i = q - j * nrows;
-            out[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i] =
-                stripe[bl * sh.str + q];
+            obase[bl_off[bl] + q_off[q]] = stripe[bl * sh.str + q];
+            q += dq;
+            bl += dbl;
+            if (q >= len) {
+                q -= len;
+                ++bl;
+            }
         }
     }
 }
@@ -1232,7 +1240,7 @@ template <typename T, int LMAXB>
 static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
     using R = typename RecOf<T>::type;
-    const size_t shm = 256 + sizeof(T) * OVL_TILE * sh.str;
+    const size_t shm = 8 * OVL_TILE + 4 * ((sh.str + 3) & ~3) + sizeof(T) * OVL_TILE * sh.str;
     auto kern = &ovl2c_fwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
