@@ -506,9 +506,9 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
 // acc += sum over real primitive pairs of G * sum_lin g[lin] * d/dA (s_x s_y s_z), with
 // d/dA_x [x_A^i e^{-alpha x_A^2}] = 2 alpha x_A^{i+1} e - i x_A^{i-1} e, i.e. the 1-D
 // tables raised by one on the bra: d_x(i, j) = 2 alpha s_x(i+1, j) - i s_x(i-1, j).
-template <typename T, int LA, int LB>
+template <typename T, int LA, int LB, typename GAcc>
 __device__ __forceinline__ void pair_dx(const typename RecOf<T>::type *__restrict__ recs, int nra, int nrb, int K,
-                                        double R2, const T (&ab)[3], const T *g, int nrows, double (&acc)[3])
+                                        double R2, const T (&ab)[3], const GAcc &g, double (&acc)[3])
 {
     constexpr int NBA = ShellC<LA>::nbf, N1 = 2 * LA + 1;
     for (int i = 0; i < nra; ++i)
@@ -534,7 +534,7 @@ __device__ __forceinline__ void pair_dx(const typename RecOf<T>::type *__restric
                         for (int n2 = 0; n2 <= LB - ll2; ++n2) {
                             const int m2 = LB - ll2 - n2;
                             const int lin = index1 + N1 * index2;
-                            const T gv = g[lin % NBA + nrows * (lin / NBA)];
+                            const T gv = g(lin % NBA, lin / NBA);
                             const T x0 = sx[ll1][ll2], y0 = sy[m1][m2], z0 = sz[n1][n2];
                             const T dxv = a2 * sx[ll1 + 1][ll2] - (ll1 > 0 ? T(ll1) * sx[ll1 > 0 ? ll1 - 1 : 0][ll2] : T(0));
                             const T dyv = a2 * sy[m1 + 1][m2] - (m1 > 0 ? T(m1) * sy[m1 > 0 ? m1 - 1 : 0][m2] : T(0));
@@ -563,6 +563,39 @@ __host__ __device__ constexpr bool bwd_direct()
 #define AOK_LL_CASES(X)                                                                                       \
     X(0, 0) X(0, 1) X(0, 2) X(0, 3) X(0, 4) X(1, 0) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(2, 0) X(2, 1) X(2, 2) \
         X(2, 3) X(2, 4) X(3, 0) X(3, 1) X(3, 2) X(3, 3) X(3, 4) X(4, 0) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
+
+// Walks the elements e = lane, lane + 64, ... of a tile's chunk stripe (64 pairs x len
+// elements, len = nrows * nbb): e = bl * len + j * nrows + i, advanced by counters instead
+// of divisions.
+struct StripeWalk {
+    int bl, j, i, dbl, dj, di;
+    __device__ __forceinline__ StripeWalk(int lane, int nrows, int nbb)
+    {
+        const int len = nrows * nbb;
+        bl = lane / len;
+        const int q = lane - bl * len;
+        j = q / nrows;
+        i = q - j * nrows;
+        dbl = OVL_TILE / len;
+        const int dq = OVL_TILE - dbl * len;
+        dj = dq / nrows;
+        di = dq - dj * nrows;
+    }
+    __device__ __forceinline__ void next(int nrows, int nbb)
+    {
+        i += di;
+        j += dj;
+        bl += dbl;
+        if (i >= nrows) {
+            i -= nrows;
+            ++j;
+        }
+        if (j >= nbb) {
+            j -= nbb;
+            ++bl;
+        }
+    }
+};
 
 // ---------------------------------------------------------------------------------
 // Tile bookkeeping shared by the fast kernels
@@ -703,19 +736,13 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     const int sb = ch.sb, lb = sh.ls[sb];
     const int nbb = nbf_of(lb), nrows = ch.nrows;
     const int len = nrows * nbb, total = OVL_TILE * len;
-    const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
     const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
     // stage the cotangent stripe (zeros for inactive lanes)
     if constexpr (!DIRECT) {
-        for (int e = lane; e < total; e += OVL_TILE) {
-            const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
-            const int q = e - bl * len;
-            const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
-            const int i = q - j * nrows;
-            stripe[bl * sh.str + q] =
-                bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i]
-                             : T(0);
-        }
+        StripeWalk w(lane, nrows, nbb);
+        for (int e = lane; e < total; e += OVL_TILE, w.next(nrows, nbb))
+            stripe[w.bl * sh.str + w.j * nrows + w.i] =
+                w.bl < ti.nact ? dS[static_cast<int64_t>(bl_b[w.bl]) * NN + colbase + N * w.j + w.i] : T(0);
     }
     __syncthreads();
     const bool active = lane < ti.nact;
@@ -772,6 +799,8 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     int *bl_b = reinterpret_cast<int *>(smem_raw);
     T *stripe = reinterpret_cast<T *>(smem_raw + 256);
+    // as the pullback: FP32 with shells up to d reads each shell pair's block into registers
+    constexpr bool DIRECT = bwd_direct<T, LMAXB>();
     TileInfo ti;
     const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
@@ -793,17 +822,16 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
     const int sb = ch.sb, lb = sh.ls[sb];
     const int nbb = nbf_of(lb), nrows = ch.nrows;
     const int len = nrows * nbb, total = OVL_TILE * len;
-    const float inv_len = 1.0f / static_cast<float>(len), inv_rows = 1.0f / static_cast<float>(nrows);
     const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
-    for (int e = lane; e < total; e += OVL_TILE) {
-        const int bl = static_cast<int>((static_cast<float>(e) + 0.5f) * inv_len);
-        const int q = e - bl * len;
-        const int j = static_cast<int>((static_cast<float>(q) + 0.5f) * inv_rows);
-        const int i = q - j * nrows;
-        stripe[bl * sh.str + q] =
-            bl < ti.nact ? dS[static_cast<int64_t>(bl_b[bl]) * NN + colbase + static_cast<int64_t>(N) * j + i] : T(0);
+    if constexpr (!DIRECT) {
+        StripeWalk w(lane, nrows, nbb);
+        for (int e = lane; e < total; e += OVL_TILE, w.next(nrows, nbb))
+            stripe[w.bl * sh.str + w.j * nrows + w.i] =
+                w.bl < ti.nact ? dS[static_cast<int64_t>(bl_b[w.bl]) * NN + colbase + N * w.j + w.i] : T(0);
     }
     __syncthreads();
+    const bool active = lane < ti.nact;
+    const T *gblk = dS + static_cast<int64_t>(b) * NN + colbase - ch.row0;  // + off[sa] + i + N * j
     const int nrb = nreal[sgB * ns + sb];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
@@ -812,11 +840,20 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
         const typename RecOf<T>::type *recs = rec + g * K * K;
         const int nra = nreal[sgA * ns + sa];
         const T *gsl = mine + (sh.off[sa] - ch.row0);
-#define AOK_DX_CASE(A, B_)                                                       \
-    if constexpr (A <= LMAXB && B_ <= LMAXB)                                     \
-        if (la == A && lb == B_) {                                               \
-            pair_dx<T, A, B_>(recs, nra, nrb, K, R2, ab, gsl, nrows, acc);       \
-            continue;                                                            \
+#define AOK_DX_CASE(A, B_)                                                                          \
+    if constexpr (A <= LMAXB && B_ <= LMAXB)                                                        \
+        if (la == A && lb == B_) {                                                                  \
+            if constexpr (DIRECT) {                                                                 \
+                GRegs<T, ShellC<A>::nbf, ShellC<B_>::nbf> gr;                                       \
+                const T *src = gblk + sh.off[sa];                                                   \
+                _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                         \
+                    _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                      \
+                        gr.v[i + ShellC<A>::nbf * j] = active ? src[i + static_cast<int64_t>(N) * j] : T(0); \
+                pair_dx<T, A, B_>(recs, nra, nrb, K, R2, ab, gr, acc);                              \
+            } else {                                                                                \
+                pair_dx<T, A, B_>(recs, nra, nrb, K, R2, ab, GStripe<T>{gsl, nrows}, acc);          \
+            }                                                                                       \
+            continue;                                                                               \
         }
         AOK_LL_CASES(AOK_DX_CASE)
 #undef AOK_DX_CASE
@@ -1377,7 +1414,7 @@ static aok_status fast_dx_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape
                                  double *partial)
 {
     using R = typename RecOf<T>::type;
-    const size_t shm = 256 + sizeof(T) * OVL_TILE * sh.str;
+    const size_t shm = 256 + (bwd_direct<T, LMAXB>() ? 0 : sizeof(T) * OVL_TILE * sh.str);
     auto kern = &ovl2c_dx_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
