@@ -308,7 +308,7 @@ __device__ __forceinline__ void os1d(T PA, T PB, T i2p, T (&s)[IA + 1][IB + 1])
 
 // Unroll factors of the primitive-pair loops (launch-shape tuning, fixed at build time).
 #ifndef AOK_FWD_UNROLL
-#define AOK_FWD_UNROLL 1
+#define AOK_FWD_UNROLL 4
 #endif
 #ifndef AOK_PB_UNROLL
 #define AOK_PB_UNROLL 1
