@@ -392,9 +392,19 @@ struct GRegs {
 };
 
 template <typename T, int LA, int LB, bool RAISED, typename G>
-__device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, const T (&ab)[3], const G &g, T &H,
-                                              T &Ha, T &Hb)
+__device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, const T (&ab)[3], T r2, const G &g,
+                                              T &H, T &Ha, T &Hb)
 {
+    if constexpr (LA == 0 && LB == 0) {
+        // s-s: s_x s_y s_z = 1 and the raised sums close over the axes,
+        // sum_ax s_ax(2, 0) = sum_ax (PA_ax^2 + 1/(2p)) = (beta/p)^2 |AB|^2 + 3/(2p), likewise
+        // the ket with alpha/p
+        const T gv = g(0, 0);
+        H = gv;
+        Ha = RAISED ? gv * (r.bop * r.bop * r2 + T(3) * r.i2p) : T(0);
+        Hb = RAISED ? gv * (r.aop * r.aop * r2 + T(3) * r.i2p) : T(0);
+        return;
+    }
     constexpr int NBA = ShellC<LA>::nbf, N1 = 2 * LA + 1;
     constexpr int RA = RAISED ? LA + 2 : LA, RB = RAISED ? LB + 2 : LB;
     T sx[RA + 1][RB + 1], sy[RA + 1][RB + 1], sz[RA + 1][RB + 1];
@@ -445,6 +455,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
                                          double *accB, T *lac, int lane)
 {
     using R = typename RecOf<T>::type;
+    const T r2f = static_cast<T>(R2);
     const int nrow = 2 * (nla + nlb);
     T *ket = lac + 2 * nla * LROW;
     for (int v = 0; v < nrow; ++v) lac[v * LROW + lane] = T(0);
@@ -456,7 +467,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             const R r = recs[i * K + j];
             const T e = pair_exp(r, R2);
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, true>(r, ab, g, H, Ha, Hb);
+            pair_contract<T, LA, LB, true>(r, ab, r2f, g, H, Ha, Hb);
             const T eH = e * H;
             cA += r.w1 * eH;
             aA -= r.Kab * e * Ha;
@@ -472,7 +483,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
         for (int j = 0; j < nrb; ++j) {
             const R r = recs[i * K + j];
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, false>(r, ab, g, H, Ha, Hb);
+            pair_contract<T, LA, LB, false>(r, ab, r2f, g, H, Ha, Hb);
             cA += r.w1 * pair_exp(r, R2) * H;
         }
         lac[2 * i * LROW + lane] = cA;
@@ -483,7 +494,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
         for (int i = 0; i < nra; ++i) {
             const R r = recs[i * K + j];
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, false>(r, ab, g, H, Ha, Hb);
+            pair_contract<T, LA, LB, false>(r, ab, r2f, g, H, Ha, Hb);
             cB += r.w2 * pair_exp(r, R2) * H;
         }
         ket[2 * j * LROW + lane] = cB;
