@@ -636,7 +636,7 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ boff, const int *__restrict__ bcnt,
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
-                                                     const int *__restrict__ nreal, T *__restrict__ out)
+                                                     const int *__restrict__ nreal, T *__restrict__ out, int chunk0)
 {
     extern __shared__ unsigned char smem_raw[];
     // LDS: the lanes' block offsets b * N * N, the column-segment offsets of the chunk's
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     T *stripe = reinterpret_cast<T *>(smem_raw + 8 * OVL_TILE + 4 * ((sh.str + 3) & ~3));
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     TileInfo ti;
-    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
+    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y) + chunk0;
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
                                                      const int *__restrict__ nreal, const int *__restrict__ nlist,
-                                                     double *__restrict__ partial)
+                                                     double *__restrict__ partial, int chunk0)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -726,7 +726,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     T *stripe = reinterpret_cast<T *>(accA + pstride);
     T *lac = DIRECT ? stripe : stripe + OVL_TILE * sh.str;  // per-lane sums [4 * K][LROW]
     TileInfo ti;
-    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
+    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y) + chunk0;
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     for (int q = lane; q < pstride; q += OVL_TILE) accA[q] = 0.0;
@@ -804,7 +804,8 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
                                                     const int *__restrict__ boff, const int *__restrict__ bcnt,
                                                     const int *__restrict__ toff,
                                                     const typename RecOf<T>::type *__restrict__ rec,
-                                                    const int *__restrict__ nreal, double *__restrict__ partial)
+                                                    const int *__restrict__ nreal, double *__restrict__ partial,
+                                                    int chunk0)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -813,7 +814,7 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
     // as the pullback: FP32 with shells up to d reads each shell pair's block into registers
     constexpr bool DIRECT = bwd_direct<T, LMAXB>();
     TileInfo ti;
-    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y);
+    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y) + chunk0;
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
@@ -1285,29 +1286,31 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
 }
 
 template <typename T, int LMAXB>
-static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
+static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws,
+                                  int chunk0, int nch)
 {
+    if (nch <= 0) return AOK_OK;
     using R = typename RecOf<T>::type;
     const size_t shm = 8 * OVL_TILE + 4 * ((sh.str + 3) & ~3) + sizeof(T) * OVL_TILE * sh.str;
     auto kern = &ovl2c_fwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    time_begin(ctx, "ovl2c_fwd_fast");
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
-                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out));
-    time_end(ctx);
+                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), chunk0);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
 
 template <typename T, int LMAXB>
-static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
+static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws,
+                                  int chunk0, int nch)
 {
+    if (nch <= 0) return AOK_OK;
     using R = typename RecOf<T>::type;
     const size_t shm = 256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 +
                        sizeof(T) * ((bwd_direct<T, LMAXB>() ? 0 : OVL_TILE * sh.str) + 4 * sh.K * LROW);
@@ -1315,17 +1318,20 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    time_begin(ctx, "ovl2c_bwd_fast");
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.dS),
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.nlist),
-                       reinterpret_cast<double *>(ws + w.partial));
-    time_end(ctx);
+                       reinterpret_cast<double *>(ws + w.partial), chunk0);
     AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
+}
+
+static aok_status bwd_reduce_launch(aok_ctx ctx, const OvlShape &sh, const Ovl2cWs &w, char *ws)
+{
     const int nslot = sh.NZ * sh.ns * sh.K;
     hipLaunchKernelGGL(ovl2c_bwd_reduce, dim3(nslot, 2 * sh.NZ), dim3(256), 0, ctx->stream, sh,
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const int *>(ws + w.pcls),
@@ -1410,10 +1416,13 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
 }
 
 // Shape of a call: validation and the stripe plan.
-static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh)
+// Stripe chunks of one call.  The pullback and the position cotangents run their light
+// chunks in the l <= OVL_LIGHT_L instance (their wide instances hold one or two waves per
+// SIMD); the forward keeps the fewer, longer chunks of the unsplit plan in one instance.
+static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh, bool split_light)
 {
     const int cap = ctx->tune.ovl_cap > 0 ? ctx->tune.ovl_cap : OVL_STRIPE_CAP;
-    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, cap, sh);
+    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, cap, sh, split_light);
     if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
     if (c.B < 0) return fail(ctx, AOK_ERR_ARG, "B must be >= 0");
     if (c.B > (int64_t(1) << 30)) return fail(ctx, AOK_ERR_ARG, "B too large");
@@ -1422,30 +1431,63 @@ static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh)
 
 template <typename T, int LMAXB>
 static aok_status fast_dx_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws,
-                                 double *partial)
+                                 double *partial, int chunk0, int nch)
 {
+    if (nch <= 0) return AOK_OK;
     using R = typename RecOf<T>::type;
     const size_t shm = 256 + (bwd_direct<T, LMAXB>() ? 0 : sizeof(T) * OVL_TILE * sh.str);
     auto kern = &ovl2c_dx_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    time_begin(ctx, "ovl2c_dx_fast");
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles, sh.nchunk), dim3(OVL_TILE), shm, ctx->stream, sh, c.B,
+    hipLaunchKernelGGL(kern, dim3(w.maxtiles, nch), dim3(OVL_TILE), shm, ctx->stream, sh, c.B,
                        static_cast<const T *>(c.dS), static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
                        static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
                        reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
                        reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
                        reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int *>(ws + w.nreal),
-                       partial);
-    time_end(ctx);
-    AOK_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL((ovl2c_dx_reduce<T>), dim3(static_cast<unsigned>((c.B + 255) / 256)), dim3(256), 0,
-                       ctx->stream, c.B, sh.nchunk, reinterpret_cast<const int *>(ws + w.perm), partial, c.scale,
-                       static_cast<T *>(c.dA), static_cast<T *>(c.dB));
+                       partial, chunk0);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
+
+// The light chunks in the l <= OVL_LIGHT_L instance (with their own stripe length), the
+// rest in the instance for the basis' largest l.  LAUNCH<T, L>::run(ctx, c, shape, w, ws,
+// x..., chunk0, nch) launches one instance over chunks chunk0 .. chunk0 + nch - 1.
+template <typename T, template <typename, int> class LAUNCH, typename... X>
+static aok_status launch_split(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws,
+                               X... x)
+{
+    const int lmax = lmax_of(c.ls, c.ns);
+    if (lmax == 0) return LAUNCH<T, 0>::run(ctx, c, sh, w, ws, x..., 0, sh.nchunk);
+    if (lmax == 1) return LAUNCH<T, 1>::run(ctx, c, sh, w, ws, x..., 0, sh.nchunk);
+    if (lmax == 2) return LAUNCH<T, 2>::run(ctx, c, sh, w, ws, x..., 0, sh.nchunk);
+    if (sh.nlight > 0) {
+        OvlShape shl = sh;
+        shl.str = sh.strl;
+        const aok_status st = LAUNCH<T, OVL_LIGHT_L>::run(ctx, c, shl, w, ws, x..., 0, sh.nlight);
+        if (st != AOK_OK) return st;
+    }
+    const int nh = sh.nchunk - sh.nlight;
+    if (lmax == 3) return LAUNCH<T, 3>::run(ctx, c, sh, w, ws, x..., sh.nlight, nh);
+    return LAUNCH<T, 4>::run(ctx, c, sh, w, ws, x..., sh.nlight, nh);
+}
+
+template <typename T, int L>
+struct FwdL {
+    template <typename... A>
+    static aok_status run(A &&...a) { return fast_fwd_launch<T, L>(a...); }
+};
+template <typename T, int L>
+struct BwdL {
+    template <typename... A>
+    static aok_status run(A &&...a) { return fast_bwd_launch<T, L>(a...); }
+};
+template <typename T, int L>
+struct DxL {
+    template <typename... A>
+    static aok_status run(A &&...a) { return fast_dx_launch<T, L>(a...); }
+};
 
 // Position cotangents on a prepared workspace; `partial` holds nchunk * B * 3 doubles.
 template <typename T>
@@ -1453,13 +1495,15 @@ static aok_status fast_dx(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, c
                           double *partial)
 {
     if (c.B == 0) return AOK_OK;
-    switch (lmax_of(c.ls, c.ns)) {
-        case 0: return fast_dx_launch<T, 0>(ctx, c, sh, w, ws, partial);
-        case 1: return fast_dx_launch<T, 1>(ctx, c, sh, w, ws, partial);
-        case 2: return fast_dx_launch<T, 2>(ctx, c, sh, w, ws, partial);
-        case 3: return fast_dx_launch<T, 3>(ctx, c, sh, w, ws, partial);
-        default: return fast_dx_launch<T, 4>(ctx, c, sh, w, ws, partial);
-    }
+    time_begin(ctx, "ovl2c_dx_fast");
+    aok_status st = launch_split<T, DxL>(ctx, c, sh, w, ws, partial);
+    time_end(ctx);
+    if (st != AOK_OK) return st;
+    hipLaunchKernelGGL((ovl2c_dx_reduce<T>), dim3(static_cast<unsigned>((c.B + 255) / 256)), dim3(256), 0,
+                       ctx->stream, c.B, sh.nchunk, reinterpret_cast<const int *>(ws + w.perm), partial, c.scale,
+                       static_cast<T *>(c.dA), static_cast<T *>(c.dB));
+    AOK_HIP(ctx, hipGetLastError());
+    return AOK_OK;
 }
 
 static size_t dx_partial_bytes(const OvlShape &sh, int64_t B)
@@ -1471,13 +1515,10 @@ template <typename T>
 static aok_status fast_forward(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
 {
     if (c.B == 0) return AOK_OK;
-    switch (lmax_of(c.ls, c.ns)) {
-        case 0: return fast_fwd_launch<T, 0>(ctx, c, sh, w, ws);
-        case 1: return fast_fwd_launch<T, 1>(ctx, c, sh, w, ws);
-        case 2: return fast_fwd_launch<T, 2>(ctx, c, sh, w, ws);
-        case 3: return fast_fwd_launch<T, 3>(ctx, c, sh, w, ws);
-        default: return fast_fwd_launch<T, 4>(ctx, c, sh, w, ws);
-    }
+    time_begin(ctx, "ovl2c_fwd_fast");
+    const aok_status st = launch_split<T, FwdL>(ctx, c, sh, w, ws);
+    time_end(ctx);
+    return st;
 }
 
 template <typename T>
@@ -1487,14 +1528,11 @@ static aok_status fast_pullback(aok_ctx ctx, const Ovl2cCall &c, const OvlShape 
     const int64_t npar = static_cast<int64_t>(sh.NZ) * sh.ns * sh.K;
     if (c.B == 0)  // no pairs: zero cotangents (sums over no parts)
         return out_stage<T>(ctx, c, sh, nullptr, nullptr, 0, npar);
-    switch (lmax_of(c.ls, c.ns)) {
-        case 0: st = fast_bwd_launch<T, 0>(ctx, c, sh, w, ws); break;
-        case 1: st = fast_bwd_launch<T, 1>(ctx, c, sh, w, ws); break;
-        case 2: st = fast_bwd_launch<T, 2>(ctx, c, sh, w, ws); break;
-        case 3: st = fast_bwd_launch<T, 3>(ctx, c, sh, w, ws); break;
-        default: st = fast_bwd_launch<T, 4>(ctx, c, sh, w, ws); break;
-    }
+    time_begin(ctx, "ovl2c_bwd_fast");
+    st = launch_split<T, BwdL>(ctx, c, sh, w, ws);
+    time_end(ctx);
     if (st != AOK_OK) return st;
+    if ((st = bwd_reduce_launch(ctx, sh, w, ws)) != AOK_OK) return st;
     const double *part = reinterpret_cast<const double *>(ws + w.part2);
     return out_stage<T>(ctx, c, sh, part, part + 2 * sh.NZ * npar, 2 * sh.NZ, npar);
 }
@@ -1516,7 +1554,7 @@ template <typename T>
 aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback)
 {
     OvlShape sh;
-    aok_status st = shape_of(ctx, c, sh);
+    aok_status st = shape_of(ctx, c, sh, pullback);
     if (st != AOK_OK) return st;
     if (c.flags & AOK_PLAIN) return plain_run<T>(ctx, c, sh, pullback);
     const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), pullback);
@@ -1535,7 +1573,8 @@ aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback)
 template <typename T>
 aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p)
 {
-    aok_status st = shape_of(ctx, c, p.sh);
+    aok_status st = shape_of(ctx, c, p.sh, true);
+    if (st == AOK_OK) st = shape_of(ctx, c, p.shf, false);
     if (st != AOK_OK) return st;
     p.call = c;
     if (c.flags & AOK_PLAIN) return AOK_OK;
@@ -1570,7 +1609,7 @@ aok_status ovl2c_plan_exec(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall
     c.o2 = io.o2;
     if (c.flags & AOK_PLAIN) return plain_run<T>(ctx, c, p.sh, pullback);
     char *ws = static_cast<char *>(p.buf);
-    return pullback ? fast_pullback<T>(ctx, c, p.sh, p.w, ws) : fast_forward<T>(ctx, c, p.sh, p.w, ws);
+    return pullback ? fast_pullback<T>(ctx, c, p.sh, p.w, ws) : fast_forward<T>(ctx, c, p.shf, p.w, ws);
 }
 
 template aok_status ovl2c_plan_build<float>(aok_ctx, const Ovl2cCall &, Ovl2cPlanState &);
@@ -1617,7 +1656,7 @@ template <typename T>
 aok_status ovl2c_dx_run(aok_ctx ctx, const Ovl2cCall &c)
 {
     OvlShape sh;
-    aok_status st = shape_of(ctx, c, sh);
+    aok_status st = shape_of(ctx, c, sh, true);
     if (st != AOK_OK) return st;
     if (c.flags & AOK_PLAIN) return fail(ctx, AOK_ERR_ARG, "position cotangents are computed by the fast path only");
     const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), false);

@@ -54,10 +54,16 @@ __host__ __device__ inline OvlChunk unpack_chunk(OvlChunkP p)
                     static_cast<int>(p.r >> 16)};
 }
 
+// Chunks whose shells all have l <= OVL_LIGHT_L come first (nlight of them, stripe length
+// strl) and run in a kernel instance compiled for l <= OVL_LIGHT_L (fewer registers, more
+// waves per SIMD); the others run in the instance for the basis' largest l.  str is the
+// longest stripe of all chunks.
+constexpr int OVL_LIGHT_L = 2;
+
 struct OvlShape {
     int ns, K, NZ, N;
-    int nchunk;
-    int str;  // LDS floats per lane (odd)
+    int nchunk, nlight;
+    int str, strl;  // LDS floats per lane (odd): all chunks, light chunks
     int ls[OVL_MAXSH];
     int off[OVL_MAXSH + 1];
     OvlChunkP ch[OVL_MAXCH];
@@ -89,7 +95,7 @@ __host__ __device__ inline int nbf_of(int l) { return (l + 1) * (l + 2) / 2; }
 // Stripe plan: for each ket shell sb, consecutive bra shells grouped while
 // rows * nbf(sb) fits the per-lane LDS capacity `cap`.
 inline std::string build_shape(int64_t ns, int64_t K, int64_t NZ, const int64_t *ls, int lmax_allowed, int cap,
-                               OvlShape &s)
+                               OvlShape &s, bool split_light = false)
 {
     if (ns < 1 || ns > OVL_MAXSH) return "number of shells must be in 1.." + std::to_string(OVL_MAXSH);
     if (K < 1 || K > OVL_MAXK) return "primitive slots K must be in 1.." + std::to_string(OVL_MAXK);
@@ -104,25 +110,47 @@ inline std::string build_shape(int64_t ns, int64_t K, int64_t NZ, const int64_t 
         s.off[q + 1] = s.off[q] + nbf_of(static_cast<int>(ls[q]));
     }
     s.N = s.off[ns];
-    int maxlen = 0;
-    s.nchunk = 0;
-    for (int sb = 0; sb < ns; ++sb) {
-        const int nb = nbf_of(s.ls[sb]);
-        int sa = 0;
-        while (sa < ns) {
-            int sa1 = sa, rows = 0;
-            while (sa1 < ns && (rows + nbf_of(s.ls[sa1])) * nb <= cap) rows += nbf_of(s.ls[sa1++]);
-            if (sa1 == sa) {  // a single shell pair larger than cap still gets its own chunk
-                rows = nbf_of(s.ls[sa]);
-                sa1 = sa + 1;
+    // split (asked for by split_light and used when some shell is heavy): pass 0 emits the light chunks (ket and bra shells all l <= OVL_LIGHT_L), pass 1
+    // the rest, and bra runs never mix light and heavy shells.  When the split needs more
+    // than OVL_MAXCH chunks (shell lists that alternate light and heavy), every chunk runs
+    // in the wide instance instead (nlight = 0).
+    auto plan = [&](bool split) -> bool {
+        int maxl = 0, maxh = 0;
+        s.nchunk = 0;
+        s.nlight = 0;
+        auto light = [&](int q) { return !split || s.ls[q] <= OVL_LIGHT_L; };
+        for (int pass = split ? 0 : 1; pass < 2; ++pass) {
+            for (int sb = 0; sb < ns; ++sb) {
+                const int nb = nbf_of(s.ls[sb]);
+                int sa = 0;
+                while (sa < ns) {
+                    const bool cls = light(sa);
+                    int sa1 = sa, rows = 0;
+                    while (sa1 < ns && light(sa1) == cls && (rows + nbf_of(s.ls[sa1])) * nb <= cap)
+                        rows += nbf_of(s.ls[sa1++]);
+                    if (sa1 == sa) {  // a single shell pair larger than cap still gets its own chunk
+                        rows = nbf_of(s.ls[sa]);
+                        sa1 = sa + 1;
+                    }
+                    const bool is_light = split && cls && light(sb);
+                    if (is_light == (pass == 0)) {
+                        if (s.nchunk >= OVL_MAXCH) return false;
+                        s.ch[s.nchunk++] = pack_chunk(OvlChunk{sb, sa, sa1, s.off[sa], rows});
+                        int &m = is_light ? maxl : maxh;
+                        m = std::max(m, rows * nb);
+                    }
+                    sa = sa1;
+                }
             }
-            if (s.nchunk >= OVL_MAXCH) return "too many stripe chunks";
-            s.ch[s.nchunk++] = pack_chunk(OvlChunk{sb, sa, sa1, s.off[sa], rows});
-            maxlen = std::max(maxlen, rows * nb);
-            sa = sa1;
+            if (pass == 0) s.nlight = s.nchunk;
         }
-    }
-    s.str = maxlen | 1;
+        s.str = std::max(maxl, maxh) | 1;
+        s.strl = split ? (maxl | 1) : s.str;
+        return true;
+    };
+    bool any_heavy = false;
+    for (int q = 0; q < s.ns; ++q) any_heavy |= s.ls[q] > OVL_LIGHT_L;
+    if (!(split_light && any_heavy && plan(true)) && !plan(false)) return "too many stripe chunks";
     return std::string();
 }
 
@@ -213,7 +241,8 @@ struct Ovl2cPlanState {
     int dtype = 0;
     int64_t ls[OVL_MAXSH] = {};  // host copy of the shell l's (call.ls points here)
     Ovl2cCall call{};
-    OvlShape sh{};
+    OvlShape sh{};   // pullback / position-cotangent chunks (light ones split off)
+    OvlShape shf{};  // forward chunks
     Ovl2cWs w{};
     void *buf = nullptr;
     size_t bytes = 0;
