@@ -16,7 +16,7 @@ from .basisset import BasisSet, Shell, atomic_number, basis_set, normalise_shell
 from .compiled import UNITFUL_HINT, CompiledBasis, Length, adapt_basis, compile_shells, to_bohr
 from .orbitals import (NO_TANGENT, NoTangent, PStates, evaluate, evaluate_ed, pullback_ps, pullback_x,
                        rrule_evaluate)
-from .overlap import (Ovl2cPlan, batch_overlap, batch_overlap_, batch_overlap_3c, batch_overlap_3c_,
+from .overlap import (OverlapStepGraph, Ovl2cPlan, batch_overlap, batch_overlap_, batch_overlap_3c, batch_overlap_3c_,
                       overlap_3c_pb_coef_alpha, overlap_pb_coef_alpha, overlap_pullback_x, rrule_batch_overlap,
                       rrule_batch_overlap_3c)
 from .overlap import pullback_ps as overlap_pullback_ps
@@ -24,7 +24,7 @@ from .overlap import pullback_ps as overlap_pullback_ps
 __all__ = [
     "ANG2BOHR", "AOKArgumentError", "AOKDeviceError", "AOKError", "AOKLibraryError", "AOKLmaxError",
     "AOKSpeciesError", "AtomicOrbitals", "BasisSet", "CartesianGTOBasis", "CompiledBasis", "Context",
-    "GaussianTypeRadials", "Length", "NO_TANGENT", "NoTangent", "Ovl2cPlan", "PStates", "Shell", "SlaterTypeRadials",
+    "GaussianTypeRadials", "Length", "NO_TANGENT", "NoTangent", "OverlapStepGraph", "Ovl2cPlan", "PStates", "Shell", "SlaterTypeRadials",
     "UNITFUL_HINT", "adapt_basis", "as_fortran", "atomic_number", "basis_set", "batch_overlap", "batch_overlap_",
     "batch_overlap_3c", "batch_overlap_3c_", "compile_basis", "compile_coef", "compile_coef_pb", "compile_shells",
     "context", "evaluate", "evaluate_ed", "fortran_from_numpy", "fortran_zeros", "gaussian_orbitals",

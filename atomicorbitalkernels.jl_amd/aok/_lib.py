@@ -8,6 +8,7 @@ wrappers (which end in ``KA.synchronize``).
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import threading
@@ -111,7 +112,12 @@ _SIGS = {
                               _dbl, ctypes.POINTER(_vp)],
     "aok_ovl2c_plan_forward": [_vp, _vp, _vp, _vp],
     "aok_ovl2c_plan_pullback": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "aok_ovl2c_plan_update": [_vp],
     "aok_ovl2c_plan_destroy": [_vp],
+    "aok_graph_begin": [_vp],
+    "aok_graph_end": [_vp, ctypes.POINTER(_vp)],
+    "aok_graph_launch": [_vp],
+    "aok_graph_destroy": [_vp],
     "aok_ovl3c_pullback": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p,
                            _vp, _vp, _dbl, _vp, _vp],
     "aok_ovl3c": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp,
@@ -165,8 +171,14 @@ class Context:
     device, or a blocking stream ordered with it when that is the NULL stream; or the
     stream handle given), workspace and error state."""
 
-    def __init__(self, device: int = 0, stream: int | None = None):
+    def __init__(self, device: int = 0, stream: int | None = None, torch_stream=None):
         self.lib = load()
+        # a torch.cuda.Stream the context runs on (side_stream): calls wait for torch's
+        # current stream through events instead of relying on NULL-stream ordering
+        self.torch_stream = torch_stream
+        self.capturing = False
+        if torch_stream is not None:
+            stream = torch_stream.cuda_stream
         if stream is None:
             # torch's default stream: shared when it is a real stream; when it is the legacy
             # NULL stream the library creates a blocking stream, which is ordered with it
@@ -197,8 +209,16 @@ class Context:
         msg = self.lib.aok_last_error(self.handle).decode(errors="replace")
         raise _ERRORS.get(rc, AOKError)(f"{what}: {msg} (status {rc})")
 
+    @classmethod
+    def side_stream(cls, device: int = 0):
+        """A context on a new torch side stream (non-blocking, so it can be captured into
+        a HIP graph while torch keeps the NULL stream)."""
+        import torch
+
+        return cls(device, torch_stream=torch.cuda.Stream(device))
+
     def call(self, name: str, *args):
-        _order_inputs(self.device)
+        _order_inputs(self)
         rc = getattr(self.lib, name)(self.handle, *args)
         self.check(rc, name)
 
@@ -241,6 +261,29 @@ class Context:
         self.call("aok_tune_get", key.encode(), ctypes.byref(v))
         return int(v.value)
 
+    @contextlib.contextmanager
+    def capture(self):
+        """Record this context's calls into a HIP graph (aok_graph_begin / _end) instead of
+        running them:  ``with ctx.capture() as g: ...`` then ``g.replay()``.  Buffers,
+        plans and tensors used inside stay referenced by the graph's device pointers: keep
+        them alive (``g.keep(...)``) and update inputs in place.  Run the same calls once
+        before capturing so that workspaces are sized.  Capture a context on a side stream
+        (``Context.side_stream``) unless the library's own stream is used throughout."""
+        g = Graph(self)
+        self.check(self.lib.aok_graph_begin(self.handle), "aok_graph_begin")
+        self.capturing = True
+        ok = False
+        try:
+            yield g
+            ok = True
+        finally:
+            self.capturing = False
+            h = _vp()
+            rc = self.lib.aok_graph_end(self.handle, ctypes.byref(h))
+            if ok:
+                self.check(rc, "aok_graph_end")
+                g.handle = h
+
     def last_kernel_ms(self) -> float:
         return float(self.lib.aok_last_kernel_ms(self.handle))
 
@@ -248,13 +291,49 @@ class Context:
         return self.lib.aok_last_kernel_name(self.handle).decode()
 
 
-def _order_inputs(device: int):
-    """The library's own stream is ordered after torch's default stream; if torch is
-    currently queueing on another stream, wait for it so the inputs are complete."""
+class Graph:
+    """A recorded sequence of one context's calls (aok_graph_*), replayed as one launch
+    on the context stream.  ``keep`` pins the tensors and plans the recording uses."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.handle = None
+        self._keep = []
+
+    def keep(self, *objs):
+        self._keep.extend(objs)
+
+    def replay(self):
+        _order_inputs(self.ctx)
+        self.ctx.check(self.ctx.lib.aok_graph_launch(self.handle), "aok_graph_launch")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.aok_graph_destroy(self.handle)
+            self.handle = None
+        self._keep = []
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _order_inputs(ctx: "Context"):
+    """Order the context's next work after the inputs torch has queued.  A context on
+    its own blocking stream is ordered after torch's default (NULL) stream by HIP; if
+    torch is queueing on another stream, wait for it.  A context on a torch side stream
+    waits for torch's current stream with an event, except while it is being captured
+    (a capture records no dependencies outside itself: inputs are ordered at replay)."""
     import torch
 
-    cur = torch.cuda.current_stream(device)
-    if cur != torch.cuda.default_stream(device):
+    cur = torch.cuda.current_stream(ctx.device)
+    if ctx.torch_stream is not None:
+        if not ctx.capturing and cur != ctx.torch_stream:
+            ctx.torch_stream.wait_stream(cur)
+        return
+    if cur != torch.cuda.default_stream(ctx.device):
         cur.synchronize()
 
 

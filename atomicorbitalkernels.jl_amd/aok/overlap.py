@@ -232,7 +232,7 @@ class Ovl2cPlan:
         b, _ = _sidx_ptr(sB)
         ns, K, NZ = p1.shape
         h = _lib.c_void_p()
-        _lib._order_inputs(self.ctx.device)
+        _lib._order_inputs(self.ctx)
         rc = self.ctx.lib.aok_ovl2c_plan_create(self.ctx.handle, _dtype_code(dtype), _flags(mode), a, b, it, B, ns, K,
                                                 NZ, _lib.i64_array(ls), params, p1.data_ptr(), p2.data_ptr(),
                                                 float(lengthscale), _lib.byref(h))
@@ -241,16 +241,26 @@ class Ovl2cPlan:
 
     def forward(self, pA, pB, out):
         _lib.require_device(pA, pB, out)
-        _lib._order_inputs(self.ctx.device)
+        _lib._order_inputs(self.ctx)
         rc = self.ctx.lib.aok_ovl2c_plan_forward(self.handle, pA.data_ptr(), pB.data_ptr(), out.data_ptr())
         self.ctx.check(rc, "aok_ovl2c_plan_forward")
         return out
 
-    def pullback(self, dS, pA, pB):
+    def update(self):
+        """Rebuild the bucketing and primitive-pair records from the species-index and
+        parameter tensors this plan holds, as they are now (aok_ovl2c_plan_update; no
+        reference counterpart)."""
+        _lib._order_inputs(self.ctx)
+        self.ctx.check(self.ctx.lib.aok_ovl2c_plan_update(self.handle), "aok_ovl2c_plan_update")
+
+    def pullback(self, dS, pA, pB, out=None):
         _lib.require_device(dS, pA, pB)
-        d1 = fortran_empty(self.shape, self.dtype, dS.device)
-        d2 = fortran_empty(self.shape, self.dtype, dS.device)
-        _lib._order_inputs(self.ctx.device)
+        if out is None:
+            d1 = fortran_empty(self.shape, self.dtype, dS.device)
+            d2 = fortran_empty(self.shape, self.dtype, dS.device)
+        else:
+            d1, d2 = out
+        _lib._order_inputs(self.ctx)
         rc = self.ctx.lib.aok_ovl2c_plan_pullback(self.handle, dS.data_ptr(), pA.data_ptr(), pB.data_ptr(),
                                                   d1.data_ptr(), d2.data_ptr())
         self.ctx.check(rc, "aok_ovl2c_plan_pullback")
@@ -261,7 +271,7 @@ class Ovl2cPlan:
         _lib.require_device(dS, pA, pB)
         dA = torch.empty((self.B, 3), dtype=self.dtype, device=dS.device)
         dB = torch.empty((self.B, 3), dtype=self.dtype, device=dS.device)
-        _lib._order_inputs(self.ctx.device)
+        _lib._order_inputs(self.ctx)
         rc = self.ctx.lib.aok_ovl2c_plan_pullback_x(self.handle, dS.data_ptr(), pA.data_ptr(), pB.data_ptr(),
                                                     dA.data_ptr(), dB.data_ptr())
         self.ctx.check(rc, "aok_ovl2c_plan_pullback_x")
@@ -313,6 +323,61 @@ def rrule_batch_overlap(orb: AtomicOrbitals, XA, XB, ps, st=None, *, mode="fast"
         return NO_TANGENT, NO_TANGENT, NO_TANGENT, NO_TANGENT, dps, NO_TANGENT
 
     return S, pullback
+
+
+class OverlapStepGraph:
+    """The rrule(batch_overlap) step of one fixed batch -- plan update (bucketing and
+    primitive-pair records from the current parameters), forward S and (zeta, D) pullback
+    of the cotangent ``dS`` -- recorded once into a HIP graph and replayed as one launch
+    (no reference counterpart: the reference relaunches its kernels per call).
+
+    Replays read ``params["Rnl"]["zeta"]``, ``params["Rnl"]["D"]``, ``posA``, ``posB`` and
+    ``dS`` (the tensors held here) as they are at replay time and write ``S``, ``dzeta``
+    and ``dD``: update them in place between replays (an optimiser step, the next
+    cotangent).  The graph runs on its own context on a torch side stream unless ``ctx``
+    is given."""
+
+    def __init__(self, orb: AtomicOrbitals, XA, XB, ps, *, dtype=None, ctx=None):
+        if orb.Rnl.decay != _lib.AOK_DECAY_GAUSSIAN:
+            raise _lib.AOKArgumentError("the overlap kernels integrate plain Gaussians")
+        ls, _, off = compile_struct(orb)
+        pA, sA, pB, sB = _pair_inputs(XA, XB, orb.Rnl.zlist)
+        zeta, D = ps["Rnl"]["zeta"], ps["Rnl"]["D"]
+        dt = dtype or torch.promote_types(pA.dtype, torch.promote_types(zeta.dtype, D.dtype))
+        dev = pA.device
+        self.ctx = ctx or _lib.Context.side_stream(dev.index or 0)
+        self.params = {"Rnl": {"zeta": as_fortran(zeta.to(dt)).clone(memory_format=torch.preserve_format),
+                               "D": as_fortran(D.to(dt)).clone(memory_format=torch.preserve_format)}, "Ylm": {}}
+        self.posA, self.posB = pA.to(dt).clone(), pB.to(dt).clone()
+        B, N = pA.shape[0], off[-1]
+        self.B, self.N = B, N
+        z, d = self.params["Rnl"]["zeta"], self.params["Rnl"]["D"]
+        self.plan = Ovl2cPlan(ls, sA, sB, B, z, d, _lib.AOK_PARAMS_PS, orb.lengthscale, dt, ctx=self.ctx)
+        self.S = _out_tensor(N, B, dt, dev)
+        self.dS = _out_tensor(N, B, dt, dev).zero_()
+        self.dzeta = fortran_empty(tuple(z.shape), dt, dev)
+        self.dD = fortran_empty(tuple(z.shape), dt, dev)
+        torch.cuda.synchronize(dev)
+        self._record()  # eager pass: sizes every workspace before the capture
+        with self.ctx.capture() as g:
+            self._record()
+        g.keep(self.plan, self.params, self.posA, self.posB, self.S, self.dS, self.dzeta, self.dD, sA, sB)
+        self.graph = g
+
+    def _record(self):
+        self.plan.update()
+        self.plan.forward(self.posA, self.posB, self.S)
+        self.plan.pullback(self.dS, self.posA, self.posB, out=(self.dzeta, self.dD))
+
+    def replay(self):
+        """One step: returns (S, dps) -- the graph's own output tensors, overwritten by the
+        next replay."""
+        self.graph.replay()
+        return self.S, {"Rnl": {"zeta": self.dzeta, "D": self.dD}, "Ylm": {}}
+
+    def close(self):
+        self.graph.close()
+        self.plan.close()
 
 
 def batch_overlap_3c_(out, basis, XA, XB, XC, *, mode="fast", ctx=None):

@@ -51,6 +51,15 @@ struct aok_context {
     std::deque<std::pair<const char *, double>> done;
     double last_ms = 0.0;
     const char *last_kernel = "";
+    // HIP-graph capture (aok_graph_begin .. aok_graph_end, or any capture of this stream):
+    // calls enqueue without synchronising, timing or clearing the error word, and may not
+    // grow the workspace.  A call that fails while capturing marks the capture failed.
+    bool capturing = false;
+    bool capture_failed = false;
+    // Graphs of this context still alive: device buffers they may reference (workspace,
+    // plan buffers) are retired instead of freed until the last one is destroyed.
+    int live_graphs = 0;
+    std::vector<void *> retired;
     // Tuning knobs (0 = built-in default), set by aok_tune or the autotuner.
     struct {
         int ao_ppl = 0;   // AO values-only: consecutive points per lane (1, 2, 4)
@@ -68,7 +77,10 @@ inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 // Status helpers ---------------------------------------------------------------
 inline aok_status fail(aok_ctx ctx, aok_status st, const std::string &msg)
 {
-    if (ctx) ctx->last_error = msg;
+    if (ctx) {
+        ctx->last_error = msg;
+        if (ctx->capturing) ctx->capture_failed = true;
+    }
     return st;
 }
 
@@ -82,6 +94,12 @@ inline aok_status fail(aok_ctx ctx, aok_status st, const std::string &msg)
 
 // Ensure at least `bytes` of workspace; returns the base pointer or nullptr.
 aok_status ensure_ws(aok_ctx ctx, size_t bytes);
+
+// True while the context stream is being captured into a HIP graph.
+bool in_capture(aok_ctx ctx);
+// Give back a device buffer of the context: freed once the stream has drained, or kept
+// until the last live graph is destroyed (a graph may reference it).
+aok_status release_buffer(aok_ctx ctx, void *buf);
 
 // Clear the error word (host store, before the call's first launch).
 aok_status clear_err(aok_ctx ctx);

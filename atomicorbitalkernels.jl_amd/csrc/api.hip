@@ -8,16 +8,40 @@
 
 namespace aok {
 
+bool in_capture(aok_ctx ctx)
+{
+    if (ctx->capturing) return true;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+}
+
+aok_status release_buffer(aok_ctx ctx, void *buf)
+{
+    if (!buf) return AOK_OK;
+    if (ctx->live_graphs > 0 || in_capture(ctx)) {
+        ctx->retired.push_back(buf);
+        return AOK_OK;
+    }
+    AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    AOK_HIP(ctx, hipFree(buf));
+    return AOK_OK;
+}
+
 aok_status ensure_ws(aok_ctx ctx, size_t bytes)
 {
     if (bytes <= ctx->ws_bytes) return AOK_OK;
+    if (in_capture(ctx))
+        return fail(ctx, AOK_ERR_ARG,
+                    "a workspace of " + std::to_string(bytes) +
+                        " bytes is needed inside a graph capture: run the same calls once before capturing");
     size_t want = std::max(bytes, ctx->ws_bytes + ctx->ws_bytes / 2);
     want = (want + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
     if (ctx->ws) {
-        AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        AOK_HIP(ctx, hipFree(ctx->ws));
+        void *old = ctx->ws;
         ctx->ws = nullptr;
         ctx->ws_bytes = 0;
+        aok_status st = release_buffer(ctx, old);
+        if (st != AOK_OK) return st;
     }
     if (hipMalloc(&ctx->ws, want) != hipSuccess) {
         ctx->ws = nullptr;
@@ -34,7 +58,7 @@ aok_status ensure_ws(aok_ctx ctx, size_t bytes)
 // every call until aok_sync reads and clears it.
 aok_status clear_err(aok_ctx ctx)
 {
-    if (ctx->blocking) {
+    if (ctx->blocking && !in_capture(ctx)) {
         *reinterpret_cast<volatile unsigned *>(ctx->h_err) = 0u;
         // a blocking call reports its own timed kernel (none: 0)
         ctx->last_ms = 0.0;
@@ -61,7 +85,7 @@ static hipEvent_t take_event(aok_ctx ctx)
 void time_begin(aok_ctx ctx, const char *name)
 {
     ctx->open = false;
-    if (!ctx->timing) return;
+    if (!ctx->timing || in_capture(ctx)) return;
     hipEvent_t a = take_event(ctx), b = a ? take_event(ctx) : nullptr;
     if (!a || !b || hipEventRecord(a, ctx->stream) != hipSuccess) {
         (void)hipGetLastError();
@@ -106,8 +130,9 @@ void collect_times(aok_ctx ctx)
 
 aok_status finish(aok_ctx ctx, const char *what)
 {
-    // stream-ordered: return with the work enqueued (timed launches are read later)
-    if (!ctx->blocking) return AOK_OK;
+    // stream-ordered: return with the work enqueued (timed launches are read later);
+    // under graph capture the work is only recorded
+    if (!ctx->blocking || in_capture(ctx)) return AOK_OK;
     AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     collect_times(ctx);
     if (*reinterpret_cast<volatile unsigned *>(ctx->h_err) & ERR_SPECIES)
@@ -171,6 +196,7 @@ aok_status aok_destroy(aok_ctx ctx)
         if (p->buf) (void)hipFree(p->buf);
         p->buf = nullptr, p->bytes = 0, p->ctx = nullptr;
     }
+    for (void *r : ctx->retired) (void)hipFree(r);
     if (ctx->h_err) (void)hipHostFree(ctx->h_err);
     for (const auto &t : ctx->pend) {
         (void)hipEventDestroy(t.a);
@@ -261,6 +287,7 @@ aok_status aok_set_timing(aok_ctx ctx, int enable)
 aok_status aok_sync(aok_ctx ctx)
 {
     AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, !in_capture(ctx), "aok_sync inside a graph capture");
     AOK_HIP(ctx, hipSetDevice(ctx->device));
     AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     collect_times(ctx);
@@ -583,6 +610,19 @@ aok_status aok_ovl2c_plan_pullback_x(aok_ovl2c_plan plan, const void *dS, const 
     return finish(ctx, "aok_ovl2c_plan_pullback_x");
 }
 
+aok_status aok_ovl2c_plan_update(aok_ovl2c_plan plan)
+{
+    if (!plan || !plan->st.ctx) return AOK_ERR_ARG;
+    aok_ctx ctx = plan->st.ctx;
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    st = plan->st.dtype == AOK_F32 ? ovl2c_plan_refresh<float>(ctx, plan->st)
+                                   : ovl2c_plan_refresh<double>(ctx, plan->st);
+    if (st != AOK_OK) return st;
+    return finish(ctx, "aok_ovl2c_plan_update");
+}
+
 aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan)
 {
     if (!plan) return AOK_OK;
@@ -612,10 +652,7 @@ aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan)
         drop = ctx->plan_cache;
         ctx->plan_cache = buf, ctx->plan_cache_bytes = bytes;
     }
-    if (drop) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipFree(drop);
-    }
+    (void)release_buffer(ctx, drop);
     return AOK_OK;
 }
 
@@ -662,6 +699,90 @@ aok_status aok_ovl3c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, cons
     st = dtype == AOK_F32 ? ovl3c_pullback_run<float>(ctx, c) : ovl3c_pullback_run<double>(ctx, c);
     if (st != AOK_OK) return st;
     return finish(ctx, "aok_ovl3c_pullback");
+}
+
+// ---- HIP graphs ----------------------------------------------------------------------
+struct aok_graph_s {
+    aok_ctx ctx;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+aok_status aok_graph_begin(aok_ctx ctx)
+{
+    AOK_CHECK_CTX(ctx);
+    AOK_REQUIRE(ctx, !in_capture(ctx), "the context stream is already being captured");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    AOK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    ctx->capturing = true;
+    ctx->capture_failed = false;
+    return AOK_OK;
+}
+
+aok_status aok_graph_end(aok_ctx ctx, aok_graph *graph)
+{
+    AOK_CHECK_CTX(ctx);
+    if (graph) *graph = nullptr;
+    AOK_REQUIRE(ctx, ctx->capturing, "no capture in progress (aok_graph_begin)");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->capturing = false;
+    const bool failed = ctx->capture_failed;
+    ctx->capture_failed = false;
+    if (e != hipSuccess || failed || !graph) {
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess)
+            return fail(ctx, AOK_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        if (!graph) return fail(ctx, AOK_ERR_ARG, "graph is NULL");
+        return fail(ctx, AOK_ERR_ARG, "a call failed during the capture: " + ctx->last_error);
+    }
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphDestroy(g);
+        return fail(ctx, AOK_ERR_DEVICE, "hipGraphInstantiate failed");
+    }
+    auto *gr = new (std::nothrow) aok_graph_s{ctx, g, x};
+    if (!gr) {
+        (void)hipGraphExecDestroy(x);
+        (void)hipGraphDestroy(g);
+        return fail(ctx, AOK_ERR_NOMEM, "graph allocation failed");
+    }
+    ++ctx->live_graphs;
+    *graph = gr;
+    return AOK_OK;
+}
+
+aok_status aok_graph_launch(aok_graph graph)
+{
+    if (!graph || !graph->ctx) return AOK_ERR_ARG;
+    aok_ctx ctx = graph->ctx;
+    AOK_REQUIRE(ctx, !in_capture(ctx), "cannot launch a graph into a stream under capture");
+    AOK_HIP(ctx, hipSetDevice(ctx->device));
+    aok_status st = clear_err(ctx);
+    if (st != AOK_OK) return st;
+    AOK_HIP(ctx, hipGraphLaunch(graph->exec, ctx->stream));
+    return finish(ctx, "aok_graph_launch");
+}
+
+aok_status aok_graph_destroy(aok_graph graph)
+{
+    if (!graph) return AOK_OK;
+    aok_ctx ctx = graph->ctx;
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);  // a launched replay may still run
+    }
+    (void)hipGraphExecDestroy(graph->exec);
+    (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    if (!ctx) return AOK_OK;
+    if (--ctx->live_graphs == 0) {
+        for (void *r : ctx->retired) (void)hipFree(r);
+        ctx->retired.clear();
+    }
+    return AOK_OK;
 }
 
 }  // extern "C"

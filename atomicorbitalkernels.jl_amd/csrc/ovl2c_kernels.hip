@@ -1580,11 +1580,16 @@ aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p)
     if (c.flags & AOK_PLAIN) return AOK_OK;
     p.w = ovl2c_ws_map(c.B, p.sh.ns, p.sh.K, p.sh.NZ, p.sh.nchunk, sizeof(typename RecOf<T>::type), true);
     if (p.w.total > p.bytes) {
+        if (in_capture(ctx))
+            return fail(ctx, AOK_ERR_ARG,
+                        "a plan workspace of " + std::to_string(p.w.total) +
+                            " bytes is needed inside a graph capture: build the plan before capturing");
         if (p.buf) {
-            AOK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            AOK_HIP(ctx, hipFree(p.buf));
+            void *old = p.buf;
             p.buf = nullptr;
             p.bytes = 0;
+            st = release_buffer(ctx, old);
+            if (st != AOK_OK) return st;
         }
         if (hipMalloc(&p.buf, p.w.total) != hipSuccess) {
             p.buf = nullptr;
@@ -1594,6 +1599,15 @@ aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p)
     }
     if (c.B > 0) return fast_prep<T>(ctx, c, p.sh, p.w, static_cast<char *>(p.buf));
     return AOK_OK;
+}
+
+// Rebuild a plan's bucketing and records from the species indices and parameters it was
+// created with, as they are now in device memory (same sizes and pointers).
+template <typename T>
+aok_status ovl2c_plan_refresh(aok_ctx ctx, Ovl2cPlanState &p)
+{
+    if ((p.call.flags & AOK_PLAIN) || p.call.B == 0) return AOK_OK;
+    return fast_prep<T>(ctx, p.call, p.sh, p.w, static_cast<char *>(p.buf));
 }
 
 template <typename T>
@@ -1614,6 +1628,8 @@ aok_status ovl2c_plan_exec(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall
 
 template aok_status ovl2c_plan_build<float>(aok_ctx, const Ovl2cCall &, Ovl2cPlanState &);
 template aok_status ovl2c_plan_build<double>(aok_ctx, const Ovl2cCall &, Ovl2cPlanState &);
+template aok_status ovl2c_plan_refresh<float>(aok_ctx, Ovl2cPlanState &);
+template aok_status ovl2c_plan_refresh<double>(aok_ctx, Ovl2cPlanState &);
 template aok_status ovl2c_plan_exec<float>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &, bool);
 template aok_status ovl2c_plan_exec<double>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &, bool);
 

@@ -253,6 +253,8 @@ aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback);
 template <typename T>
 aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p);
 template <typename T>
+aok_status ovl2c_plan_refresh(aok_ctx ctx, Ovl2cPlanState &p);
+template <typename T>
 aok_status ovl2c_plan_exec(aok_ctx ctx, const Ovl2cPlanState &p, const Ovl2cCall &io, bool pullback);
 template <typename T>
 aok_status ovl2c_dx_run(aok_ctx ctx, const Ovl2cCall &c);

@@ -218,6 +218,11 @@ aok_status aok_ovl2c_plan_pullback(aok_ovl2c_plan plan, const void *dS, const vo
 /* Position cotangents of dS through the plan: (dA, dB) [3*B] as aok_ovl2c_pullback_x. */
 aok_status aok_ovl2c_plan_pullback_x(aok_ovl2c_plan plan, const void *dS, const void *posA,
                                      const void *posB, void *dA, void *dB);
+/* No reference counterpart: re-read the species indices and parameters the plan was
+ * created from (same pointers and sizes, contents changed in place, e.g. by an optimiser
+ * step) and rebuild its bucketing and primitive-pair records.  Captured in a graph, it
+ * makes every replay see the parameters as they are at replay time. */
+aok_status aok_ovl2c_plan_update(aok_ovl2c_plan plan);
 aok_status aok_ovl2c_plan_destroy(aok_ovl2c_plan plan);
 
 /* ---- H2: 3-centre overlap ------------------------------------------------------------ */
@@ -247,6 +252,23 @@ aok_status aok_ovl3c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, cons
                               aok_itype itype, int64_t B, int64_t ns, int64_t K, int64_t NZ,
                               const int64_t *ls, const void *coef, const void *alpha,
                               double lengthscale, void *dcoef, void *dalpha);
+
+/* ---- HIP graphs ---------------------------------------------------------------------- */
+/* No reference counterpart (the reference relaunches every kernel per call).  Between
+ * aok_graph_begin and aok_graph_end the context's calls are recorded, not run: nothing
+ * synchronises, no timing events are taken, and a call that would need a larger
+ * workspace (run it once eagerly first) or plan buffer fails and fails the capture.
+ * The graph keeps every device pointer it was recorded with: the caller keeps those
+ * buffers and plans alive, and updates inputs in place, until aok_graph_destroy.
+ * aok_graph_launch replays the whole recorded sequence as one launch on the context
+ * stream; in blocking mode it waits and reports device-side errors like any call.
+ * Workspaces the library outgrows while graphs are alive are kept until the last graph
+ * of the context is destroyed. */
+typedef struct aok_graph_s *aok_graph;
+aok_status aok_graph_begin(aok_ctx ctx);
+aok_status aok_graph_end(aok_ctx ctx, aok_graph *graph);
+aok_status aok_graph_launch(aok_graph graph);
+aok_status aok_graph_destroy(aok_graph graph);
 
 /* ---- tuning -------------------------------------------------------------------------------- */
 /* No reference counterpart (the reference has no tunables): set / read a per-context
