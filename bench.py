@@ -195,7 +195,29 @@ def main():
         "fwd_kernel_pairs_per_s": B / (fwd_ms * 1e-3), "grad_kernel_pairs_per_s": B / (bwd_ms * 1e-3),
     }
 
+    def graph_step_ms(orb, XA, XB, ps, dS, n):
+        """The same rrule step recorded once into a HIP graph (OverlapStepGraph: plan update,
+        forward, pullback, normalisation VJP) and replayed n times, stream-ordered."""
+        gs = aok.OverlapStepGraph(orb, XA, XB, ps)
+        gs.dS.copy_(dS)
+        torch.cuda.synchronize()
+        gs.ctx.set_blocking(False)
+        for _ in range(2):
+            gs.replay()
+        gs.ctx.sync()
+        ts = time.perf_counter()
+        for _ in range(n):
+            gs.replay()
+        gs.ctx.sync()
+        ms = (time.perf_counter() - ts) * 1e3 / n
+        gs.close()
+        return ms
+
     if not args.no_extra:
+        log("C1 graph-replayed step")
+        gms = graph_step_ms(orb32, XA32, XB32, ps32, dS32, args.steps)
+        out["c1_graph_step"] = {"ms_per_step": gms, "pairs_per_s": B / (gms * 1e-3),
+                                "kernels": "plan update (3) + forward + pullback (3), one hipGraphLaunch per step"}
         # FP32 vs FP64 (fast FP64 kernels, themselves pinned to the oracle by the tests)
         log("C1 FP64 reference run for the FP32 error")
         orb64 = orbital(torch.float64)
@@ -349,6 +371,8 @@ def main():
             bk.append(ctx.last_kernel_ms())
         torch.cuda.synchronize()
         wall2 = (time.perf_counter() - ts) * 1e3 / 5
+        log("C2 graph-replayed step")
+        gwall2 = graph_step_ms(o2, X2A, X2B, p2, dS2, 5)
         f2f, f2b = algorithmic_flops(cb2, d2["sidxA"], d2["sidxB"])
         o2d = orbital2(torch.float64)
         S2d = aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
@@ -363,6 +387,7 @@ def main():
         plain2 = ctx.last_kernel_ms()
         out["ovl2c_c2"] = {
             "B": B2, "N": cb2.N, "NZ": cb2.NZ, "step_ms": wall2, "pairs_per_s": B2 / (wall2 * 1e-3),
+            "graph_step_ms": gwall2, "graph_pairs_per_s": B2 / (gwall2 * 1e-3),
             "fwd_kernel_ms": float(np.mean(fk)), "bwd_kernel_ms": float(np.mean(bk)),
             "fwd_hermite_model_TFLOPs": f2f / (np.mean(fk) * 1e-3) / 1e12, "bwd_hermite_model_TFLOPs": f2b / (np.mean(bk) * 1e-3) / 1e12,
             "fp32_blockwise_max_rel_err": err2, "plain_f64_fwd_kernel_ms": plain2,
