@@ -65,6 +65,7 @@ struct aok_context {
         int ao_ppl = 0;   // AO values-only: consecutive points per lane (1, 2, 4)
         int ao_grid = 0;  // AO: persistent grid size (blocks)
         int ovl_cap = 0;  // 2c: LDS stripe capacity (floats per lane)
+        int ovl_split = 0;  // 2c pullback: light chunks in the l <= 2 instance (0/1 yes, 2 no)
     } tune;
 };
 

@@ -259,6 +259,9 @@ aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
     } else if (k == "ovl_cap") {
         AOK_REQUIRE(ctx, value == 0 || (value >= 16 && value <= 1024), "ovl_cap must be 0 or in 16..1024");
         ctx->tune.ovl_cap = static_cast<int>(value);
+    } else if (k == "ovl_split") {
+        AOK_REQUIRE(ctx, value >= 0 && value <= 2, "ovl_split must be 0, 1 or 2");
+        ctx->tune.ovl_split = static_cast<int>(value);
     } else {
         return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
     }
@@ -273,6 +276,7 @@ aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value)
     if (k == "ao_ppl") *value = ctx->tune.ao_ppl;
     else if (k == "ao_grid") *value = ctx->tune.ao_grid;
     else if (k == "ovl_cap") *value = ctx->tune.ovl_cap;
+    else if (k == "ovl_split") *value = ctx->tune.ovl_split;
     else return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
     return AOK_OK;
 }

@@ -1422,7 +1422,8 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
 static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh, bool split_light)
 {
     const int cap = ctx->tune.ovl_cap > 0 ? ctx->tune.ovl_cap : OVL_STRIPE_CAP;
-    const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, cap, sh, split_light);
+    const std::string msg =
+        build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, cap, sh, split_light && ctx->tune.ovl_split != 2);
     if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
     if (c.B < 0) return fail(ctx, AOK_ERR_ARG, "B must be >= 0");
     if (c.B > (int64_t(1) << 30)) return fail(ctx, AOK_ERR_ARG, "B too large");

@@ -195,10 +195,13 @@ def main():
         "fwd_kernel_pairs_per_s": B / (fwd_ms * 1e-3), "grad_kernel_pairs_per_s": B / (bwd_ms * 1e-3),
     }
 
-    def graph_step_ms(orb, XA, XB, ps, dS, n):
+    def graph_step_ms(orb, XA, XB, ps, dS, n, knobs=None):
         """The same rrule step recorded once into a HIP graph (OverlapStepGraph: plan update,
         forward, pullback, normalisation VJP) and replayed n times, stream-ordered."""
-        gs = aok.OverlapStepGraph(orb, XA, XB, ps)
+        gctx = aok.Context.side_stream(0)
+        for k, v in (knobs or {}).items():
+            gctx.tune(k, v)
+        gs = aok.OverlapStepGraph(orb, XA, XB, ps, ctx=gctx)
         gs.dS.copy_(dS)
         torch.cuda.synchronize()
         gs.ctx.set_blocking(False)
@@ -211,6 +214,7 @@ def main():
         gs.ctx.sync()
         ms = (time.perf_counter() - ts) * 1e3 / n
         gs.close()
+        gctx.close()
         return ms
 
     if not args.no_extra:
@@ -358,6 +362,11 @@ def main():
                torch.as_tensor(d2["sidxB"], dtype=torch.int32, device=dev))
         g2 = torch.Generator(device=dev).manual_seed(9)
         dS2 = torch.randn((B2, cb2.N, cb2.N), generator=g2, device=dev, dtype=torch.float32).permute(2, 1, 0)
+        # launch-shape knobs chosen by the autotuner on this workload (stripe capacity,
+        # light/heavy chunk split), then the timed steps with them
+        log("C2 autotune")
+        tknobs, tms, tres = aok.autotune.tune_overlap(o2, X2A, X2B, p2, dS2, ctx=ctx, save=False, steps=2,
+                                                      trials=2)
         for _ in range(2):
             S2, pb2 = aok.rrule_batch_overlap(o2, X2A, X2B, p2, ctx=ctx)
             pb2(dS2)
@@ -372,7 +381,7 @@ def main():
         torch.cuda.synchronize()
         wall2 = (time.perf_counter() - ts) * 1e3 / 5
         log("C2 graph-replayed step")
-        gwall2 = graph_step_ms(o2, X2A, X2B, p2, dS2, 5)
+        gwall2 = graph_step_ms(o2, X2A, X2B, p2, dS2, 5, tknobs)
         f2f, f2b = algorithmic_flops(cb2, d2["sidxA"], d2["sidxB"])
         o2d = orbital2(torch.float64)
         S2d = aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
@@ -388,11 +397,16 @@ def main():
         out["ovl2c_c2"] = {
             "B": B2, "N": cb2.N, "NZ": cb2.NZ, "step_ms": wall2, "pairs_per_s": B2 / (wall2 * 1e-3),
             "graph_step_ms": gwall2, "graph_pairs_per_s": B2 / (gwall2 * 1e-3),
+            "autotune": {"knobs": tknobs, "best_ms": tms,
+                         "candidates": [[k["ovl_cap"], k["ovl_split"], round(m, 4)] for k, m in tres],
+                         "candidate_fields": ["ovl_cap", "ovl_split", "step_ms"]},
             "fwd_kernel_ms": float(np.mean(fk)), "bwd_kernel_ms": float(np.mean(bk)),
             "fwd_hermite_model_TFLOPs": f2f / (np.mean(fk) * 1e-3) / 1e12, "bwd_hermite_model_TFLOPs": f2b / (np.mean(bk) * 1e-3) / 1e12,
             "fp32_blockwise_max_rel_err": err2, "plain_f64_fwd_kernel_ms": plain2,
             "fwd_speedup_vs_plain_f64": plain2 / float(np.mean(fk))}
         del S2, dS2, pb2
+        ctx.tune("ovl_cap", 0)
+        ctx.tune("ovl_split", 0)
         # 3-centre overlaps, B = 2^14 triples over the same 3 species, FP32
         log("3c B=2^14")
         c4 = configs.c4a()
