@@ -149,8 +149,8 @@ function _plan_destroy(p::Plan2c)
     p.h = C_NULL
 end
 
-function rrule(::typeof(AOK.batch_overlap), orb::AOK.GaussianTypeOrbitals, XA::AbstractVector,
-               XB::AbstractVector, ps, st)
+# the plan of one (orb, XA, XB, ps): device copies of positions, species and (ζ, D)
+function _plan2c(orb::AOK.GaussianTypeOrbitals, XA::AbstractVector, XB::AbstractVector, ps)
     str = AOK._compile_struct(orb)
     FT = promote_type(eltype(ps.Rnl.ζ), eltype(ps.Rnl.D))
     B = length(XA)
@@ -167,20 +167,73 @@ function rrule(::typeof(AOK.batch_overlap), orb::AOK.GaussianTypeOrbitals, XA::A
                AOK_PARAMS_PS, pointer(ζ), pointer(D), Float64(orb.lengthscale), h)
     check(rc, "batch_overlap (plan)")
     plan = finalizer(_plan_destroy, Plan2c(h[], (sA, sB, ζ, D)))
-    S = similar(posA, FT, str.nbf_total, str.nbf_total, B)
-    rc = ccall((:aok_ovl2c_plan_forward, libaok), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
-               plan.h, pointer(posA), pointer(posB), pointer(S))
-    check(rc, "batch_overlap")
+    return plan, posA, posB, ζ, D, FT, str.nbf_total, B
+end
+
+_plan_forward(plan, posA, posB, S) =
+    check(ccall((:aok_ovl2c_plan_forward, libaok), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                plan.h, pointer(posA), pointer(posB), pointer(S)), "batch_overlap")
+_plan_pullback(plan, ∂S, posA, posB, ∂ζ, ∂D) =
+    check(ccall((:aok_ovl2c_plan_pullback, libaok), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                plan.h, pointer(∂S), pointer(posA), pointer(posB), pointer(∂ζ), pointer(∂D)), "pullback_ps")
+
+function rrule(::typeof(AOK.batch_overlap), orb::AOK.GaussianTypeOrbitals, XA::AbstractVector,
+               XB::AbstractVector, ps, st)
+    plan, posA, posB, ζ, D, FT, N, B = _plan2c(orb, XA, XB, ps)
+    S = similar(posA, FT, N, N, B)
+    _plan_forward(plan, posA, posB, S)
     function batch_overlap_pullback(∂S)
         ∂Sd = ROCArray{FT}(∂S); ∂ζ = similar(ζ); ∂D = similar(D)
-        rc = ccall((:aok_ovl2c_plan_pullback, libaok), Cint,
-                   (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
-                   plan.h, pointer(∂Sd), pointer(posA), pointer(posB), pointer(∂ζ), pointer(∂D))
-        check(rc, "pullback_ps")
+        _plan_pullback(plan, ∂Sd, posA, posB, ∂ζ, ∂D)
         return NoTangent(), NoTangent(), NoTangent(), NoTangent(), (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple()),
                NoTangent()
     end
     return S, batch_overlap_pullback
+end
+
+# ---- HIP-graph training step (no reference counterpart) --------------------------------
+# The rrule's plan update + forward + (ζ, D) pullback recorded once (aok_graph_*) and
+# replayed as one launch.  Between replays update `sg.ζ`, `sg.D` (the graph's device copies
+# of ps.Rnl) and `sg.∂S` in place; `replay!` returns the graph's own S and cotangents.  The
+# shim's context must run on a stream other than the NULL stream (AMDGPU's task-local
+# streams are).
+mutable struct StepGraph
+    g::Ptr{Cvoid}
+    plan::Plan2c
+    posA::Any; posB::Any; ζ::Any; D::Any
+    S::Any; ∂S::Any; ∂ζ::Any; ∂D::Any
+end
+function _graph_destroy(sg::StepGraph)
+    sg.g == C_NULL || ccall((:aok_graph_destroy, libaok), Cint, (Ptr{Cvoid},), sg.g)
+    sg.g = C_NULL
+end
+
+function StepGraph(orb::AOK.GaussianTypeOrbitals, XA::AbstractVector, XB::AbstractVector, ps)
+    plan, posA, posB, ζ, D, FT, N, B = _plan2c(orb, XA, XB, ps)
+    S = similar(posA, FT, N, N, B); ∂S = fill!(similar(S), zero(FT)); ∂ζ = similar(ζ); ∂D = similar(D)
+    function record()
+        check(ccall((:aok_ovl2c_plan_update, libaok), Cint, (Ptr{Cvoid},), plan.h), "plan update")
+        _plan_forward(plan, posA, posB, S)
+        _plan_pullback(plan, ∂S, posA, posB, ∂ζ, ∂D)
+    end
+    record()                                   # eager pass: sizes every workspace
+    check(ccall((:aok_graph_begin, libaok), Cint, (Ptr{Cvoid},), ctx().h), "graph begin")
+    g = Ref{Ptr{Cvoid}}(C_NULL)
+    ok = false
+    try
+        record()
+        ok = true
+    finally                                    # always end the recording; report the first error
+        rc = ccall((:aok_graph_end, libaok), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), ctx().h, g)
+        ok && check(rc, "graph end")
+    end
+    return finalizer(_graph_destroy, StepGraph(g[], plan, posA, posB, ζ, D, S, ∂S, ∂ζ, ∂D))
+end
+
+function replay!(sg::StepGraph)
+    check(ccall((:aok_graph_launch, libaok), Cint, (Ptr{Cvoid},), sg.g), "graph launch")
+    return sg.S, (Rnl = (ζ = sg.∂ζ, D = sg.∂D), Ylm = NamedTuple())
 end
 
 # ---- H2 3-centre: batch_overlap_3c!(out, basis, posA, posB, posC) (kernels_3c.jl:183-200) --
