@@ -15,6 +15,7 @@
 namespace aok {
 struct Ovl2cPlanState;
 }
+struct aok_graph_s;
 
 struct aok_context {
     int device = 0;
@@ -58,7 +59,7 @@ struct aok_context {
     bool capture_failed = false;
     // Graphs of this context still alive: device buffers they may reference (workspace,
     // plan buffers) are retired instead of freed until the last one is destroyed.
-    int live_graphs = 0;
+    std::vector<aok_graph_s *> graphs;
     std::vector<void *> retired;
     // Tuning knobs (0 = built-in default), set by aok_tune or the autotuner.
     struct {

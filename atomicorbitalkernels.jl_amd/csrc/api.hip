@@ -18,7 +18,7 @@ bool in_capture(aok_ctx ctx)
 aok_status release_buffer(aok_ctx ctx, void *buf)
 {
     if (!buf) return AOK_OK;
-    if (ctx->live_graphs > 0 || in_capture(ctx)) {
+    if (!ctx->graphs.empty() || in_capture(ctx)) {
         ctx->retired.push_back(buf);
         return AOK_OK;
     }
@@ -145,6 +145,21 @@ static bool bad_itype(aok_itype t) { return t != AOK_I64 && t != AOK_I32; }
 
 }  // namespace aok
 
+// A recorded graph of one context.  Destroying the context first detaches it: the
+// executable is released and the handle then only accepts aok_graph_destroy.
+struct aok_graph_s {
+    aok_ctx ctx;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+static void detach_graph(aok_graph_s *g)
+{
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->exec = nullptr, g->graph = nullptr, g->ctx = nullptr;
+}
+
 using namespace aok;
 
 #define AOK_CHECK_CTX(ctx) \
@@ -196,6 +211,7 @@ aok_status aok_destroy(aok_ctx ctx)
         if (p->buf) (void)hipFree(p->buf);
         p->buf = nullptr, p->bytes = 0, p->ctx = nullptr;
     }
+    for (aok_graph_s *g : ctx->graphs) detach_graph(g);
     for (void *r : ctx->retired) (void)hipFree(r);
     if (ctx->h_err) (void)hipHostFree(ctx->h_err);
     for (const auto &t : ctx->pend) {
@@ -705,12 +721,6 @@ aok_status aok_ovl3c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, cons
     return finish(ctx, "aok_ovl3c_pullback");
 }
 
-// ---- HIP graphs ----------------------------------------------------------------------
-struct aok_graph_s {
-    aok_ctx ctx;
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-};
 
 aok_status aok_graph_begin(aok_ctx ctx)
 {
@@ -753,14 +763,14 @@ aok_status aok_graph_end(aok_ctx ctx, aok_graph *graph)
         (void)hipGraphDestroy(g);
         return fail(ctx, AOK_ERR_NOMEM, "graph allocation failed");
     }
-    ++ctx->live_graphs;
+    ctx->graphs.push_back(gr);
     *graph = gr;
     return AOK_OK;
 }
 
 aok_status aok_graph_launch(aok_graph graph)
 {
-    if (!graph || !graph->ctx) return AOK_ERR_ARG;
+    if (!graph || !graph->ctx) return AOK_ERR_ARG;  // NULL or detached
     aok_ctx ctx = graph->ctx;
     AOK_REQUIRE(ctx, !in_capture(ctx), "cannot launch a graph into a stream under capture");
     AOK_HIP(ctx, hipSetDevice(ctx->device));
@@ -774,15 +784,21 @@ aok_status aok_graph_destroy(aok_graph graph)
 {
     if (!graph) return AOK_OK;
     aok_ctx ctx = graph->ctx;
-    if (ctx) {
-        (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);  // a launched replay may still run
+    if (!ctx) {  // detached by aok_destroy
+        delete graph;
+        return AOK_OK;
     }
-    (void)hipGraphExecDestroy(graph->exec);
-    (void)hipGraphDestroy(graph->graph);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);  // a launched replay may still run
+    detach_graph(graph);
+    auto &v = ctx->graphs;
+    for (size_t i = 0; i < v.size(); ++i)
+        if (v[i] == graph) {
+            v.erase(v.begin() + static_cast<std::ptrdiff_t>(i));
+            break;
+        }
     delete graph;
-    if (!ctx) return AOK_OK;
-    if (--ctx->live_graphs == 0) {
+    if (v.empty() && !in_capture(ctx)) {
         for (void *r : ctx->retired) (void)hipFree(r);
         ctx->retired.clear();
     }

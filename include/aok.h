@@ -263,7 +263,8 @@ aok_status aok_ovl3c_pullback(aok_ctx ctx, aok_dtype dtype, unsigned flags, cons
  * aok_graph_launch replays the whole recorded sequence as one launch on the context
  * stream; in blocking mode it waits and reports device-side errors like any call.
  * Workspaces the library outgrows while graphs are alive are kept until the last graph
- * of the context is destroyed. */
+ * of the context is destroyed.  Destroying the context first detaches its graphs: they
+ * then only accept aok_graph_destroy. */
 typedef struct aok_graph_s *aok_graph;
 aok_status aok_graph_begin(aok_ctx ctx);
 aok_status aok_graph_end(aok_ctx ctx, aok_graph *graph);

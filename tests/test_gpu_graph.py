@@ -91,4 +91,12 @@ def test_generic_capture_and_errors(dev):
     # still usable eagerly
     aok.batch_overlap_(out, basis, XA, XB, ctx=ctx)
     assert torch.equal(out, ref)
+    # destroying the context first detaches its graphs: replay refuses, destroy works
+    with ctx.capture() as gr2:
+        aok.batch_overlap_(out, basis, XA, XB, ctx=ctx)
+    gr2.keep(out, basis, XA, XB)
+    ctx.close()
+    with pytest.raises(aok.AOKArgumentError):
+        gr2.replay()
+    gr2.close()
     del ps
