@@ -342,8 +342,19 @@ __host__ __device__ inline AoPbLayout ao_pb_layout(int nRad, int K, int NZ)
     return y;
 }
 
+// Waves per SIMD the FP32 pullback is compiled for: at most 72 VGPRs, no spills for
+// L <= 3 (measured on nX = 2^24, 10 radials x 4 primitives, L = 3: 4 waves/SIMD at the
+// compiler's own 99 VGPRs 1.06-1.11 ms, 7 waves 0.89 ms with the grid matched to the
+// resident blocks).  FP64 and arbitrary orbital maps keep the compiler's choice.
 template <typename T, int L, bool GENERIC>
-__global__ __launch_bounds__(AO_PB_THREADS) void ao_pullback_kernel(
+constexpr int ao_pb_waves()
+{
+    return std::is_same<T, float>::value && L <= 3 && !GENERIC ? 7 : 1;
+}
+
+template <typename T, int L, bool GENERIC>
+__global__ __launch_bounds__(AO_PB_THREADS) __attribute__((amdgpu_waves_per_eu(ao_pb_waves<T, L, GENERIC>(), 8)))
+void ao_pullback_kernel(
     const T *__restrict__ X, const void *__restrict__ sidx, int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
     int nRad, int K, int NZ, T scale, int decay, AoRunTable runs, const T *__restrict__ dPin, T *__restrict__ dX,
     double *__restrict__ partial, int want_params, unsigned *__restrict__ err)
@@ -621,8 +632,24 @@ static aok_status launch_pb_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &ru
                               void *dD)
 {
     const int64_t ntiles = (a.nX + AO_PB_THREADS - 1) / AO_PB_THREADS;
-    const int grid = static_cast<int>(std::min<int64_t>(ntiles, ctx->tune.ao_grid > 0 ? ctx->tune.ao_grid : 2048));
     const AoPbLayout y = ao_pb_layout<T>(a.nRad, a.K, a.NZ);
+    const size_t shm = y.total;
+    auto kern = &ao_pullback_kernel<T, L, GENERIC>;
+    // persistent grid: every block resident at once (blocks per CU from the kernel's
+    // registers and LDS x CUs), or the ao_grid knob
+    int cap = ctx->tune.ao_grid;
+    if (cap <= 0 && ntiles > 0) {
+        if (ctx->num_cu <= 0)
+            AOK_HIP(ctx, hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        if (shm > 64 * 1024 && shm <= 160 * 1024)
+            AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
+        int nb = 0;
+        AOK_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(kern),
+                                                                  AO_PB_THREADS, shm));
+        cap = std::max(1, nb) * std::max(1, ctx->num_cu);
+    }
+    const int grid = static_cast<int>(std::min<int64_t>(ntiles, cap));
     const bool want = (dzeta != nullptr) || (dD != nullptr);
     if (grid <= 0) {
         if (want) {
@@ -634,7 +661,6 @@ static aok_status launch_pb_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &ru
         }
         return AOK_OK;
     }
-    const size_t shm = y.total;
     if (shm > 160 * 1024)
         return fail(ctx, AOK_ERR_UNSUPPORTED, "ao_pullback: basis too large for the LDS tile");
     const size_t part_bytes = align256(sizeof(double) * 2 * static_cast<size_t>(y.nitem) * grid);
@@ -642,7 +668,6 @@ static aok_status launch_pb_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &ru
     aok_status st = param_table<T>(ctx, a, part_bytes, gtab);
     if (st != AOK_OK) return st;
     double *partial = static_cast<double *>(ctx->ws);
-    auto kern = &ao_pullback_kernel<T, L, GENERIC>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));

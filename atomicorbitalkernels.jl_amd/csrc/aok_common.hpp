@@ -19,6 +19,7 @@ struct aok_graph_s;
 
 struct aok_context {
     int device = 0;
+    int num_cu = 0;  // compute units of the device (read on first use)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string last_error;
