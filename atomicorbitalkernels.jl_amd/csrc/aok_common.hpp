@@ -116,7 +116,7 @@ void time_end(aok_ctx ctx);
 void collect_times(aok_ctx ctx);
 
 // Device helpers ----------------------------------------------------------------
-// 1-based species index (int64 or int32) -> 0-based sigma, -1 for NULL input.
+// 1-based species index (int64 or int32) -> 0-based sigma; NULL input means species 1.
 __device__ __forceinline__ int load_sigma(const void *sidx, int itype, int64_t i)
 {
     if (sidx == nullptr) return 0;
@@ -129,60 +129,5 @@ __device__ __forceinline__ void store_nt(T *p, T v)
 {
     __builtin_nontemporal_store(v, p);
 }
-
-template <typename T>
-__device__ __forceinline__ T fast_exp_neg(T x);  // exp(-x), see specialisations
-
-// exp(-x) for float via the hardware base-2 exponential (v_exp_f32) on x*log2(e).
-template <>
-__device__ __forceinline__ float fast_exp_neg<float>(float x)
-{
-    return __builtin_amdgcn_exp2f(-x * 1.4426950408889634f);
-}
-template <>
-__device__ __forceinline__ double fast_exp_neg<double>(double x)
-{
-    return exp(-x);
-}
-
-// Wave64 sums with DPP lane moves (no LDS traffic): an inclusive scan inside each row of
-// 16 lanes (row_shr 1, 2, 4, 8; lanes whose source falls outside the row read 0), then
-// row_bcast:15 adds each even row's total into the next row and row_bcast:31 adds lanes
-// 0-31's total into rows 2-3, so lane 63 holds the sum; readlane makes it wave-uniform.
-// Every lane of the wave must be active.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_take(float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_take(double v)
-{
-    const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(u), CTRL, ROWS, 0xF, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(u >> 32), CTRL, ROWS, 0xF, true);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<long long>(static_cast<unsigned>(lo)));
-}
-__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
-__device__ __forceinline__ double lane63(double v)
-{
-    const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(u), 63);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), 63);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<long long>(static_cast<unsigned>(lo)));
-}
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
-{
-    v += dpp_take<0x111, 0xF>(v);  // row_shr:1
-    v += dpp_take<0x112, 0xF>(v);  // row_shr:2
-    v += dpp_take<0x114, 0xF>(v);  // row_shr:4
-    v += dpp_take<0x118, 0xF>(v);  // row_shr:8
-    v += dpp_take<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_take<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-    return lane63(v);
-}
-
-inline int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
 
 }  // namespace aok
