@@ -360,7 +360,8 @@ __global__ __launch_bounds__(64) void ovl3c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename Rec3Of<T>::type *__restrict__ rec,
                                                      const int64_t *__restrict__ grp_off,
-                                                     const int *__restrict__ nreal, T *__restrict__ out, int maxtiles)
+                                                     const int *__restrict__ nreal, T *__restrict__ out, int maxtiles,
+                                                     int part)
 {
     const int ns = sh.ns, NZ = sh.NZ, N = sh.N, NK = NZ * NZ * NZ;
     // (tile, shell pair (s_c, s_b)) of the block, tile fastest
@@ -397,8 +398,12 @@ __global__ __launch_bounds__(64) void ovl3c_fwd_fast(const OvlShape sh, const T 
     const int sc = cb / ns, sb = cb - sc * ns;
     const int lc = sh.ls[sc], nc = nreal[sgC * ns + sc];
     const int lb = sh.ls[sb], nb = nreal[sgB * ns + sb];
+    // part 1: only triples of s and p shells; part 2: only triples with a d shell; 0: all
+    const bool light_bc = lb <= 1 && lc <= 1;
+    if (part == 1 && !light_bc) return;
     for (int sa = 0; sa < ns; ++sa) {
         const int la = sh.ls[sa], na = nreal[sgA * ns + sa];
+        if ((part == 1 && la > 1) || (part == 2 && light_bc && la <= 1)) continue;
         const int64_t g = ((static_cast<int64_t>(key) * ns + sc) * ns + sb) * ns + sa;
         const typename Rec3Of<T>::type *recs = rec + grp_off[g];
 #define AOK_F3_CASE(A_, B_, C_)                                                                                 \
@@ -788,15 +793,26 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.lidx),
                        reinterpret_cast<const int64_t *>(ws + w.grp_off), reinterpret_cast<R *>(ws + w.rec));
     AOK_HIP(ctx, hipGetLastError());
+    // With d shells, the triples of s and p shells alone run in the l <= 1 instance (62
+    // VGPRs, 8 waves/SIMD) and the rest in the l <= 2 one (its (d, d, d) case holds 216
+    // accumulators: 256 VGPRs, 1 wave/SIMD).
     time_begin(ctx, "ovl3c_fwd_fast");
     const int grid = w.maxtiles * sh.ns * sh.ns;
-    hipLaunchKernelGGL((ovl3c_fwd_fast<T, LMAXB>), dim3(grid), dim3(OVL_TILE), 0, ctx->stream, sh,
-                       static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<const T *>(c.posC),
-                       static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
-                       reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
-                       reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
-                       reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int64_t *>(ws + w.grp_off),
-                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), w.maxtiles);
+    auto launch = [&](auto kern, int part) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(OVL_TILE), 0, ctx->stream, sh, static_cast<const T *>(c.posA),
+                           static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
+                           reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
+                           reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
+                           reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
+                           reinterpret_cast<const int64_t *>(ws + w.grp_off),
+                           reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), w.maxtiles, part);
+    };
+    if constexpr (LMAXB == 2) {
+        launch(&ovl3c_fwd_fast<T, 1>, 1);
+        launch(&ovl3c_fwd_fast<T, 2>, 2);
+    } else {
+        launch(&ovl3c_fwd_fast<T, LMAXB>, 0);
+    }
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
