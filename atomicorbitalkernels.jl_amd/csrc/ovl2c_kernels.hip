@@ -918,7 +918,7 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
 {
     __shared__ int rel_ci[OVL_MAXCH], rel_col[OVL_MAXCH];
     __shared__ int wcnt[4];
-    __shared__ double red[256];
+    __shared__ double red[8];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, nchunk = sh.nchunk;
     const int pstride = (ns + 1) * K * 2;
     const int64_t npar = static_cast<int64_t>(NZ) * ns * K;
@@ -963,8 +963,7 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
         c += pt[0];
         a += pt[1];
     }
-    c = block_sum256(c, red);
-    a = block_sum256(a, red);
+    block_sum256x2(c, a, red);
     if (threadIdx.x == 0) {
         const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
         part[blockIdx.y * npar + dst] = c;

@@ -735,7 +735,7 @@ __global__ __launch_bounds__(256) void ovl3_bwd_reduce(const OvlShape sh, int64_
                                                        const double *__restrict__ rows, T *__restrict__ dcoef,
                                                        T *__restrict__ dalpha)
 {
-    __shared__ double red[256];
+    __shared__ double red[8];
     const int ns = sh.ns, K = sh.K;
     const int q = blockIdx.x;  // [s + ns * (k + K * sg)], the Julia index
     const int s = q % ns, k = (q / ns) % K, sg = q / (ns * K);
@@ -750,8 +750,7 @@ __global__ __launch_bounds__(256) void ovl3_bwd_reduce(const OvlShape sh, int64_
             c += r[0];
             a += r[1];
         }
-    c = block_sum256(c, red);
-    a = block_sum256(a, red);
+    block_sum256x2(c, a, red);
     if (threadIdx.x == 0) {
         dcoef[q] = static_cast<T>(c);
         dalpha[q] = static_cast<T>(a);

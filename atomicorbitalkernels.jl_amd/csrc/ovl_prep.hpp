@@ -29,19 +29,26 @@ __device__ __forceinline__ int64_t wave_incl_scan64(int64_t v)
     return v;
 }
 
-// Fixed-order sum of one double per thread over a 256-thread block (LDS tree); the
-// result is returned to every thread.  `red` holds 256 doubles.
-__device__ __forceinline__ double block_sum256(double v, double *red)
+// Fixed-order sums of two doubles per thread over a 256-thread block, returned to every
+// thread: a butterfly inside each wave (lanes i and i ^ d add the same two operands, so
+// every lane holds the identical wave total), then the four wave totals in wave order.
+// `red` holds 8 doubles.
+__device__ __forceinline__ void block_sum256x2(double &c, double &a, double *red)
 {
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (static_cast<int>(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        c += __shfl_xor(c, d, 64);
+        a += __shfl_xor(a, d, 64);
     }
-    const double r = red[0];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave] = c;
+        red[4 + wave] = a;
+    }
     __syncthreads();
-    return r;
+    c = (red[0] + red[1]) + (red[2] + red[3]);
+    a = (red[4] + red[5]) + (red[6] + red[7]);
+    __syncthreads();
 }
 
 // Exclusive prefix sum over a block of whole waves; `wtot` holds one int64 per wave.
