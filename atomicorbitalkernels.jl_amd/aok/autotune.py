@@ -10,7 +10,8 @@ without timing again (``TuneTable.apply``).
 Knobs (include/aok.h, aok_tune):
   * 2-centre overlaps: ``ovl_cap`` (LDS stripe capacity per lane, i.e. how many bra
     shells share one (tile, chunk) workgroup), ``ovl_split`` (light chunks of bases with
-    f/g shells in the l <= 2 kernel instance or not);
+    f/g shells in the l <= 2 kernel instance or not), ``ovl_order`` (tile-major or
+    XCD-grouped workgroup order);
   * AO evaluation: ``ao_ppl`` (consecutive points per lane), ``ao_grid`` (persistent grid).
 
 The 2-centre knobs never change S (each element is computed by one lane in a fixed
@@ -30,7 +31,7 @@ import torch
 
 from . import _lib
 
-OVL_SPACE = {"ovl_cap": [24, 32, 40, 56, 80, 112], "ovl_split": [1, 2]}
+OVL_SPACE = {"ovl_cap": [24, 32, 40, 56, 80, 112], "ovl_split": [1, 2], "ovl_order": [1, 2]}
 AO_SPACE = {"ao_ppl": [1, 2, 4], "ao_grid": [0, 1024, 2048, 4096]}
 
 

@@ -275,6 +275,9 @@ aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
     } else if (k == "ovl_cap") {
         AOK_REQUIRE(ctx, value == 0 || (value >= 16 && value <= 1024), "ovl_cap must be 0 or in 16..1024");
         ctx->tune.ovl_cap = static_cast<int>(value);
+    } else if (k == "ovl_order") {
+        AOK_REQUIRE(ctx, value >= 0 && value <= 2, "ovl_order must be 0, 1 or 2");
+        ctx->tune.ovl_order = static_cast<int>(value);
     } else if (k == "ovl_split") {
         AOK_REQUIRE(ctx, value >= 0 && value <= 2, "ovl_split must be 0, 1 or 2");
         ctx->tune.ovl_split = static_cast<int>(value);
@@ -293,6 +296,7 @@ aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value)
     else if (k == "ao_grid") *value = ctx->tune.ao_grid;
     else if (k == "ovl_cap") *value = ctx->tune.ovl_cap;
     else if (k == "ovl_split") *value = ctx->tune.ovl_split;
+    else if (k == "ovl_order") *value = ctx->tune.ovl_order;
     else return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
     return AOK_OK;
 }

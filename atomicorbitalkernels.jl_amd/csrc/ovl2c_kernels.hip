@@ -611,6 +611,22 @@ struct StripeWalk {
 // ---------------------------------------------------------------------------------
 // Tile bookkeeping shared by the fast kernels
 // ---------------------------------------------------------------------------------
+// Workgroup -> (tile, chunk).  Tile-major grid (tiles, chunks), order 0: consecutive
+// workgroups take consecutive tiles of one chunk, so the waves resident on a CU run the
+// same shell-pair code and records (small blocks).  XCD-grouped grid (8 x chunks,
+// tiles / 8), order 1: workgroups go to the 8 XCDs round-robin by linear id, so tile
+// 8 y + x % 8 runs all its chunks on XCD x % 8 one after another and they share its L2
+// (the pairs' cotangent lines, the output block's partial lines) -- large blocks.
+__device__ __forceinline__ void tile_chunk(int order, int chunk0, int &tile, int &chunk)
+{
+    if (order == 1) {
+        tile = static_cast<int>(blockIdx.y) * 8 + static_cast<int>(blockIdx.x & 7u);
+        chunk = static_cast<int>(blockIdx.x >> 3) + chunk0;
+    } else {
+        tile = static_cast<int>(blockIdx.x);
+        chunk = static_cast<int>(blockIdx.y) + chunk0;
+    }
+}
 struct TileInfo {
     int key, first, nact;
 };
@@ -636,7 +652,8 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ boff, const int *__restrict__ bcnt,
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
-                                                     const int *__restrict__ nreal, T *__restrict__ out, int chunk0)
+                                                     const int *__restrict__ nreal, T *__restrict__ out, int chunk0,
+                                                     int order)
 {
     extern __shared__ unsigned char smem_raw[];
     // LDS: the lanes' block offsets b * N * N, the column-segment offsets of the chunk's
@@ -646,7 +663,8 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     T *stripe = reinterpret_cast<T *>(smem_raw + 8 * OVL_TILE + 4 * ((sh.str + 3) & ~3));
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     TileInfo ti;
-    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y) + chunk0;
+    int tile, chunk;
+    tile_chunk(order, chunk0, tile, chunk);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
@@ -712,7 +730,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
                                                      const int *__restrict__ nreal, const int *__restrict__ nlist,
-                                                     double *__restrict__ partial, int chunk0)
+                                                     double *__restrict__ partial, int chunk0, int order)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -726,7 +744,8 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     T *stripe = reinterpret_cast<T *>(accA + pstride);
     T *lac = DIRECT ? stripe : stripe + OVL_TILE * sh.str;  // per-lane sums [4 * K][LROW]
     TileInfo ti;
-    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y) + chunk0;
+    int tile, chunk;
+    tile_chunk(order, chunk0, tile, chunk);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     for (int q = lane; q < pstride; q += OVL_TILE) accA[q] = 0.0;
@@ -805,7 +824,7 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
                                                     const int *__restrict__ toff,
                                                     const typename RecOf<T>::type *__restrict__ rec,
                                                     const int *__restrict__ nreal, double *__restrict__ partial,
-                                                    int chunk0)
+                                                    int chunk0, int order)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -814,7 +833,8 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
     // as the pullback: FP32 with shells up to d reads each shell pair's block into registers
     constexpr bool DIRECT = bwd_direct<T, LMAXB>();
     TileInfo ti;
-    const int tile = static_cast<int>(blockIdx.x), chunk = static_cast<int>(blockIdx.y) + chunk0;
+    int tile, chunk;
+    tile_chunk(order, chunk0, tile, chunk);
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
@@ -1284,6 +1304,21 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
     return AOK_OK;
 }
 
+// Workgroup order of the fast kernels (tile_chunk): the ovl_order knob, else XCD-grouped
+// for output blocks above 32 x 32 (measured: C2, N = 50, forward 3.0 -> 1.8 ms and
+// pullback 7.6 -> 6.6 ms; C1, N = 15, is faster tile-major).
+static int grid_order(aok_ctx ctx, const OvlShape &sh)
+{
+    if (ctx->tune.ovl_order > 0) return ctx->tune.ovl_order - 1;
+    return sh.N > 32 ? 1 : 0;
+}
+
+static dim3 launch_grid(aok_ctx ctx, const OvlShape &sh, const Ovl2cWs &w, int nch)
+{
+    if (grid_order(ctx, sh) == 1) return dim3(8 * nch, (w.maxtiles + 7) / 8);
+    return dim3(w.maxtiles, nch);
+}
+
 template <typename T, int LMAXB>
 static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws,
                                   int chunk0, int nch)
@@ -1295,12 +1330,13 @@ static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, launch_grid(ctx, sh, w, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
-                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), chunk0);
+                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), chunk0,
+                       grid_order(ctx, sh));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
@@ -1317,14 +1353,14 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, launch_grid(ctx, sh, w, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.dS),
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.nlist),
-                       reinterpret_cast<double *>(ws + w.partial), chunk0);
+                       reinterpret_cast<double *>(ws + w.partial), chunk0, grid_order(ctx, sh));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
@@ -1440,13 +1476,13 @@ static aok_status fast_dx_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    hipLaunchKernelGGL(kern, dim3(w.maxtiles, nch), dim3(OVL_TILE), shm, ctx->stream, sh, c.B,
+    hipLaunchKernelGGL(kern, launch_grid(ctx, sh, w, nch), dim3(OVL_TILE), shm, ctx->stream, sh, c.B,
                        static_cast<const T *>(c.dS), static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
                        static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
                        reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
                        reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
                        reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int *>(ws + w.nreal),
-                       partial, chunk0);
+                       partial, chunk0, grid_order(ctx, sh));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }

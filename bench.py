@@ -398,8 +398,8 @@ def main():
             "B": B2, "N": cb2.N, "NZ": cb2.NZ, "step_ms": wall2, "pairs_per_s": B2 / (wall2 * 1e-3),
             "graph_step_ms": gwall2, "graph_pairs_per_s": B2 / (gwall2 * 1e-3),
             "autotune": {"knobs": tknobs, "best_ms": tms,
-                         "candidates": [[k["ovl_cap"], k["ovl_split"], round(m, 4)] for k, m in tres],
-                         "candidate_fields": ["ovl_cap", "ovl_split", "step_ms"]},
+                         "candidates": [[k["ovl_cap"], k["ovl_split"], k["ovl_order"], round(m, 4)] for k, m in tres],
+                         "candidate_fields": ["ovl_cap", "ovl_split", "ovl_order", "step_ms"]},
             "fwd_kernel_ms": float(np.mean(fk)), "bwd_kernel_ms": float(np.mean(bk)),
             "fwd_hermite_model_TFLOPs": f2f / (np.mean(fk) * 1e-3) / 1e12, "bwd_hermite_model_TFLOPs": f2b / (np.mean(bk) * 1e-3) / 1e12,
             "fp32_blockwise_max_rel_err": err2, "plain_f64_fwd_kernel_ms": plain2,
@@ -407,6 +407,7 @@ def main():
         del S2, dS2, pb2
         ctx.tune("ovl_cap", 0)
         ctx.tune("ovl_split", 0)
+        ctx.tune("ovl_order", 0)
         # 3-centre overlaps, B = 2^14 triples over the same 3 species, FP32
         log("3c B=2^14")
         c4 = configs.c4a()
