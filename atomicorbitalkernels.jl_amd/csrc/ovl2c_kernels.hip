@@ -1316,7 +1316,7 @@ static int grid_order(aok_ctx ctx, const OvlShape &sh)
 static dim3 launch_grid(aok_ctx ctx, const OvlShape &sh, const Ovl2cWs &w, int nch)
 {
     if (grid_order(ctx, sh) == 1) return dim3(8 * nch, (w.maxtiles + 7) / 8);
-    return dim3(w.maxtiles, nch);
+    return dim3((w.maxtiles + 7) & ~7, nch);
 }
 
 template <typename T, int LMAXB>
