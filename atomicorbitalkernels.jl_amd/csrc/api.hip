@@ -276,7 +276,7 @@ aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
         AOK_REQUIRE(ctx, value == 0 || (value >= 16 && value <= 1024), "ovl_cap must be 0 or in 16..1024");
         ctx->tune.ovl_cap = static_cast<int>(value);
     } else if (k == "ovl_order") {
-        AOK_REQUIRE(ctx, value >= 0 && value <= 2, "ovl_order must be 0, 1 or 2");
+        AOK_REQUIRE(ctx, value >= 0 && value <= OVL_MAXCH + 1, "ovl_order must be in 0..161");
         ctx->tune.ovl_order = static_cast<int>(value);
     } else if (k == "ovl_split") {
         AOK_REQUIRE(ctx, value >= 0 && value <= 2, "ovl_split must be 0, 1 or 2");

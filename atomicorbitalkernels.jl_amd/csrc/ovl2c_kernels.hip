@@ -611,21 +611,22 @@ struct StripeWalk {
 // ---------------------------------------------------------------------------------
 // Tile bookkeeping shared by the fast kernels
 // ---------------------------------------------------------------------------------
-// Workgroup -> (tile, chunk).  Tile-major grid (tiles, chunks), order 0: consecutive
-// workgroups take consecutive tiles of one chunk, so the waves resident on a CU run the
-// same shell-pair code and records (small blocks).  XCD-grouped grid (8 x chunks,
-// tiles / 8), order 1: workgroups go to the 8 XCDs round-robin by linear id, so tile
-// 8 y + x % 8 runs all its chunks on XCD x % 8 one after another and they share its L2
-// (the pairs' cotangent lines, the output block's partial lines) -- large blocks.
-__device__ __forceinline__ void tile_chunk(int order, int chunk0, int &tile, int &chunk)
+// Workgroup -> (tile, chunk), grid (8 G, tiles/8 x chunk groups): the 8 XCDs receive
+// workgroups round-robin by linear id, so tile 8 y + x % 8 always runs on XCD x % 8, and
+// within a group of G consecutive chunks its workgroups follow one another.  G = 1: the
+// workgroups resident on a CU run one chunk's shell-pair code and records (small
+// blocks); G = all chunks: a tile's chunks back to back share the XCD's L2 (the pairs'
+// cotangent lines, the output block's partial lines) -- large blocks.  Returns false for
+// the padding workgroups of the last group.
+__device__ __forceinline__ bool tile_chunk(int chunk0, int nch, int group, int &tile, int &chunk)
 {
-    if (order == 1) {
-        tile = static_cast<int>(blockIdx.y) * 8 + static_cast<int>(blockIdx.x & 7u);
-        chunk = static_cast<int>(blockIdx.x >> 3) + chunk0;
-    } else {
-        tile = static_cast<int>(blockIdx.x);
-        chunk = static_cast<int>(blockIdx.y) + chunk0;
-    }
+    const int ngroups = (nch + group - 1) / group;
+    const int t8 = static_cast<int>(gridDim.y) / ngroups;
+    const int ty = static_cast<int>(blockIdx.y) % t8, grp = static_cast<int>(blockIdx.y) / t8;
+    tile = ty * 8 + static_cast<int>(blockIdx.x & 7u);
+    const int c = grp * group + static_cast<int>(blockIdx.x >> 3);
+    chunk = c + chunk0;
+    return c < nch;
 }
 struct TileInfo {
     int key, first, nact;
@@ -653,7 +654,7 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
                                                      const int *__restrict__ nreal, T *__restrict__ out, int chunk0,
-                                                     int order)
+                                                     int nch, int group)
 {
     extern __shared__ unsigned char smem_raw[];
     // LDS: the lanes' block offsets b * N * N, the column-segment offsets of the chunk's
@@ -664,7 +665,7 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     TileInfo ti;
     int tile, chunk;
-    tile_chunk(order, chunk0, tile, chunk);
+    if (!tile_chunk(chunk0, nch, group, tile, chunk)) return;
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
@@ -730,7 +731,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
                                                      const int *__restrict__ nreal, const int *__restrict__ nlist,
-                                                     double *__restrict__ partial, int chunk0, int order)
+                                                     double *__restrict__ partial, int chunk0, int nch, int group)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     T *lac = DIRECT ? stripe : stripe + OVL_TILE * sh.str;  // per-lane sums [4 * K][LROW]
     TileInfo ti;
     int tile, chunk;
-    tile_chunk(order, chunk0, tile, chunk);
+    if (!tile_chunk(chunk0, nch, group, tile, chunk)) return;
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     for (int q = lane; q < pstride; q += OVL_TILE) accA[q] = 0.0;
@@ -824,7 +825,7 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
                                                     const int *__restrict__ toff,
                                                     const typename RecOf<T>::type *__restrict__ rec,
                                                     const int *__restrict__ nreal, double *__restrict__ partial,
-                                                    int chunk0, int order)
+                                                    int chunk0, int nch, int group)
 {
     extern __shared__ unsigned char smem_raw[];
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -834,7 +835,7 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
     constexpr bool DIRECT = bwd_direct<T, LMAXB>();
     TileInfo ti;
     int tile, chunk;
-    tile_chunk(order, chunk0, tile, chunk);
+    if (!tile_chunk(chunk0, nch, group, tile, chunk)) return;
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
@@ -1304,19 +1305,22 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
     return AOK_OK;
 }
 
-// Workgroup order of the fast kernels (tile_chunk): the ovl_order knob, else XCD-grouped
-// for output blocks above 32 x 32 (measured: C2, N = 50, forward 3.0 -> 1.8 ms and
-// pullback 7.6 -> 6.6 ms; C1, N = 15, is faster tile-major).
-static int grid_order(aok_ctx ctx, const OvlShape &sh)
+// Chunk group of the fast kernels' workgroup order (tile_chunk): the ovl_order knob (1:
+// one chunk, 2: all chunks of a launch), else all chunks for output blocks above 32 x 32
+// (measured: C2, N = 50, forward 3.0 -> 1.8 ms and pullback 7.6 -> 6.6 ms; C1, N = 15,
+// is faster one chunk at a time).
+static int grid_group(aok_ctx ctx, const OvlShape &sh, int nch)
 {
-    if (ctx->tune.ovl_order > 0) return ctx->tune.ovl_order - 1;
-    return sh.N > 32 ? 1 : 0;
+    const int o = ctx->tune.ovl_order > 0 ? ctx->tune.ovl_order : (sh.N > 32 ? 2 : 1);
+    if (o == 1) return 1;
+    if (o == 2) return nch;
+    return std::min(nch, o - 1);  // 3, 4, ...: groups of 2, 3, ... chunks
 }
 
 static dim3 launch_grid(aok_ctx ctx, const OvlShape &sh, const Ovl2cWs &w, int nch)
 {
-    if (grid_order(ctx, sh) == 1) return dim3(8 * nch, (w.maxtiles + 7) / 8);
-    return dim3((w.maxtiles + 7) & ~7, nch);
+    const int g = grid_group(ctx, sh, nch);
+    return dim3(8 * g, (w.maxtiles + 7) / 8 * ((nch + g - 1) / g));
 }
 
 template <typename T, int LMAXB>
@@ -1335,8 +1339,8 @@ static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
-                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), chunk0,
-                       grid_order(ctx, sh));
+                       reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), chunk0, nch,
+                       grid_group(ctx, sh, nch));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
@@ -1360,7 +1364,7 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.nlist),
-                       reinterpret_cast<double *>(ws + w.partial), chunk0, grid_order(ctx, sh));
+                       reinterpret_cast<double *>(ws + w.partial), chunk0, nch, grid_group(ctx, sh, nch));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
@@ -1482,7 +1486,7 @@ static aok_status fast_dx_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape
                        reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
                        reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
                        reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int *>(ws + w.nreal),
-                       partial, chunk0, grid_order(ctx, sh));
+                       partial, chunk0, nch, grid_group(ctx, sh, nch));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }

@@ -280,10 +280,10 @@ aok_status aok_graph_destroy(aok_graph graph);
  *   "ovl_split" 2-centre pullback / position cotangents of bases with f or g shells: run
  *              the chunks of s, p, d shells in the kernel instance compiled for l <= 2
  *              (0 = default = 1; 2 = every chunk in the instance for the largest l)
- *   "ovl_order" 2-centre fast kernels' workgroup order: 1 tile-major (consecutive
- *              workgroups take consecutive tiles of one stripe chunk), 2 XCD-grouped (each
- *              tile's chunks run on one XCD one after another, sharing its L2); 0 = default:
- *              XCD-grouped when N > 32
+ *   "ovl_order" 2-centre fast kernels' workgroup order (every tile's workgroups run on one
+ *              XCD): 1 tile-major (consecutive workgroups take consecutive tiles of one
+ *              stripe chunk), 2 XCD-grouped (a tile's chunks one after another, sharing the
+ *              XCD's L2), k >= 3 groups of k - 1 chunks; 0 = default: 2 when N > 32, else 1
  * Unknown keys or out-of-range values return AOK_ERR_ARG. */
 aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value);
 aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value);
