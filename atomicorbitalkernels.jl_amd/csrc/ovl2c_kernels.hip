@@ -611,8 +611,8 @@ struct StripeWalk {
 // ---------------------------------------------------------------------------------
 // Tile bookkeeping shared by the fast kernels
 // ---------------------------------------------------------------------------------
-// Workgroup -> (tile, chunk), grid (8 G, tiles/8 x chunk groups): the 8 XCDs receive
-// workgroups round-robin by linear id, so tile 8 y + x % 8 always runs on XCD x % 8, and
+// Workgroup -> (tile, chunk), grid (8 G x tiles/8, chunk groups): the 8 XCDs receive
+// workgroups round-robin by linear id, so tile 8 q + x % 8 always runs on XCD x % 8, and
 // within a group of G consecutive chunks its workgroups follow one another.  G = 1: the
 // workgroups resident on a CU run one chunk's shell-pair code and records (small
 // blocks); G = all chunks: a tile's chunks back to back share the XCD's L2 (the pairs'
@@ -620,11 +620,10 @@ struct StripeWalk {
 // the padding workgroups of the last group.
 __device__ __forceinline__ bool tile_chunk(int chunk0, int nch, int group, int &tile, int &chunk)
 {
-    const int ngroups = (nch + group - 1) / group;
-    const int t8 = static_cast<int>(gridDim.y) / ngroups;
-    const int ty = static_cast<int>(blockIdx.y) % t8, grp = static_cast<int>(blockIdx.y) / t8;
-    tile = ty * 8 + static_cast<int>(blockIdx.x & 7u);
-    const int c = grp * group + static_cast<int>(blockIdx.x >> 3);
+    const int r = static_cast<int>(blockIdx.x >> 3);  // q * group + chunk in group
+    const int q = r / group, gi = r - q * group;
+    tile = q * 8 + static_cast<int>(blockIdx.x & 7u);
+    const int c = static_cast<int>(blockIdx.y) * group + gi;
     chunk = c + chunk0;
     return c < nch;
 }
@@ -1306,21 +1305,21 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
 }
 
 // Chunk group of the fast kernels' workgroup order (tile_chunk): the ovl_order knob (1:
-// one chunk, 2: all chunks of a launch), else all chunks for output blocks above 32 x 32
-// (measured: C2, N = 50, forward 3.0 -> 1.8 ms and pullback 7.6 -> 6.6 ms; C1, N = 15,
-// is faster one chunk at a time).
-static int grid_group(aok_ctx ctx, const OvlShape &sh, int nch)
+// one chunk, 2: all chunks of a launch, k >= 3: k - 1), else all chunks for output blocks
+// above 32 x 32 (measured: C2, N = 50, forward 3.0 -> 1.8 ms and pullback 7.6 -> 6.6 ms;
+// C1, N = 15, is faster one chunk at a time).  Grids too large for groups use G = 1.
+static int grid_group(aok_ctx ctx, const OvlShape &sh, const Ovl2cWs &w, int nch)
 {
     const int o = ctx->tune.ovl_order > 0 ? ctx->tune.ovl_order : (sh.N > 32 ? 2 : 1);
-    if (o == 1) return 1;
-    if (o == 2) return nch;
-    return std::min(nch, o - 1);  // 3, 4, ...: groups of 2, 3, ... chunks
+    const int g = o == 1 ? 1 : (o == 2 ? nch : std::min(nch, o - 1));
+    const int64_t t8 = (w.maxtiles + 7) / 8;
+    return 8 * static_cast<int64_t>(g) * t8 <= INT32_MAX ? g : 1;
 }
 
-static dim3 launch_grid(aok_ctx ctx, const OvlShape &sh, const Ovl2cWs &w, int nch)
+static dim3 launch_grid(const Ovl2cWs &w, int nch, int g)
 {
-    const int g = grid_group(ctx, sh, nch);
-    return dim3(8 * g, (w.maxtiles + 7) / 8 * ((nch + g - 1) / g));
+    const unsigned t8 = static_cast<unsigned>((w.maxtiles + 7) / 8);
+    return dim3(8u * static_cast<unsigned>(g) * t8, static_cast<unsigned>((nch + g - 1) / g));
 }
 
 template <typename T, int LMAXB>
@@ -1329,18 +1328,19 @@ static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
 {
     if (nch <= 0) return AOK_OK;
     using R = typename RecOf<T>::type;
+    const int grp = grid_group(ctx, sh, w, nch);
     const size_t shm = 8 * OVL_TILE + 4 * ((sh.str + 3) & ~3) + sizeof(T) * OVL_TILE * sh.str;
     auto kern = &ovl2c_fwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    hipLaunchKernelGGL(kern, launch_grid(ctx, sh, w, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, launch_grid(w, nch, grp), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                        reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), chunk0, nch,
-                       grid_group(ctx, sh, nch));
+                       grp);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
@@ -1351,20 +1351,21 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
 {
     if (nch <= 0) return AOK_OK;
     using R = typename RecOf<T>::type;
+    const int grp = grid_group(ctx, sh, w, nch);
     const size_t shm = 256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 +
                        sizeof(T) * ((bwd_direct<T, LMAXB>() ? 0 : OVL_TILE * sh.str) + 4 * sh.K * LROW);
     auto kern = &ovl2c_bwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    hipLaunchKernelGGL(kern, launch_grid(ctx, sh, w, nch), dim3(OVL_TILE), shm, ctx->stream, sh,
+    hipLaunchKernelGGL(kern, launch_grid(w, nch, grp), dim3(OVL_TILE), shm, ctx->stream, sh,
                        static_cast<const T *>(c.dS),
                        static_cast<const T *>(c.posA), static_cast<const T *>(c.posB), static_cast<T>(c.scale),
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.nlist),
-                       reinterpret_cast<double *>(ws + w.partial), chunk0, nch, grid_group(ctx, sh, nch));
+                       reinterpret_cast<double *>(ws + w.partial), chunk0, nch, grp);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
@@ -1475,18 +1476,19 @@ static aok_status fast_dx_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape
 {
     if (nch <= 0) return AOK_OK;
     using R = typename RecOf<T>::type;
+    const int grp = grid_group(ctx, sh, w, nch);
     const size_t shm = 256 + (bwd_direct<T, LMAXB>() ? 0 : sizeof(T) * OVL_TILE * sh.str);
     auto kern = &ovl2c_dx_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
-    hipLaunchKernelGGL(kern, launch_grid(ctx, sh, w, nch), dim3(OVL_TILE), shm, ctx->stream, sh, c.B,
+    hipLaunchKernelGGL(kern, launch_grid(w, nch, grp), dim3(OVL_TILE), shm, ctx->stream, sh, c.B,
                        static_cast<const T *>(c.dS), static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
                        static_cast<T>(c.scale), reinterpret_cast<const int *>(ws + w.perm),
                        reinterpret_cast<const int *>(ws + w.tile_key), reinterpret_cast<const int *>(ws + w.boff),
                        reinterpret_cast<const int *>(ws + w.bcnt), reinterpret_cast<const int *>(ws + w.toff),
                        reinterpret_cast<const R *>(ws + w.rec), reinterpret_cast<const int *>(ws + w.nreal),
-                       partial, chunk0, nch, grid_group(ctx, sh, nch));
+                       partial, chunk0, nch, grp);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
