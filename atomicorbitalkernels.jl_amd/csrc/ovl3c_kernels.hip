@@ -839,6 +839,10 @@ aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c)
     const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, 1 << 20, sh);
     if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
     if (c.B < 0 || c.B > (int64_t(1) << 28)) return fail(ctx, AOK_ERR_ARG, "B out of range");
+    // one workgroup per (64-triple tile, shell pair), or 128 items of (triple, shell triple)
+    const int64_t ns = sh.ns;
+    if (((c.B + OVL_TILE - 1) / OVL_TILE + 512) * ns * ns > INT32_MAX || (c.B * ns * ns * ns + 127) / 128 > INT32_MAX)
+        return fail(ctx, AOK_ERR_ARG, "B x shells^3 too large for one call: split the batch");
     int lmax = 0;
     for (int q = 0; q < sh.ns; ++q) lmax = std::max<int>(lmax, sh.ls[q]);
     const bool fast = !(c.flags & AOK_PLAIN) && lmax <= OVL3_FAST_LMAX && sh.NZ <= OVL3_MAXNZ && sh.ns <= 24;
