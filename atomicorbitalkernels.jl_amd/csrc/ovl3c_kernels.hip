@@ -590,6 +590,21 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     double *rA = row + 2 * (static_cast<int64_t>(0 * ns + sa) * K);
     double *rB = row + 2 * (static_cast<int64_t>(1 * ns + sb) * K);
     double *rC = row + 2 * (static_cast<int64_t>(2 * ns + sc) * K);
+    // small blocks: the cotangent block of this shell triple in registers, read once
+    constexpr int NBC = Sh3<LC>::nbf, NG = NBA * NBB * NBC;
+    constexpr bool GREG = NG <= 27;
+    T gr[GREG ? NG : 1];
+    if constexpr (GREG) {
+#pragma unroll
+        for (int kk = 0; kk < NBC; ++kk)
+#pragma unroll
+            for (int jj = 0; jj < NBB; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < NBA; ++ii)
+                    gr[ii + NBA * (jj + NBB * kk)] =
+                        g[(sh.off[sa] + ii) +
+                          static_cast<int64_t>(N) * ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
+    }
     for (int i = 0; i < nla; ++i)
         for (int j = 0; j < nlb; ++j)
             for (int k = 0; k < nlc; ++k) {
@@ -641,9 +656,13 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                                         const int m3 = LC - ll3 - n3;
                                         const int lin = index1 + N1 * index2 + N1 * N2 * index3;
                                         const int ii = lin % NBA, jj = (lin / NBA) % NBB, kk = lin / (NBA * NBB);
-                                        const T gv = g[(sh.off[sa] + ii) +
-                                                       static_cast<int64_t>(N) *
-                                                           ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
+                                        T gv;
+                                        if constexpr (GREG)
+                                            gv = gr[ii + NBA * (jj + NBB * kk)];
+                                        else
+                                            gv = g[(sh.off[sa] + ii) +
+                                                   static_cast<int64_t>(N) *
+                                                       ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
                                         const T x = sx[ll1][ll2][ll3], y = sy[m1][m2][m3], z = sz[n1][n2][n3];
                                         H += gv * (x * y * z);
                                         Ha += gv * (sx[ll1 + 2][ll2][ll3] * y * z + x * sy[m1 + 2][m2][m3] * z +
