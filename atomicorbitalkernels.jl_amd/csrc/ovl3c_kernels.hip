@@ -605,8 +605,11 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                         g[(sh.off[sa] + ii) +
                           static_cast<int64_t>(N) * ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
     }
-    for (int i = 0; i < nla; ++i)
-        for (int j = 0; j < nlb; ++j)
+    // bra sums over (j, k) and ket-B sums over k in registers, one row update each
+    for (int i = 0; i < nla; ++i) {
+        double sA0 = 0.0, sA1 = 0.0;
+        for (int j = 0; j < nlb; ++j) {
+            double sB0 = 0.0, sB1 = 0.0;
             for (int k = 0; k < nlc; ++k) {
                 if ((i >= nra) + (j >= nrb) + (k >= nrc) > 1) continue;
                 const int64_t qa = sa + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ia) * K + i] +
@@ -679,13 +682,19 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                     }
                 const double wH = w * static_cast<double>(H);
                 const double cw = ca * cb * cc * w;
-                rA[2 * i] += cb * cc * wH;
-                rB[2 * j] += ca * cc * wH;
+                sA0 += cb * cc * wH;
+                sB0 += ca * cc * wH;
                 rC[2 * k] += ca * cb * wH;
-                rA[2 * i + 1] -= cw * static_cast<double>(Ha);
-                rB[2 * j + 1] -= cw * static_cast<double>(Hb);
+                sA1 -= cw * static_cast<double>(Ha);
+                sB1 -= cw * static_cast<double>(Hb);
                 rC[2 * k + 1] -= cw * static_cast<double>(Hc);
             }
+            rB[2 * j] += sB0;
+            rB[2 * j + 1] += sB1;
+        }
+        rA[2 * i] += sA0;
+        rA[2 * i + 1] += sA1;
+    }
 }
 
 template <typename T, int LMAXB>
