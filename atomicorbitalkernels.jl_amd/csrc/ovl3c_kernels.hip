@@ -579,7 +579,8 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                                            const double *__restrict__ coef64, const double *__restrict__ alpha64,
                                            const int *__restrict__ nreal, const int *__restrict__ nlist,
                                            const int *__restrict__ lidx, const Geo3 &geo, const T (&ab)[3],
-                                           const T (&ac)[3], const T *__restrict__ g, double *__restrict__ row)
+                                           const T (&ac)[3], const T *__restrict__ g, double *__restrict__ row,
+                                           double *__restrict__ cacc)
 {
     constexpr int NBA = Sh3<LA>::nbf, NBB = Sh3<LB>::nbf;
     constexpr int N1 = 2 * LA + 1, N2 = 2 * LB + 1;
@@ -605,7 +606,9 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                         g[(sh.off[sa] + ii) +
                           static_cast<int64_t>(N) * ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
     }
-    // bra sums over (j, k) and ket-B sums over k in registers, one row update each
+    // bra sums over (j, k) and ket-B sums over k in registers, ket-C sums over (i, j) in the
+    // lane's LDS column (cacc[(2 k + c) * 64]); one row update each
+    for (int k = 0; k < 2 * nlc; ++k) cacc[64 * k] = 0.0;
     for (int i = 0; i < nla; ++i) {
         double sA0 = 0.0, sA1 = 0.0;
         for (int j = 0; j < nlb; ++j) {
@@ -684,10 +687,10 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                 const double cw = ca * cb * cc * w;
                 sA0 += cb * cc * wH;
                 sB0 += ca * cc * wH;
-                rC[2 * k] += ca * cb * wH;
+                cacc[64 * (2 * k)] += ca * cb * wH;
                 sA1 -= cw * static_cast<double>(Ha);
                 sB1 -= cw * static_cast<double>(Hb);
-                rC[2 * k + 1] -= cw * static_cast<double>(Hc);
+                cacc[64 * (2 * k + 1)] -= cw * static_cast<double>(Hc);
             }
             rB[2 * j] += sB0;
             rB[2 * j + 1] += sB1;
@@ -695,6 +698,7 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
         rA[2 * i] += sA0;
         rA[2 * i + 1] += sA1;
     }
+    for (int k = 0; k < 2 * nlc; ++k) rC[k] += cacc[64 * k];
 }
 
 template <typename T, int LMAXB>
@@ -738,6 +742,8 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
         ac[ax] = static_cast<T>(dac);
     }
     const T *g = dT + static_cast<int64_t>(N) * N * N * b;
+    extern __shared__ double cacc_all[];  // [2 K][64]
+    double *cacc = cacc_all + threadIdx.x;
     for (int sc = 0; sc < ns; ++sc)
         for (int sb = 0; sb < ns; ++sb)
             for (int sa = 0; sa < ns; ++sa) {
@@ -746,7 +752,7 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
     if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB)                                                     \
         if (la == A_ && lb == B_ && lc == C_) {                                                                  \
             triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, coef64, alpha64, nreal, nlist, lidx, geo, ab, ac, g,  \
-                                      row);                                                                      \
+                                      row, cacc);                                                                \
             continue;                                                                                            \
         }
                 AOK_L3_CASES(AOK_B3_CASE)
@@ -920,8 +926,8 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     AOK_HIP(ctx, hipGetLastError());
     if (c.B > 0) {
         time_begin(ctx, "ovl3c_bwd_rows");
-        hipLaunchKernelGGL((ovl3c_bwd_rows<T, LMAXB>), dim3(static_cast<unsigned>((c.B + 63) / 64)), dim3(64), 0,
-                           ctx->stream, sh, c.B, static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
+        hipLaunchKernelGGL((ovl3c_bwd_rows<T, LMAXB>), dim3(static_cast<unsigned>((c.B + 63) / 64)), dim3(64),
+                           sizeof(double) * 64 * 2 * sh.K, ctx->stream, sh, c.B, static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
                            static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
                            c.sidxA, c.sidxB, c.sidxC, c.itype, c64, a64, reinterpret_cast<const int *>(ws + o_nr),
                            reinterpret_cast<const int *>(ws + o_nl), reinterpret_cast<const int *>(ws + o_li),
