@@ -574,6 +574,21 @@ __global__ __launch_bounds__(64) void ovl3_pb_lists(const OvlShape sh, const dou
     nlist[idx] = n;
 }
 
+template <typename T>
+__device__ __forceinline__ double exp_neg3(double x);
+template <>
+__device__ __forceinline__ double exp_neg3<float>(double x)
+{
+    const double y = x * 1.4426950408889634074;  // log2(e)
+    const double n = rint(y);
+    return static_cast<double>(ldexpf(__builtin_amdgcn_exp2f(-static_cast<float>(y - n)), -static_cast<int>(n)));
+}
+template <>
+__device__ __forceinline__ double exp_neg3<double>(double x)
+{
+    return exp(-x);
+}
+
 template <typename T, int LA, int LB, int LC>
 __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, int sc, const int (&sg)[3],
                                            const double *__restrict__ coef64, const double *__restrict__ alpha64,
@@ -611,22 +626,27 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     for (int k = 0; k < 2 * nlc; ++k) cacc[64 * k] = 0.0;
     for (int i = 0; i < nla; ++i) {
         double sA0 = 0.0, sA1 = 0.0;
+        const int64_t qa = sa + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ia) * K + i] +
+                                                            static_cast<int64_t>(K) * sg[0]);
+        const double ca = coef64[qa], al = alpha64[qa];
         for (int j = 0; j < nlb; ++j) {
             double sB0 = 0.0, sB1 = 0.0;
+            const int64_t qb = sb + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ib) * K + j] +
+                                                                static_cast<int64_t>(K) * sg[1]);
+            const double cb = coef64[qb], be = alpha64[qb];
+            const double p = al + be, ip = 1.0 / p, eab = al * be * geo.rab;
+            const T aop = static_cast<T>(al * ip), bop = static_cast<T>(be * ip);
             for (int k = 0; k < nlc; ++k) {
                 if ((i >= nra) + (j >= nrb) + (k >= nrc) > 1) continue;
-                const int64_t qa = sa + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ia) * K + i] +
-                                                                    static_cast<int64_t>(K) * sg[0]);
-                const int64_t qb = sb + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ib) * K + j] +
-                                                                    static_cast<int64_t>(K) * sg[1]);
                 const int64_t qc = sc + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ic) * K + k] +
                                                                     static_cast<int64_t>(K) * sg[2]);
-                const double ca = coef64[qa], cb = coef64[qb], cc = coef64[qc];
-                const double al = alpha64[qa], be = alpha64[qb], ga = alpha64[qc];
-                const double p = al + be, q = p + ga, iq = 1.0 / q, ip = 1.0 / p;
+                const double cc = coef64[qc], ga = alpha64[qc];
+                const double q = p + ga, iq = 1.0 / q;
+                // exp(-(ab rab + ag rac + bg rbc) / q): the argument in FP64, the exponential
+                // in the kernel precision (FP32: range reduction in FP64, then v_exp_f32)
                 const double w = 5.568327996831707845 * iq * sqrt(iq) *
-                                 exp(-(al * be * geo.rab + al * ga * geo.rac + be * ga * geo.rbc) * iq);
-                const T aop = static_cast<T>(al * ip), bop = static_cast<T>(be * ip), gq = static_cast<T>(ga * iq);
+                                 exp_neg3<T>((eab + ga * (al * geo.rac + be * geo.rbc)) * iq);
+                const T gq = static_cast<T>(ga * iq);
                 const T pq = static_cast<T>(p * iq), i2q = static_cast<T>(0.5 * iq);
                 T sx[LA + 3][LB + 3][LC + 3], sy[LA + 3][LB + 3][LC + 3], sz[LA + 3][LB + 3][LC + 3];
                 {
