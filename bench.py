@@ -431,7 +431,22 @@ def main():
         B3 = c4["posA"].shape[0]
         out["ovl3c_c4a"] = {"B": B3, "kernel_ms": float(np.mean(ks)), "wall_ms": wall3,
                             "triples_per_s": B3 / (wall3 * 1e-3), "kernel_triples_per_s": B3 / (np.mean(ks) * 1e-3)}
-        del V
+        # the 3-centre (coef, alpha) pullback of a random cotangent at the same size
+        log("3c pullback B=2^14")
+        g3 = torch.Generator(device=dev).manual_seed(11)
+        dT = torch.randn(tuple(reversed(V.shape)), generator=g3, device=dev, dtype=torch.float32).permute(3, 2, 1, 0)
+        aok.overlap_3c_pb_coef_alpha(dT, basis3, *X3, ctx=ctx)
+        ks = []
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(3):
+            aok.overlap_3c_pb_coef_alpha(dT, basis3, *X3, ctx=ctx)
+            ks.append(ctx.last_kernel_ms())
+        torch.cuda.synchronize()
+        wall3b = (time.perf_counter() - ts) * 1e3 / 3
+        out["ovl3c_c4a_pullback"] = {"B": B3, "kernel": "ovl3c_bwd_rows", "kernel_ms": float(np.mean(ks)),
+                                     "wall_ms": wall3b, "triples_per_s": B3 / (wall3b * 1e-3)}
+        del V, dT
 
     if not args.no_cpu:
         log("CPU baseline (C oracle, FP64, 1 thread)")
