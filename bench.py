@@ -218,6 +218,16 @@ def main():
         return ms
 
     if not args.no_extra:
+        # position cotangents of the same step (rrule with wrt_positions): their kernel alone
+        log("C1 position cotangents")
+        _, pbx = aok.rrule_batch_overlap(orb32, XA32, XB32, ps32, ctx=ctx, wrt_positions=True)
+        pbx(dS32)
+        ctx.kernel_times()
+        for _ in range(5):
+            pbx(dS32)
+        dxk = [ms for name, ms in ctx.kernel_times() if name == "ovl2c_dx_fast"]
+        out["c1_position_cotangents"] = {"kernel": "ovl2c_dx_fast", "kernel_ms": float(np.mean(dxk)),
+                                         "pairs_per_s": B / (float(np.mean(dxk)) * 1e-3)}
         log("C1 graph-replayed step")
         gms = graph_step_ms(orb32, XA32, XB32, ps32, dS32, args.steps)
         out["c1_graph_step"] = {"ms_per_step": gms, "pairs_per_s": B / (gms * 1e-3),
