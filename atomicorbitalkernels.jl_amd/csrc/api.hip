@@ -281,6 +281,9 @@ aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
     } else if (k == "ovl_split") {
         AOK_REQUIRE(ctx, value >= 0 && value <= 2, "ovl_split must be 0, 1 or 2");
         ctx->tune.ovl_split = static_cast<int>(value);
+    } else if (k == "ovl3_parts") {
+        AOK_REQUIRE(ctx, value >= 0 && value <= 3, "ovl3_parts must be 0, 1, 2 or 3");
+        ctx->tune.ovl3_parts = static_cast<int>(value);
     } else {
         return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
     }
@@ -297,6 +300,7 @@ aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value)
     else if (k == "ovl_cap") *value = ctx->tune.ovl_cap;
     else if (k == "ovl_split") *value = ctx->tune.ovl_split;
     else if (k == "ovl_order") *value = ctx->tune.ovl_order;
+    else if (k == "ovl3_parts") *value = ctx->tune.ovl3_parts;
     else return fail(ctx, AOK_ERR_ARG, "unknown tuning key '" + k + "'");
     return AOK_OK;
 }

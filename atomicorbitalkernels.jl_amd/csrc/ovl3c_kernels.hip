@@ -24,6 +24,7 @@ namespace aok {
 
 constexpr int OVL3_MAXNZ = 8;
 constexpr int OVL3_FAST_LMAX = 2;
+constexpr size_t OVL3_ROWS_CAP = size_t(1) << 30;  // pullback partial rows, bytes (part choice)
 
 struct alignas(16) TripRecF {
     double cab, cac, cbc;  // (pairwise exponent products / q) * log2(e)
@@ -528,10 +529,15 @@ __global__ __launch_bounds__(128) void ovl3c_fwd_plain(const OvlShape sh, int64_
 // raised-momentum identity d/dalpha x^l e^{-alpha x^2} = -x^{l+2} e^{-alpha x^2}.
 // Per (species, shell) a list holds the real slots, then one representative per
 // distinct padded exponent (pcls maps every slot to its list entry).  One thread per
-// batch element walks every shell triple and every list triple with at most one padded
-// entry (two padded entries contribute nothing), accumulating into its own FP64 row
-// [side][shell][list entry][coef, exponent]; ovl3_bwd_reduce then sums the rows of each
-// slot over the batch in a fixed order.  Deterministic, no atomics.
+// (batch element, part) walks the part's shell triples and every list triple with at
+// most one padded entry (two padded entries contribute nothing), accumulating into its
+// own FP64 row [side][shell][list entry][coef, exponent]; ovl3_bwd_reduce then sums the
+// rows of each slot over the batch and the parts in a fixed order.  Deterministic, no
+// atomics.  Parts split a triple's shell loops over the grid's y dimension so that a
+// batch of a few thousand triples still fills the chip: by third-centre shell (the row
+// then holds that one C shell) and optionally by second-centre shell too (one B shell).
+// Rows are stored element-major, batch innermost (element e of (b, part p) at
+// rows[(p * rlen + e) * B + b]): consecutive lanes touch consecutive doubles.
 // ---------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void widen_pair(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
@@ -594,7 +600,8 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                                            const double *__restrict__ coef64, const double *__restrict__ alpha64,
                                            const int *__restrict__ nreal, const int *__restrict__ nlist,
                                            const int *__restrict__ lidx, const Geo3 &geo, const T (&ab)[3],
-                                           const T (&ac)[3], const T *__restrict__ g, double *__restrict__ row,
+                                           const T (&ac)[3], const T *__restrict__ g, double *__restrict__ rA,
+                                           double *__restrict__ rB, double *__restrict__ rC, int64_t rs,
                                            double *__restrict__ cacc)
 {
     constexpr int NBA = Sh3<LA>::nbf, NBB = Sh3<LB>::nbf;
@@ -603,9 +610,6 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     const int ia = sg[0] * ns + sa, ib = sg[1] * ns + sb, ic = sg[2] * ns + sc;
     const int nra = nreal[ia], nrb = nreal[ib], nrc = nreal[ic];
     const int nla = nlist[ia], nlb = nlist[ib], nlc = nlist[ic];
-    double *rA = row + 2 * (static_cast<int64_t>(0 * ns + sa) * K);
-    double *rB = row + 2 * (static_cast<int64_t>(1 * ns + sb) * K);
-    double *rC = row + 2 * (static_cast<int64_t>(2 * ns + sc) * K);
     // small blocks: the cotangent block of this shell triple in registers, read once
     constexpr int NBC = Sh3<LC>::nbf, NG = NBA * NBB * NBC;
     constexpr bool GREG = NG <= 27;
@@ -712,13 +716,13 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                 sB1 -= cw * static_cast<double>(Hb);
                 cacc[64 * (2 * k + 1)] -= cw * static_cast<double>(Hc);
             }
-            rB[2 * j] += sB0;
-            rB[2 * j + 1] += sB1;
+            rB[(2 * j) * rs] += sB0;
+            rB[(2 * j + 1) * rs] += sB1;
         }
-        rA[2 * i] += sA0;
-        rA[2 * i + 1] += sA1;
+        rA[(2 * i) * rs] += sA0;
+        rA[(2 * i + 1) * rs] += sA1;
     }
-    for (int k = 0; k < 2 * nlc; ++k) rC[k] += cacc[64 * k];
+    for (int k = 0; k < 2 * nlc; ++k) rC[k * rs] += cacc[64 * k];
 }
 
 template <typename T, int LMAXB>
@@ -729,15 +733,20 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
                                                      int itype, const double *__restrict__ coef64,
                                                      const double *__restrict__ alpha64,
                                                      const int *__restrict__ nreal, const int *__restrict__ nlist,
-                                                     const int *__restrict__ lidx, double *__restrict__ rows,
-                                                     unsigned *__restrict__ err)
+                                                     const int *__restrict__ lidx, int splitB, int splitC,
+                                                     double *__restrict__ rows, unsigned *__restrict__ err)
 {
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
     const int64_t b = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
     if (b >= B) return;
-    const int64_t rlen = 6 * static_cast<int64_t>(ns) * K;
-    double *row = rows + b * rlen;
-    for (int64_t q = 0; q < rlen; ++q) row[q] = 0.0;
+    // part p = pc * nbp + pb, last parts (shells listed last, usually the largest l) first
+    const int nbp = splitB ? ns : 1;
+    const int p = static_cast<int>(gridDim.y) - 1 - static_cast<int>(blockIdx.y);
+    const int pc = p / nbp, pb = p - pc * nbp;
+    const int nB = splitB ? 1 : ns, nC = splitC ? 1 : ns;
+    const int64_t rlen = 2 * static_cast<int64_t>(K) * (ns + nB + nC);
+    double *row = rows + static_cast<int64_t>(p) * rlen * B + b;
+    for (int64_t q = 0; q < rlen; ++q) row[q * B] = 0.0;
     int sg[3] = {load_sigma(sA, itype, b), load_sigma(sB, itype, b), load_sigma(sC, itype, b)};
     for (int c = 0; c < 3; ++c)
         if (sg[c] < 0 || sg[c] >= NZ) {
@@ -764,15 +773,21 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
     const T *g = dT + static_cast<int64_t>(N) * N * N * b;
     extern __shared__ double cacc_all[];  // [2 K][64]
     double *cacc = cacc_all + threadIdx.x;
-    for (int sc = 0; sc < ns; ++sc)
-        for (int sb = 0; sb < ns; ++sb)
+    const int sc0 = splitC ? pc : 0, sc1 = splitC ? pc + 1 : ns;
+    const int sb0 = splitB ? pb : 0, sb1 = splitB ? pb + 1 : ns;
+    const int64_t K2 = 2 * static_cast<int64_t>(K);
+    for (int sc = sc0; sc < sc1; ++sc)
+        for (int sb = sb0; sb < sb1; ++sb)
             for (int sa = 0; sa < ns; ++sa) {
                 const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
+                double *rA = row + K2 * sa * B;
+                double *rB = row + K2 * (ns + (splitB ? 0 : sb)) * B;
+                double *rC = row + K2 * (ns + nB + (splitC ? 0 : sc)) * B;
 #define AOK_B3_CASE(A_, B_, C_)                                                                                  \
     if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB)                                                     \
         if (la == A_ && lb == B_ && lc == C_) {                                                                  \
             triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, coef64, alpha64, nreal, nlist, lidx, geo, ab, ac, g,  \
-                                      row, cacc);                                                                \
+                                      rA, rB, rC, B, cacc);                                                      \
             continue;                                                                                            \
         }
                 AOK_L3_CASES(AOK_B3_CASE)
@@ -780,13 +795,43 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
             }
 }
 
+// Fold the parts' partial rows into one row per batch element, in the unsplit layout
+// ([side][shell][list entry][coef, exponent], batch innermost): thread per (b, element),
+// a fixed-order sum over the parts that hold the element.  Fully parallel, so the
+// per-slot reduction below then reads one row per element instead of one per part.
+__global__ __launch_bounds__(256) void ovl3_parts_sum(const OvlShape sh, int64_t B, int splitB, int splitC,
+                                                      const double *__restrict__ rows, double *__restrict__ rows1)
+{
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (b >= B) return;
+    const int ns = sh.ns;
+    const int K2 = 2 * sh.K, E = 3 * ns * K2;
+    const int nbp = splitB ? ns : 1, ncp = splitC ? ns : 1;
+    const int nB = splitB ? 1 : ns, nC = splitC ? 1 : ns;
+    const int64_t rlen = static_cast<int64_t>(K2) * (ns + nB + nC);
+    for (int e1 = blockIdx.y; e1 < E; e1 += gridDim.y) {
+        const int side = e1 / (ns * K2), r = e1 - side * ns * K2, s = r / K2, w = r - s * K2;
+        const int64_t e = side == 0   ? static_cast<int64_t>(K2) * s + w
+                          : side == 1 ? static_cast<int64_t>(K2) * (ns + (splitB ? 0 : s)) + w
+                                      : static_cast<int64_t>(K2) * (ns + nB + (splitC ? 0 : s)) + w;
+        const int c0 = side == 2 && splitC ? s : 0, c1 = side == 2 && splitC ? s + 1 : ncp;
+        const int b0 = side == 1 && splitB ? s : 0, b1 = side == 1 && splitB ? s + 1 : nbp;
+        double acc = 0.0;
+        for (int pc = c0; pc < c1; ++pc)
+            for (int pb = b0; pb < b1; ++pb) acc += rows[(static_cast<int64_t>(pc * nbp + pb) * rlen + e) * B + b];
+        rows1[static_cast<int64_t>(e1) * B + b] = acc;
+    }
+}
+
 // Block per slot (sigma, s, k): sum over the batch of the rows whose species on that
-// side is sigma, sides A, B, C; thread-strided partial sums, then a block tree.
+// side is sigma, sides A, B, C, each over the parts holding that shell's row (A: every
+// part; B, C: the parts of that shell when split by it, else every part); thread-strided
+// partial sums in a fixed order, then a block tree.
 template <typename T>
 __global__ __launch_bounds__(256) void ovl3_bwd_reduce(const OvlShape sh, int64_t B, const void *__restrict__ sA,
                                                        const void *__restrict__ sB, const void *__restrict__ sC,
-                                                       int itype, const int *__restrict__ pcls,
-                                                       const double *__restrict__ rows, T *__restrict__ dcoef,
+                                                       int itype, const int *__restrict__ pcls, int splitB,
+                                                       int splitC, const double *__restrict__ rows, T *__restrict__ dcoef,
                                                        T *__restrict__ dalpha)
 {
     __shared__ double red[8];
@@ -794,15 +839,26 @@ __global__ __launch_bounds__(256) void ovl3_bwd_reduce(const OvlShape sh, int64_
     const int q = blockIdx.x;  // [s + ns * (k + K * sg)], the Julia index
     const int s = q % ns, k = (q / ns) % K, sg = q / (ns * K);
     const int li = pcls[static_cast<int64_t>(sg * ns + s) * K + k];
-    const int64_t rlen = 6 * static_cast<int64_t>(ns) * K;
+    const int nbp = splitB ? ns : 1, ncp = splitC ? ns : 1;
+    const int nB = splitB ? 1 : ns, nC = splitC ? 1 : ns;
+    const int64_t K2 = 2 * static_cast<int64_t>(K);
+    const int64_t rlen = K2 * (ns + nB + nC);
+    // element of this slot per side, and the parts [c0, c1) x [b0, b1) that hold it
+    const int64_t e[3] = {K2 * s + 2 * li, K2 * (ns + (splitB ? 0 : s)) + 2 * li,
+                          K2 * (ns + nB + (splitC ? 0 : s)) + 2 * li};
+    const int c0[3] = {0, 0, splitC ? s : 0}, c1[3] = {ncp, ncp, splitC ? s + 1 : ncp};
+    const int b0[3] = {0, splitB ? s : 0, 0}, b1[3] = {nbp, splitB ? s + 1 : nbp, nbp};
     const void *sides[3] = {sA, sB, sC};
     double c = 0.0, a = 0.0;
     for (int64_t b = threadIdx.x; b < B; b += 256)
         for (int side = 0; side < 3; ++side) {
             if (load_sigma(sides[side], itype, b) != sg) continue;
-            const double *r = rows + b * rlen + 2 * (static_cast<int64_t>(side * ns + s) * K + li);
-            c += r[0];
-            a += r[1];
+            for (int pc = c0[side]; pc < c1[side]; ++pc)
+                for (int pb = b0[side]; pb < b1[side]; ++pb) {
+                    const double *r = rows + (static_cast<int64_t>(pc * nbp + pb) * rlen + e[side]) * B + b;
+                    c += r[0];
+                    a += r[B];
+                }
         }
     block_sum256x2(c, a, red);
     if (threadIdx.x == 0) {
@@ -931,7 +987,24 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     const size_t o_c = take(sizeof(double) * npar), o_a = take(sizeof(double) * npar);
     const size_t o_nr = take(sizeof(int) * sh.NZ * sh.ns), o_nl = take(sizeof(int) * sh.NZ * sh.ns);
     const size_t o_li = take(sizeof(int) * npar), o_pc = take(sizeof(int) * npar);
-    const size_t o_rows = take(sizeof(double) * 6 * static_cast<size_t>(sh.ns) * sh.K * static_cast<size_t>(c.B > 0 ? c.B : 1));
+    // parts: by (third, second) shell while the rows fit OVL3_ROWS_CAP, else by third shell,
+    // else none; the "ovl3_parts" knob forces one (1 none, 2 third, 3 third and second)
+    const size_t Bz = static_cast<size_t>(c.B > 0 ? c.B : 1), nsz = sh.ns;
+    auto rows_bytes = [&](int sb_, int sc_) {
+        const size_t parts = (sb_ ? nsz : 1) * (sc_ ? nsz : 1);
+        return sizeof(double) * 2 * sh.K * (nsz + (sb_ ? 1 : nsz) + (sc_ ? 1 : nsz)) * parts * Bz;
+    };
+    int splitB = 1, splitC = 1;
+    if (ctx->tune.ovl3_parts != 0) {
+        splitC = ctx->tune.ovl3_parts >= 2;
+        splitB = ctx->tune.ovl3_parts >= 3;
+    } else if (rows_bytes(1, 1) > OVL3_ROWS_CAP) {
+        splitB = 0;
+        if (rows_bytes(0, 1) > OVL3_ROWS_CAP) splitC = 0;
+    }
+    const int nparts = (splitB ? sh.ns : 1) * (splitC ? sh.ns : 1);
+    const size_t o_rows = take(rows_bytes(splitB, splitC));
+    const size_t o_rows1 = nparts > 1 ? take(rows_bytes(0, 0)) : o_rows;  // parts folded
     aok_status st = ensure_ws(ctx, o);
     if (st != AOK_OK) return st;
     char *ws = static_cast<char *>(ctx->ws);
@@ -946,18 +1019,26 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     AOK_HIP(ctx, hipGetLastError());
     if (c.B > 0) {
         time_begin(ctx, "ovl3c_bwd_rows");
-        hipLaunchKernelGGL((ovl3c_bwd_rows<T, LMAXB>), dim3(static_cast<unsigned>((c.B + 63) / 64)), dim3(64),
+        hipLaunchKernelGGL((ovl3c_bwd_rows<T, LMAXB>),
+                           dim3(static_cast<unsigned>((c.B + 63) / 64), static_cast<unsigned>(nparts)), dim3(64),
                            sizeof(double) * 64 * 2 * sh.K, ctx->stream, sh, c.B, static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
                            static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
                            c.sidxA, c.sidxB, c.sidxC, c.itype, c64, a64, reinterpret_cast<const int *>(ws + o_nr),
-                           reinterpret_cast<const int *>(ws + o_nl), reinterpret_cast<const int *>(ws + o_li),
-                           reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
+                           reinterpret_cast<const int *>(ws + o_nl), reinterpret_cast<const int *>(ws + o_li), splitB,
+                           splitC, reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
         time_end(ctx);
         AOK_HIP(ctx, hipGetLastError());
+        if (nparts > 1) {
+            const int E = 6 * sh.ns * sh.K;
+            hipLaunchKernelGGL(ovl3_parts_sum, dim3(static_cast<unsigned>((c.B + 255) / 256), std::min(E, 65535)),
+                               dim3(256), 0, ctx->stream, sh, c.B, splitB, splitC,
+                               reinterpret_cast<const double *>(ws + o_rows), reinterpret_cast<double *>(ws + o_rows1));
+            AOK_HIP(ctx, hipGetLastError());
+        }
     }
     hipLaunchKernelGGL((ovl3_bwd_reduce<T>), dim3(static_cast<unsigned>(npar)), dim3(256), 0, ctx->stream, sh, c.B,
-                       c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const int *>(ws + o_pc),
-                       reinterpret_cast<const double *>(ws + o_rows), static_cast<T *>(c.o1),
+                       c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const int *>(ws + o_pc), 0, 0,
+                       reinterpret_cast<const double *>(ws + o_rows1), static_cast<T *>(c.o1),
                        static_cast<T *>(c.o2));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;

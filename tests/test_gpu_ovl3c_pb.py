@@ -52,6 +52,35 @@ def test_c4a_coef_alpha_pullback_vs_oracle(dev, dtype, tol):
     assert np.any(host(dc)[pad] != 0.0)
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-13), (torch.float32, 1e-6)])
+def test_parts_agree(dev, dtype, tol):
+    """Every split of the shell loops (ovl3_parts 1 none, 2 by third-centre shell, 3 by
+    (third, second) shell pair, 0 automatic) sums the same terms in another grouping:
+    equal to rounding.  B = 333 leaves a ragged last batch tile."""
+    B = 333
+    d = configs.c4a(B=B)
+    cb = d["basis"]
+    coef = rounded(O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls), dtype)
+    basis = _basis(cb.ls, cb.zeta, coef, dtype, dev)
+    rng = np.random.default_rng(11)
+    dT = dev_f(rng.standard_normal((cb.N, cb.N, cb.N, B)), dtype, dev)
+    ctx = aok.Context(dev.index or 0)
+    X = _inputs(d, dtype, dev)
+    got = {}
+    try:
+        for parts in (1, 2, 3, 0):
+            log(f"3c pullback C4a B={B} dtype={dtype} ovl3_parts={parts}")
+            ctx.tune("ovl3_parts", parts)
+            dc, da = aok.overlap_3c_pb_coef_alpha(dT, basis, *X, ctx=ctx)
+            got[parts] = (host(dc), host(da))
+    finally:
+        ctx.tune("ovl3_parts", 0)
+    for parts in (2, 3, 0):
+        assert rel_max(got[parts][0], got[1][0]) < tol
+        assert rel_max(got[parts][1], got[1][1]) < tol
+    assert np.array_equal(got[0][0], got[3][0]) and np.array_equal(got[0][1], got[3][1])
+
+
 def test_orbital_rrule_vs_oracle(dev):
     """(zeta, D) cotangents through the FP64 normalisation VJP, FP64."""
     B = 40
