@@ -725,7 +725,10 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     for (int k = 0; k < 2 * nlc; ++k) rC[k * rs] += cacc[64 * k];
 }
 
-template <typename T, int LMAXB>
+// CLS >= 0: only the shell triples with CLS d shells (one instance per class, so that
+// each is compiled for the registers of its own cases; CLS 0 clears the rows and runs
+// first), CLS < 0: every triple.
+template <typename T, int LMAXB, int CLS>
 __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t B, const T *__restrict__ dT,
                                                      const T *__restrict__ posA, const T *__restrict__ posB,
                                                      const T *__restrict__ posC, T scale, const void *__restrict__ sA,
@@ -746,7 +749,8 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
     const int nB = splitB ? 1 : ns, nC = splitC ? 1 : ns;
     const int64_t rlen = 2 * static_cast<int64_t>(K) * (ns + nB + nC);
     double *row = rows + static_cast<int64_t>(p) * rlen * B + b;
-    for (int64_t q = 0; q < rlen; ++q) row[q * B] = 0.0;
+    if constexpr (CLS <= 0)
+        for (int64_t q = 0; q < rlen; ++q) row[q * B] = 0.0;
     int sg[3] = {load_sigma(sA, itype, b), load_sigma(sB, itype, b), load_sigma(sC, itype, b)};
     for (int c = 0; c < 3; ++c)
         if (sg[c] < 0 || sg[c] >= NZ) {
@@ -780,11 +784,14 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
         for (int sb = sb0; sb < sb1; ++sb)
             for (int sa = 0; sa < ns; ++sa) {
                 const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
+                if constexpr (CLS >= 0)
+                    if ((la == 2) + (lb == 2) + (lc == 2) != CLS) continue;
                 double *rA = row + K2 * sa * B;
                 double *rB = row + K2 * (ns + (splitB ? 0 : sb)) * B;
                 double *rC = row + K2 * (ns + nB + (splitC ? 0 : sc)) * B;
 #define AOK_B3_CASE(A_, B_, C_)                                                                                  \
-    if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB)                                                     \
+    if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB &&                                                   \
+                  (CLS < 0 || (A_ == 2) + (B_ == 2) + (C_ == 2) == CLS))                                         \
         if (la == A_ && lb == B_ && lc == C_) {                                                                  \
             triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, coef64, alpha64, nreal, nlist, lidx, geo, ab, ac, g,  \
                                       rA, rB, rC, B, cacc);                                                      \
@@ -1019,13 +1026,24 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     AOK_HIP(ctx, hipGetLastError());
     if (c.B > 0) {
         time_begin(ctx, "ovl3c_bwd_rows");
-        hipLaunchKernelGGL((ovl3c_bwd_rows<T, LMAXB>),
-                           dim3(static_cast<unsigned>((c.B + 63) / 64), static_cast<unsigned>(nparts)), dim3(64),
-                           sizeof(double) * 64 * 2 * sh.K, ctx->stream, sh, c.B, static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
-                           static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
-                           c.sidxA, c.sidxB, c.sidxC, c.itype, c64, a64, reinterpret_cast<const int *>(ws + o_nr),
-                           reinterpret_cast<const int *>(ws + o_nl), reinterpret_cast<const int *>(ws + o_li), splitB,
-                           splitC, reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
+        auto rows_launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((c.B + 63) / 64), static_cast<unsigned>(nparts)),
+                               dim3(64), sizeof(double) * 64 * 2 * sh.K, ctx->stream, sh, c.B,
+                               static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
+                               static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
+                               c.sidxA, c.sidxB, c.sidxC, c.itype, c64, a64, reinterpret_cast<const int *>(ws + o_nr),
+                               reinterpret_cast<const int *>(ws + o_nl), reinterpret_cast<const int *>(ws + o_li),
+                               splitB, splitC, reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
+        };
+        // with d shells: one launch per number of d shells in the triple (0 first: it clears)
+        if constexpr (LMAXB == 2) {
+            rows_launch(&ovl3c_bwd_rows<T, 2, 0>);
+            rows_launch(&ovl3c_bwd_rows<T, 2, 1>);
+            rows_launch(&ovl3c_bwd_rows<T, 2, 2>);
+            rows_launch(&ovl3c_bwd_rows<T, 2, 3>);
+        } else {
+            rows_launch(&ovl3c_bwd_rows<T, LMAXB, -1>);
+        }
         time_end(ctx);
         AOK_HIP(ctx, hipGetLastError());
         if (nparts > 1) {
