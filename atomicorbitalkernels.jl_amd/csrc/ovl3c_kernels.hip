@@ -552,7 +552,8 @@ __global__ __launch_bounds__(256) void widen_pair(int64_t n, const T *__restrict
 __global__ __launch_bounds__(64) void ovl3_pb_lists(const OvlShape sh, const double *__restrict__ coef64,
                                                     const double *__restrict__ alpha64, int *__restrict__ nreal,
                                                     int *__restrict__ nlist, int *__restrict__ lidx,
-                                                    int *__restrict__ pcls)
+                                                    int *__restrict__ pcls, double *__restrict__ lcoef,
+                                                    double *__restrict__ lalpha)
 {
     const int ns = sh.ns, K = sh.K;
     const int idx = static_cast<int>(blockIdx.x) * 64 + threadIdx.x;  // sigma * ns + s
@@ -578,6 +579,12 @@ __global__ __launch_bounds__(64) void ovl3_pb_lists(const OvlShape sh, const dou
         pcls[static_cast<int64_t>(idx) * K + k] = rep;
     }
     nlist[idx] = n;
+    // (coef, alpha) in list order: the pullback reads them without the index indirection
+    for (int r = 0; r < n; ++r) {
+        const int64_t qk = q(lidx[static_cast<int64_t>(idx) * K + r]);
+        lcoef[static_cast<int64_t>(idx) * K + r] = coef64[qk];
+        lalpha[static_cast<int64_t>(idx) * K + r] = alpha64[qk];
+    }
 }
 
 template <typename T>
@@ -597,12 +604,12 @@ __device__ __forceinline__ double exp_neg3<double>(double x)
 
 template <typename T, int LA, int LB, int LC>
 __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, int sc, const int (&sg)[3],
-                                           const double *__restrict__ coef64, const double *__restrict__ alpha64,
+                                           const double *__restrict__ lcoef, const double *__restrict__ lalpha,
                                            const int *__restrict__ nreal, const int *__restrict__ nlist,
-                                           const int *__restrict__ lidx, const Geo3 &geo, const T (&ab)[3],
+                                           const Geo3 &geo, const T (&ab)[3],
                                            const T (&ac)[3], const T *__restrict__ g, double *__restrict__ rA,
                                            double *__restrict__ rB, double *__restrict__ rC, int64_t rs,
-                                           double *__restrict__ cacc)
+                                           double *__restrict__ cacc, double *__restrict__ cval)
 {
     constexpr int NBA = Sh3<LA>::nbf, NBB = Sh3<LB>::nbf;
     constexpr int N1 = 2 * LA + 1, N2 = 2 * LB + 1;
@@ -627,24 +634,24 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     }
     // bra sums over (j, k) and ket-B sums over k in registers, ket-C sums over (i, j) in the
     // lane's LDS column (cacc[(2 k + c) * 64]); one row update each
-    for (int k = 0; k < 2 * nlc; ++k) cacc[64 * k] = 0.0;
+    // the C shell's (coef, alpha) list in the lane's LDS column too (cval[(2 k + c) * 64])
+    for (int k = 0; k < nlc; ++k) {
+        cacc[64 * (2 * k)] = 0.0;
+        cacc[64 * (2 * k + 1)] = 0.0;
+        cval[64 * (2 * k)] = lcoef[static_cast<int64_t>(ic) * K + k];
+        cval[64 * (2 * k + 1)] = lalpha[static_cast<int64_t>(ic) * K + k];
+    }
     for (int i = 0; i < nla; ++i) {
         double sA0 = 0.0, sA1 = 0.0;
-        const int64_t qa = sa + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ia) * K + i] +
-                                                            static_cast<int64_t>(K) * sg[0]);
-        const double ca = coef64[qa], al = alpha64[qa];
+        const double ca = lcoef[static_cast<int64_t>(ia) * K + i], al = lalpha[static_cast<int64_t>(ia) * K + i];
         for (int j = 0; j < nlb; ++j) {
             double sB0 = 0.0, sB1 = 0.0;
-            const int64_t qb = sb + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ib) * K + j] +
-                                                                static_cast<int64_t>(K) * sg[1]);
-            const double cb = coef64[qb], be = alpha64[qb];
+            const double cb = lcoef[static_cast<int64_t>(ib) * K + j], be = lalpha[static_cast<int64_t>(ib) * K + j];
             const double p = al + be, ip = 1.0 / p, eab = al * be * geo.rab;
             const T aop = static_cast<T>(al * ip), bop = static_cast<T>(be * ip);
             for (int k = 0; k < nlc; ++k) {
                 if ((i >= nra) + (j >= nrb) + (k >= nrc) > 1) continue;
-                const int64_t qc = sc + static_cast<int64_t>(ns) * (lidx[static_cast<int64_t>(ic) * K + k] +
-                                                                    static_cast<int64_t>(K) * sg[2]);
-                const double cc = coef64[qc], ga = alpha64[qc];
+                const double cc = cval[64 * (2 * k)], ga = cval[64 * (2 * k + 1)];
                 const double q = p + ga, iq = 1.0 / q;
                 // exp(-(ab rab + ag rac + bg rbc) / q): the argument in FP64, the exponential
                 // in the kernel precision (FP32: range reduction in FP64, then v_exp_f32)
@@ -733,10 +740,10 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
                                                      const T *__restrict__ posA, const T *__restrict__ posB,
                                                      const T *__restrict__ posC, T scale, const void *__restrict__ sA,
                                                      const void *__restrict__ sB, const void *__restrict__ sC,
-                                                     int itype, const double *__restrict__ coef64,
-                                                     const double *__restrict__ alpha64,
+                                                     int itype, const double *__restrict__ lcoef,
+                                                     const double *__restrict__ lalpha,
                                                      const int *__restrict__ nreal, const int *__restrict__ nlist,
-                                                     const int *__restrict__ lidx, int splitB, int splitC,
+                                                     int splitB, int splitC,
                                                      double *__restrict__ rows, unsigned *__restrict__ err)
 {
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -775,8 +782,8 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
         ac[ax] = static_cast<T>(dac);
     }
     const T *g = dT + static_cast<int64_t>(N) * N * N * b;
-    extern __shared__ double cacc_all[];  // [2 K][64]
-    double *cacc = cacc_all + threadIdx.x;
+    extern __shared__ double cacc_all[];  // [2 K][64] sums, then [2 K][64] C-list values
+    double *cacc = cacc_all + threadIdx.x, *cval = cacc_all + 64 * 2 * K + threadIdx.x;
     const int sc0 = splitC ? pc : 0, sc1 = splitC ? pc + 1 : ns;
     const int sb0 = splitB ? pb : 0, sb1 = splitB ? pb + 1 : ns;
     const int64_t K2 = 2 * static_cast<int64_t>(K);
@@ -793,8 +800,8 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
     if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB &&                                                   \
                   (CLS < 0 || (A_ == 2) + (B_ == 2) + (C_ == 2) == CLS))                                         \
         if (la == A_ && lb == B_ && lc == C_) {                                                                  \
-            triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, coef64, alpha64, nreal, nlist, lidx, geo, ab, ac, g,  \
-                                      rA, rB, rC, B, cacc);                                                      \
+            triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, lcoef, lalpha, nreal, nlist, geo, ab, ac, g, rA, rB,  \
+                                      rC, B, cacc, cval);                                                        \
             continue;                                                                                            \
         }
                 AOK_L3_CASES(AOK_B3_CASE)
@@ -994,6 +1001,7 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     const size_t o_c = take(sizeof(double) * npar), o_a = take(sizeof(double) * npar);
     const size_t o_nr = take(sizeof(int) * sh.NZ * sh.ns), o_nl = take(sizeof(int) * sh.NZ * sh.ns);
     const size_t o_li = take(sizeof(int) * npar), o_pc = take(sizeof(int) * npar);
+    const size_t o_lc = take(sizeof(double) * npar), o_la = take(sizeof(double) * npar);
     // parts: by (third, second) shell while the rows fit OVL3_ROWS_CAP, else by third shell,
     // else none; the "ovl3_parts" knob forces one (1 none, 2 third, 3 third and second)
     const size_t Bz = static_cast<size_t>(c.B > 0 ? c.B : 1), nsz = sh.ns;
@@ -1022,18 +1030,20 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     AOK_HIP(ctx, hipGetLastError());
     hipLaunchKernelGGL(ovl3_pb_lists, dim3((sh.NZ * sh.ns + 63) / 64), dim3(64), 0, ctx->stream, sh, c64, a64,
                        reinterpret_cast<int *>(ws + o_nr), reinterpret_cast<int *>(ws + o_nl),
-                       reinterpret_cast<int *>(ws + o_li), reinterpret_cast<int *>(ws + o_pc));
+                       reinterpret_cast<int *>(ws + o_li), reinterpret_cast<int *>(ws + o_pc),
+                       reinterpret_cast<double *>(ws + o_lc), reinterpret_cast<double *>(ws + o_la));
     AOK_HIP(ctx, hipGetLastError());
     if (c.B > 0) {
         time_begin(ctx, "ovl3c_bwd_rows");
         auto rows_launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((c.B + 63) / 64), static_cast<unsigned>(nparts)),
-                               dim3(64), sizeof(double) * 64 * 2 * sh.K, ctx->stream, sh, c.B,
+                               dim3(64), sizeof(double) * 64 * 4 * sh.K, ctx->stream, sh, c.B,
                                static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
                                static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
-                               c.sidxA, c.sidxB, c.sidxC, c.itype, c64, a64, reinterpret_cast<const int *>(ws + o_nr),
-                               reinterpret_cast<const int *>(ws + o_nl), reinterpret_cast<const int *>(ws + o_li),
-                               splitB, splitC, reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
+                               c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const double *>(ws + o_lc),
+                               reinterpret_cast<const double *>(ws + o_la), reinterpret_cast<const int *>(ws + o_nr),
+                               reinterpret_cast<const int *>(ws + o_nl), splitB, splitC,
+                               reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
         };
         // with d shells: one launch per number of d shells in the triple (0 first: it clears)
         if constexpr (LMAXB == 2) {
