@@ -602,6 +602,28 @@ __device__ __forceinline__ double exp_neg3<double>(double x)
     return exp(-x);
 }
 
+// pi^{3/2} q^{-3/2} and 1/q.  FP32 kernels: 1/q from v_rcp_f64 and two Newton steps
+// (FP64 accuracy, not correctly rounded), q^{-1/2} in FP32 (the prefactor multiplies FP32
+// terms); FP64 kernels: correctly rounded division and square root.
+template <typename T>
+__device__ __forceinline__ double inv_q(double q, double &w0);
+template <>
+__device__ __forceinline__ double inv_q<float>(double q, double &w0)
+{
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(fma(-q, r, 1.0), r, r);
+    r = fma(fma(-q, r, 1.0), r, r);
+    w0 = 5.568327996831707845 * r * static_cast<double>(__builtin_sqrtf(static_cast<float>(r)));
+    return r;
+}
+template <>
+__device__ __forceinline__ double inv_q<double>(double q, double &w0)
+{
+    const double r = 1.0 / q;
+    w0 = 5.568327996831707845 * r * sqrt(r);
+    return r;
+}
+
 template <typename T, int LA, int LB, int LC>
 __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, int sc, const int (&sg)[3],
                                            const double *__restrict__ lcoef, const double *__restrict__ lalpha,
@@ -652,11 +674,11 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
             for (int k = 0; k < nlc; ++k) {
                 if ((i >= nra) + (j >= nrb) + (k >= nrc) > 1) continue;
                 const double cc = cval[64 * (2 * k)], ga = cval[64 * (2 * k + 1)];
-                const double q = p + ga, iq = 1.0 / q;
+                double w0;
+                const double iq = inv_q<T>(p + ga, w0);
                 // exp(-(ab rab + ag rac + bg rbc) / q): the argument in FP64, the exponential
                 // in the kernel precision (FP32: range reduction in FP64, then v_exp_f32)
-                const double w = 5.568327996831707845 * iq * sqrt(iq) *
-                                 exp_neg3<T>((eab + ga * (al * geo.rac + be * geo.rbc)) * iq);
+                const double w = w0 * exp_neg3<T>((eab + ga * (al * geo.rac + be * geo.rbc)) * iq);
                 const T gq = static_cast<T>(ga * iq);
                 const T pq = static_cast<T>(p * iq), i2q = static_cast<T>(0.5 * iq);
                 T sx[LA + 3][LB + 3][LC + 3], sy[LA + 3][LB + 3][LC + 3], sz[LA + 3][LB + 3][LC + 3];
