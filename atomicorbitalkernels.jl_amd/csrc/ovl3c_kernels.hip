@@ -537,7 +537,9 @@ __global__ __launch_bounds__(128) void ovl3c_fwd_plain(const OvlShape sh, int64_
 // batch of a few thousand triples still fills the chip: by third-centre shell (the row
 // then holds that one C shell) and optionally by second-centre shell too (one B shell).
 // Rows are stored element-major, batch innermost (element e of (b, part p) at
-// rows[(p * rlen + e) * B + b]): consecutive lanes touch consecutive doubles.
+// rows[(p * rlen + e) * Bc + b - b0] for the chunk [b0, b0 + Bc) of the batch): consecutive
+// lanes touch consecutive doubles.  Split rows are folded per chunk into the unsplit
+// layout over the whole batch (ovl3_parts_sum), so the split rows need only one chunk.
 // ---------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void widen_pair(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
@@ -758,7 +760,8 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
 // each is compiled for the registers of its own cases; CLS 0 clears the rows and runs
 // first), CLS < 0: every triple.
 template <typename T, int LMAXB, int CLS>
-__global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t B, const T *__restrict__ dT,
+__global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t b0, int64_t Bc,
+                                                     const T *__restrict__ dT,
                                                      const T *__restrict__ posA, const T *__restrict__ posB,
                                                      const T *__restrict__ posC, T scale, const void *__restrict__ sA,
                                                      const void *__restrict__ sB, const void *__restrict__ sC,
@@ -769,17 +772,18 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
                                                      double *__restrict__ rows, unsigned *__restrict__ err)
 {
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
-    const int64_t b = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
-    if (b >= B) return;
+    const int64_t bl = static_cast<int64_t>(blockIdx.x) * 64 + threadIdx.x;
+    if (bl >= Bc) return;
+    const int64_t b = b0 + bl;
     // part p = pc * nbp + pb, last parts (shells listed last, usually the largest l) first
     const int nbp = splitB ? ns : 1;
     const int p = static_cast<int>(gridDim.y) - 1 - static_cast<int>(blockIdx.y);
     const int pc = p / nbp, pb = p - pc * nbp;
     const int nB = splitB ? 1 : ns, nC = splitC ? 1 : ns;
     const int64_t rlen = 2 * static_cast<int64_t>(K) * (ns + nB + nC);
-    double *row = rows + static_cast<int64_t>(p) * rlen * B + b;
+    double *row = rows + static_cast<int64_t>(p) * rlen * Bc + bl;
     if constexpr (CLS <= 0)
-        for (int64_t q = 0; q < rlen; ++q) row[q * B] = 0.0;
+        for (int64_t q = 0; q < rlen; ++q) row[q * Bc] = 0.0;
     int sg[3] = {load_sigma(sA, itype, b), load_sigma(sB, itype, b), load_sigma(sC, itype, b)};
     for (int c = 0; c < 3; ++c)
         if (sg[c] < 0 || sg[c] >= NZ) {
@@ -815,15 +819,15 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
                 const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
                 if constexpr (CLS >= 0)
                     if ((la == 2) + (lb == 2) + (lc == 2) != CLS) continue;
-                double *rA = row + K2 * sa * B;
-                double *rB = row + K2 * (ns + (splitB ? 0 : sb)) * B;
-                double *rC = row + K2 * (ns + nB + (splitC ? 0 : sc)) * B;
+                double *rA = row + K2 * sa * Bc;
+                double *rB = row + K2 * (ns + (splitB ? 0 : sb)) * Bc;
+                double *rC = row + K2 * (ns + nB + (splitC ? 0 : sc)) * Bc;
 #define AOK_B3_CASE(A_, B_, C_)                                                                                  \
     if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB &&                                                   \
                   (CLS < 0 || (A_ == 2) + (B_ == 2) + (C_ == 2) == CLS))                                         \
         if (la == A_ && lb == B_ && lc == C_) {                                                                  \
             triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, lcoef, lalpha, nreal, nlist, geo, ab, ac, g, rA, rB,  \
-                                      rC, B, cacc, cval);                                                        \
+                                      rC, Bc, cacc, cval);                                                        \
             continue;                                                                                            \
         }
                 AOK_L3_CASES(AOK_B3_CASE)
@@ -835,11 +839,12 @@ __global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t 
 // ([side][shell][list entry][coef, exponent], batch innermost): thread per (b, element),
 // a fixed-order sum over the parts that hold the element.  Fully parallel, so the
 // per-slot reduction below then reads one row per element instead of one per part.
-__global__ __launch_bounds__(256) void ovl3_parts_sum(const OvlShape sh, int64_t B, int splitB, int splitC,
-                                                      const double *__restrict__ rows, double *__restrict__ rows1)
+__global__ __launch_bounds__(256) void ovl3_parts_sum(const OvlShape sh, int64_t B, int64_t b0, int64_t Bc,
+                                                      int splitB, int splitC, const double *__restrict__ rows,
+                                                      double *__restrict__ rows1)
 {
-    const int64_t b = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (b >= B) return;
+    const int64_t bl = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (bl >= Bc) return;
     const int ns = sh.ns;
     const int K2 = 2 * sh.K, E = 3 * ns * K2;
     const int nbp = splitB ? ns : 1, ncp = splitC ? ns : 1;
@@ -851,11 +856,11 @@ __global__ __launch_bounds__(256) void ovl3_parts_sum(const OvlShape sh, int64_t
                           : side == 1 ? static_cast<int64_t>(K2) * (ns + (splitB ? 0 : s)) + w
                                       : static_cast<int64_t>(K2) * (ns + nB + (splitC ? 0 : s)) + w;
         const int c0 = side == 2 && splitC ? s : 0, c1 = side == 2 && splitC ? s + 1 : ncp;
-        const int b0 = side == 1 && splitB ? s : 0, b1 = side == 1 && splitB ? s + 1 : nbp;
+        const int p0 = side == 1 && splitB ? s : 0, p1 = side == 1 && splitB ? s + 1 : nbp;
         double acc = 0.0;
         for (int pc = c0; pc < c1; ++pc)
-            for (int pb = b0; pb < b1; ++pb) acc += rows[(static_cast<int64_t>(pc * nbp + pb) * rlen + e) * B + b];
-        rows1[static_cast<int64_t>(e1) * B + b] = acc;
+            for (int pb = p0; pb < p1; ++pb) acc += rows[(static_cast<int64_t>(pc * nbp + pb) * rlen + e) * Bc + bl];
+        rows1[static_cast<int64_t>(e1) * B + b0 + bl] = acc;
     }
 }
 
@@ -1024,24 +1029,31 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     const size_t o_nr = take(sizeof(int) * sh.NZ * sh.ns), o_nl = take(sizeof(int) * sh.NZ * sh.ns);
     const size_t o_li = take(sizeof(int) * npar), o_pc = take(sizeof(int) * npar);
     const size_t o_lc = take(sizeof(double) * npar), o_la = take(sizeof(double) * npar);
-    // parts: by (third, second) shell while the rows fit OVL3_ROWS_CAP, else by third shell,
-    // else none; the "ovl3_parts" knob forces one (1 none, 2 third, 3 third and second)
-    const size_t Bz = static_cast<size_t>(c.B > 0 ? c.B : 1), nsz = sh.ns;
-    auto rows_bytes = [&](int sb_, int sc_) {
+    // parts: by (third, second) shell, else by third shell, whichever is the finest whose
+    // split rows for min(B, 4096) triples fit OVL3_ROWS_CAP, else none; split rows are
+    // processed in batch chunks that fit the cap.  The "ovl3_parts" knob forces a split
+    // (1 none, 2 third, 3 third and second).
+    const int64_t Bn = c.B > 0 ? c.B : 1;
+    const size_t nsz = sh.ns;
+    auto row_bytes = [&](int sb_, int sc_) {  // per triple
         const size_t parts = (sb_ ? nsz : 1) * (sc_ ? nsz : 1);
-        return sizeof(double) * 2 * sh.K * (nsz + (sb_ ? 1 : nsz) + (sc_ ? 1 : nsz)) * parts * Bz;
+        return sizeof(double) * 2 * sh.K * (nsz + (sb_ ? 1 : nsz) + (sc_ ? 1 : nsz)) * parts;
+    };
+    auto chunk_of = [&](int sb_, int sc_) {
+        return std::max<int64_t>(1, static_cast<int64_t>(OVL3_ROWS_CAP / row_bytes(sb_, sc_)));
     };
     int splitB = 1, splitC = 1;
     if (ctx->tune.ovl3_parts != 0) {
         splitC = ctx->tune.ovl3_parts >= 2;
         splitB = ctx->tune.ovl3_parts >= 3;
-    } else if (rows_bytes(1, 1) > OVL3_ROWS_CAP) {
+    } else if (chunk_of(1, 1) < std::min<int64_t>(Bn, 4096)) {
         splitB = 0;
-        if (rows_bytes(0, 1) > OVL3_ROWS_CAP) splitC = 0;
+        if (chunk_of(0, 1) < std::min<int64_t>(Bn, 4096)) splitC = 0;
     }
     const int nparts = (splitB ? sh.ns : 1) * (splitC ? sh.ns : 1);
-    const size_t o_rows = take(rows_bytes(splitB, splitC));
-    const size_t o_rows1 = nparts > 1 ? take(rows_bytes(0, 0)) : o_rows;  // parts folded
+    const int64_t Bc = nparts > 1 ? std::min<int64_t>(Bn, chunk_of(splitB, splitC)) : Bn;
+    const size_t o_rows = take(row_bytes(splitB, splitC) * static_cast<size_t>(Bc));
+    const size_t o_rows1 = nparts > 1 ? take(row_bytes(0, 0) * static_cast<size_t>(Bn)) : o_rows;  // parts folded
     aok_status st = ensure_ws(ctx, o);
     if (st != AOK_OK) return st;
     char *ws = static_cast<char *>(ctx->ws);
@@ -1055,11 +1067,12 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
                        reinterpret_cast<int *>(ws + o_li), reinterpret_cast<int *>(ws + o_pc),
                        reinterpret_cast<double *>(ws + o_lc), reinterpret_cast<double *>(ws + o_la));
     AOK_HIP(ctx, hipGetLastError());
-    if (c.B > 0) {
+    for (int64_t b0 = 0; b0 < c.B; b0 += Bc) {
+        const int64_t n = std::min<int64_t>(Bc, c.B - b0);
         time_begin(ctx, "ovl3c_bwd_rows");
         auto rows_launch = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((c.B + 63) / 64), static_cast<unsigned>(nparts)),
-                               dim3(64), sizeof(double) * 64 * 4 * sh.K, ctx->stream, sh, c.B,
+            hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((n + 63) / 64), static_cast<unsigned>(nparts)),
+                               dim3(64), sizeof(double) * 64 * 4 * sh.K, ctx->stream, sh, b0, n,
                                static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
                                static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
                                c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const double *>(ws + o_lc),
@@ -1080,8 +1093,8 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
         AOK_HIP(ctx, hipGetLastError());
         if (nparts > 1) {
             const int E = 6 * sh.ns * sh.K;
-            hipLaunchKernelGGL(ovl3_parts_sum, dim3(static_cast<unsigned>((c.B + 255) / 256), std::min(E, 65535)),
-                               dim3(256), 0, ctx->stream, sh, c.B, splitB, splitC,
+            hipLaunchKernelGGL(ovl3_parts_sum, dim3(static_cast<unsigned>((n + 255) / 256), std::min(E, 65535)),
+                               dim3(256), 0, ctx->stream, sh, c.B, b0, n, splitB, splitC,
                                reinterpret_cast<const double *>(ws + o_rows), reinterpret_cast<double *>(ws + o_rows1));
             AOK_HIP(ctx, hipGetLastError());
         }

@@ -286,8 +286,9 @@ aok_status aok_graph_destroy(aok_graph graph);
  *              XCD's L2), k >= 3 groups of k - 1 chunks; 0 = default: 2 when N > 32, else 1
  *   "ovl3_parts" 3-centre pullback: split each triple's shell loops into parts run by
  *              separate threads (each with its own FP64 partial row): 1 none, 2 by
- *              third-centre shell, 3 by (third, second) shell pair; 0 = default: 3 while the
- *              rows fit 1 GiB, else 2 while they fit, else 1
+ *              third-centre shell, 3 by (third, second) shell pair; 0 = default: the finest
+ *              split whose rows for min(B, 4096) triples fit 1 GiB; split rows are processed
+ *              in batch chunks that fit 1 GiB
  * Unknown keys or out-of-range values return AOK_ERR_ARG. */
 aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value);
 aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value);

@@ -81,6 +81,33 @@ def test_parts_agree(dev, dtype, tol):
     assert np.array_equal(got[0][0], got[3][0]) and np.array_equal(got[0][1], got[3][1])
 
 
+def test_chunked_batch(dev):
+    """B = 30000 C4a triples: the (third, second)-shell split rows (41 KB per triple) exceed
+    the 1 GiB budget, so the batch runs in two chunks folded into one set of rows; equal
+    to rounding to the unsplit form (ovl3_parts 1), FP32."""
+    B = 30000
+    d = configs.c4a(B=B)
+    cb = d["basis"]
+    dtype = torch.float32
+    coef = rounded(O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls), dtype)
+    basis = _basis(cb.ls, cb.zeta, coef, dtype, dev)
+    g = torch.Generator(device=dev).manual_seed(12)
+    dT = torch.randn((B, cb.N, cb.N, cb.N), generator=g, device=dev, dtype=dtype).permute(3, 2, 1, 0)
+    ctx = aok.Context(dev.index or 0)
+    X = _inputs(d, dtype, dev)
+    got = {}
+    try:
+        for parts in (0, 1):
+            log(f"3c pullback C4a B={B} FP32 ovl3_parts={parts}")
+            ctx.tune("ovl3_parts", parts)
+            dc, da = aok.overlap_3c_pb_coef_alpha(dT, basis, *X, ctx=ctx)
+            got[parts] = (host(dc), host(da))
+    finally:
+        ctx.tune("ovl3_parts", 0)
+    assert rel_max(got[0][0], got[1][0]) < 1e-6
+    assert rel_max(got[0][1], got[1][1]) < 1e-6
+
+
 def test_orbital_rrule_vs_oracle(dev):
     """(zeta, D) cotangents through the FP64 normalisation VJP, FP64."""
     B = 40
