@@ -759,8 +759,18 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
 // CLS >= 0: only the shell triples with CLS d shells (one instance per class, so that
 // each is compiled for the registers of its own cases; CLS 0 clears the rows and runs
 // first), CLS < 0: every triple.
+// Waves per SIMD the FP32 instances are compiled for: 2 for the triples with no d shell
+// (222 VGPRs) and with one (256 VGPRs and an 88-byte spill instead of 288 registers at
+// 1 wave: C4a rows 4.58 -> 4.32 ms); forcing 3 on the first (212-byte spill) or 2 on the
+// two-d class measured slower.  Other instances keep the compiler's choice.
 template <typename T, int LMAXB, int CLS>
-__global__ __launch_bounds__(64) void ovl3c_bwd_rows(const OvlShape sh, int64_t b0, int64_t Bc,
+constexpr int ovl3_pb_waves()
+{
+    return std::is_same<T, float>::value && LMAXB == 2 && (CLS == 0 || CLS == 1) ? 2 : 1;
+}
+
+template <typename T, int LMAXB, int CLS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_pb_waves<T, LMAXB, CLS>(), 8))) void ovl3c_bwd_rows(const OvlShape sh, int64_t b0, int64_t Bc,
                                                      const T *__restrict__ dT,
                                                      const T *__restrict__ posA, const T *__restrict__ posB,
                                                      const T *__restrict__ posC, T scale, const void *__restrict__ sA,
