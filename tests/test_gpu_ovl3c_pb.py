@@ -108,6 +108,33 @@ def test_chunked_batch(dev):
     assert rel_max(got[0][1], got[1][1]) < 1e-6
 
 
+@pytest.mark.parametrize("ls", [[0, 0], [0, 1, 0]])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
+def test_small_l_instances_vs_oracle(dev, ls, dtype, tol):
+    """Bases without d shells take the single rows-kernel instance compiled for l <= 1 or
+    l = 0 (C4a's d shells take the per-class ones); one padded slot."""
+    B, K, NZ = 50, 3, 2
+    ns = len(ls)
+    rng = np.random.default_rng(21 + ns)
+    zeta = np.exp(rng.uniform(np.log(0.2), np.log(20.0), size=(ns, K, NZ)))
+    D = rng.uniform(-1.0, 1.0, size=(ns, K, NZ))
+    D[0, 2, 1] = 0.0
+    log(f"3c pullback ls={ls} B={B} dtype={dtype}")
+    coef = rounded(O.compile_coef(rounded(zeta, dtype), rounded(D, dtype), ls), dtype)
+    basis = _basis(ls, zeta, coef, dtype, dev)
+    pos = rng.uniform(-1.5, 1.5, size=(3, B, 3))
+    sp = rng.integers(1, NZ + 1, size=(3, B))
+    d = {"posA": pos[0], "posB": pos[1], "posC": pos[2], "sidxA": sp[0], "sidxB": sp[1], "sidxC": sp[2]}
+    N = sum((l + 1) * (l + 2) // 2 for l in ls)
+    dT = rng.standard_normal((N, N, N, B))
+    dc, da = aok.overlap_3c_pb_coef_alpha(dev_f(dT, dtype, dev), basis, *_inputs(d, dtype, dev))
+    rc, ra = O.ovl3c_pb(rounded(dT, dtype), *(rounded(d[f"pos{c}"], dtype) for c in "ABC"), d["sidxA"], d["sidxB"],
+                        d["sidxC"], ls, coef, rounded(zeta, dtype))
+    assert rel_max(host(dc), rc) < tol
+    assert rel_max(host(da), ra) < tol
+    assert host(da)[0, 2, 1] == 0.0 and host(dc)[0, 2, 1] != 0.0
+
+
 def test_orbital_rrule_vs_oracle(dev):
     """(zeta, D) cotangents through the FP64 normalisation VJP, FP64."""
     B = 40
