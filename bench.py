@@ -454,9 +454,20 @@ def main():
             ks.append(ctx.last_kernel_ms())
         torch.cuda.synchronize()
         wall3b = (time.perf_counter() - ts) * 1e3 / 3
+        # FP32 against the FP64 pullback of the same FP32-rounded inputs, relative to the
+        # largest |cotangent| of each output
+        dc32, da32 = aok.overlap_3c_pb_coef_alpha(dT, basis3, *X3, ctx=ctx)
+        basis3d = aok.CartesianGTOBasis(2, cb.ns, cb.K, cb.ls, nbf3, off3, off3[-1],
+                                        aok.as_fortran(orb32.Rnl.zeta.double()), aok.as_fortran(coef3.double()),
+                                        zlist, 1.0)
+        X3d = [(x[0].double().contiguous(), x[1]) for x in X3]
+        dc64, da64 = aok.overlap_3c_pb_coef_alpha(aok.as_fortran(dT.double()), basis3d, *X3d, ctx=ctx)
+        err3 = max(float((dc32.double() - dc64).abs().max() / dc64.abs().max()),
+                   float((da32.double() - da64).abs().max() / da64.abs().max()))
         out["ovl3c_c4a_pullback"] = {"B": B3, "kernel": "ovl3c_bwd_rows", "kernel_ms": float(np.mean(ks)),
-                                     "wall_ms": wall3b, "triples_per_s": B3 / (wall3b * 1e-3)}
-        del V, dT
+                                     "wall_ms": wall3b, "triples_per_s": B3 / (wall3b * 1e-3),
+                                     "fp32_max_rel_err": err3}
+        del V, dT, dc64, da64
 
     if not args.no_cpu:
         log("CPU baseline (C oracle, FP64, 1 thread)")
