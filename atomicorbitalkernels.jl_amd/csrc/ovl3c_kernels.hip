@@ -874,15 +874,14 @@ __global__ __launch_bounds__(256) void ovl3_parts_sum(const OvlShape sh, int64_t
     }
 }
 
-// Block per slot (sigma, s, k): sum over the batch of the rows whose species on that
-// side is sigma, sides A, B, C, each over the parts holding that shell's row (A: every
-// part; B, C: the parts of that shell when split by it, else every part); thread-strided
-// partial sums in a fixed order, then a block tree.
+// Block per slot (sigma, s, k): sum over the batch of the unsplit rows (parts already
+// folded) whose species on that side is sigma, sides A, B, C; thread-strided partial
+// sums in a fixed order, then a block tree.
 template <typename T>
 __global__ __launch_bounds__(256) void ovl3_bwd_reduce(const OvlShape sh, int64_t B, const void *__restrict__ sA,
                                                        const void *__restrict__ sB, const void *__restrict__ sC,
-                                                       int itype, const int *__restrict__ pcls, int splitB,
-                                                       int splitC, const double *__restrict__ rows, T *__restrict__ dcoef,
+                                                       int itype, const int *__restrict__ pcls,
+                                                       const double *__restrict__ rows, T *__restrict__ dcoef,
                                                        T *__restrict__ dalpha)
 {
     __shared__ double red[8];
@@ -890,26 +889,15 @@ __global__ __launch_bounds__(256) void ovl3_bwd_reduce(const OvlShape sh, int64_
     const int q = blockIdx.x;  // [s + ns * (k + K * sg)], the Julia index
     const int s = q % ns, k = (q / ns) % K, sg = q / (ns * K);
     const int li = pcls[static_cast<int64_t>(sg * ns + s) * K + k];
-    const int nbp = splitB ? ns : 1, ncp = splitC ? ns : 1;
-    const int nB = splitB ? 1 : ns, nC = splitC ? 1 : ns;
     const int64_t K2 = 2 * static_cast<int64_t>(K);
-    const int64_t rlen = K2 * (ns + nB + nC);
-    // element of this slot per side, and the parts [c0, c1) x [b0, b1) that hold it
-    const int64_t e[3] = {K2 * s + 2 * li, K2 * (ns + (splitB ? 0 : s)) + 2 * li,
-                          K2 * (ns + nB + (splitC ? 0 : s)) + 2 * li};
-    const int c0[3] = {0, 0, splitC ? s : 0}, c1[3] = {ncp, ncp, splitC ? s + 1 : ncp};
-    const int b0[3] = {0, splitB ? s : 0, 0}, b1[3] = {nbp, splitB ? s + 1 : nbp, nbp};
     const void *sides[3] = {sA, sB, sC};
     double c = 0.0, a = 0.0;
     for (int64_t b = threadIdx.x; b < B; b += 256)
         for (int side = 0; side < 3; ++side) {
             if (load_sigma(sides[side], itype, b) != sg) continue;
-            for (int pc = c0[side]; pc < c1[side]; ++pc)
-                for (int pb = b0[side]; pb < b1[side]; ++pb) {
-                    const double *r = rows + (static_cast<int64_t>(pc * nbp + pb) * rlen + e[side]) * B + b;
-                    c += r[0];
-                    a += r[B];
-                }
+            const double *r = rows + (K2 * (side * ns + s) + 2 * li) * B + b;
+            c += r[0];
+            a += r[B];
         }
     block_sum256x2(c, a, red);
     if (threadIdx.x == 0) {
@@ -1110,7 +1098,7 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
         }
     }
     hipLaunchKernelGGL((ovl3_bwd_reduce<T>), dim3(static_cast<unsigned>(npar)), dim3(256), 0, ctx->stream, sh, c.B,
-                       c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const int *>(ws + o_pc), 0, 0,
+                       c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const int *>(ws + o_pc),
                        reinterpret_cast<const double *>(ws + o_rows1), static_cast<T *>(c.o1),
                        static_cast<T *>(c.o2));
     AOK_HIP(ctx, hipGetLastError());
