@@ -353,8 +353,16 @@ __device__ __forceinline__ void triple_fwd(const typename Rec3Of<T>::type *__res
         X(1, 0, 1) X(1, 0, 2) X(1, 1, 0) X(1, 1, 1) X(1, 1, 2) X(1, 2, 0) X(1, 2, 1) X(1, 2, 2) X(2, 0, 0)       \
             X(2, 0, 1) X(2, 0, 2) X(2, 1, 0) X(2, 1, 1) X(2, 1, 2) X(2, 2, 0) X(2, 2, 1) X(2, 2, 2)
 
+// The FP32 l <= 2 instance is compiled for 2 waves/SIMD (256 VGPRs and a 12-byte spill
+// instead of 1 wave: C4a 0.795 -> 0.761 ms); the others keep the compiler's choice.
 template <typename T, int LMAXB>
-__global__ __launch_bounds__(64) void ovl3c_fwd_fast(const OvlShape sh, const T *__restrict__ posA,
+constexpr int ovl3_fwd_waves()
+{
+    return std::is_same<T, float>::value && LMAXB == 2 ? 2 : 1;
+}
+
+template <typename T, int LMAXB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_waves<T, LMAXB>(), 8))) void ovl3c_fwd_fast(const OvlShape sh, const T *__restrict__ posA,
                                                      const T *__restrict__ posB, const T *__restrict__ posC, T scale,
                                                      const int *__restrict__ perm, const int *__restrict__ tile_key,
                                                      const int *__restrict__ boff, const int *__restrict__ bcnt,
