@@ -29,7 +29,7 @@ FP32_PEAK_TFLOPS = 157.3
 # runs of this bench's C1 step (--no-extra; KiB per dispatch, mean over the launches), each
 # scaled by what the same counter reads for the bench's 2 GiB device copy (FETCH_SIZE reads
 # 1048589 KiB for 2097152 KiB moved: x2.0; WRITE_SIZE reads 2097152 KiB: x1.0).
-PMC_KIB = {"ovl2c_bwd_fast": (126522.0, 10335.0), "ovl2c_fwd_fast": (9193.0, 114989.0)}
+PMC_KIB = {"ovl2c_bwd_fast": (126480.0, 10333.0), "ovl2c_fwd_fast": (8939.0, 114411.0)}
 PMC_SCALE = (2.0, 1.0)
 # FLOPs per pair the C1 kernels execute: rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32 and
 # SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes), per launch / 65536 pairs.
