@@ -234,3 +234,22 @@ def c4b(seed=4, nX=2**24):
     X = rng.normal(0.0, 3.0, size=(nX, 3))
     sidx = rng.integers(1, 4, size=nX)
     return {"basis": basis, "X": X, "sidx": sidx, "L": 3}
+
+
+# ---- g shells: an AO / overlap basis with every l up to 4 -------------------------------------
+G_LS = [0, 0, 1, 1, 2, 3, 4]  # 2s2p1d1f1g -> 29 orbitals, 39 Cartesian functions
+
+
+def g_basis(seed=5, NZ=3, K=4, zmax=50.0):
+    """Species with s..g shells, K primitives each (depths 1..K drawn per shell, so the
+    padded (zeta = 1, D = 0) slots of the union layout are exercised), zeta ~ logU[0.1, zmax]."""
+    rng = np.random.default_rng(seed + 5000)
+    species = []
+    for _ in range(NZ):
+        shells = []
+        for l in G_LS:
+            depth = int(rng.integers(1, K + 1))
+            z = np.sort(_logu(rng, 0.1, zmax, depth))[::-1]
+            shells.append((z, rng.standard_normal(depth)))
+        species.append(shells)
+    return _pad_basis(G_LS, species, K)

@@ -668,7 +668,7 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
-    const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
+    const int64_t b = perm[ti.first + (lane < ti.nact ? lane : 0)];
     bl_off[lane] = static_cast<int64_t>(b) * N * N;
     const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
     const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
@@ -750,8 +750,8 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     const int lane = threadIdx.x;
     for (int q = lane; q < pstride; q += OVL_TILE) accA[q] = 0.0;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
-    const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
-    bl_b[lane] = b;
+    const int64_t b = perm[ti.first + (lane < ti.nact ? lane : 0)];
+    bl_b[lane] = static_cast<int>(b);  // b < 2^30 (the batch cap)
     const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
     const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
     const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);
@@ -838,8 +838,8 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
-    const int b = perm[ti.first + (lane < ti.nact ? lane : 0)];
-    bl_b[lane] = b;
+    const int64_t b = perm[ti.first + (lane < ti.nact ? lane : 0)];
+    bl_b[lane] = static_cast<int>(b);  // b < 2^30 (the batch cap)
     const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
     const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
     const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);

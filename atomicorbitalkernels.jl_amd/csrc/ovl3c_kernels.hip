@@ -381,7 +381,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_wav
     const int nact = min(OVL_TILE, boff[key] + bcnt[key] - first);
     const int lane = threadIdx.x;
     const bool active = lane < nact;
-    const int b = perm[first + (active ? lane : 0)];
+    const int64_t b = perm[first + (active ? lane : 0)];
     const int sgA = key / (NZ * NZ), sgB = (key / NZ) % NZ, sgC = key % NZ;
     T A[3], Bv[3], C[3];
 #pragma unroll

@@ -3,6 +3,11 @@ independent of both the oracle and the kernels.
 
 * Z_lm for l <= 2 from the closed forms of the SpheriCart convention (the reference
   stores harmonic (l, m) at l^2 + l + m + 1; test/orbitals/test_gaussianbasis.jl:40).
+* Z_lm for l <= 4 from the textbook definition of the real solid harmonics (associated
+  Legendre functions without the Condon-Shortley phase, sin |m| phi for m < 0, cos m phi for
+  m > 0, r^l Y_lm, L2-normalised on the sphere), cross-checked against the published
+  polynomial tables for l = 3 and l = 4 before anything is written.  This is the convention
+  SpheriCart documents (the reference's harmonics, called at src/orbitals/atomicorbitals.jl:155,200).
 * 2-centre contracted Cartesian overlaps by Gauss-Hermite quadrature of the true
   primitive overlaps, composed with the reference's contraction/read-back alias map
   (blk[mu + (2 l_a + 1) nu] += S_true[mu, nu]; out[i, j] = blk[i + nbf_a j],
@@ -15,8 +20,10 @@ Run:  python tests/golden/make_golden.py
 """
 import json
 import os
+from math import factorial, pi, sqrt
 
 import numpy as np
+from numpy.polynomial import legendre as _leg
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GH_T, GH_W = np.polynomial.hermite.hermgauss(24)
@@ -101,6 +108,51 @@ def ylm_closed(x, y, z):
             1.0925484305920792 * x * z, 0.5462742152960396 * (x * x - y * y)]
 
 
+def zlm_definition(l, m, p):
+    """r^l Y_lm from the definition: N_l|m| r^(l-|m|) P_l^(|m|)(z/r) times 1 (m = 0),
+    sqrt(2) Re (x + iy)^|m| (m > 0) or sqrt(2) Im (x + iy)^|m| (m < 0), where P_l^(|m|) is
+    the |m|-th derivative of the Legendre polynomial and N_lk = sqrt((2l+1)/(4 pi) (l-k)!/(l+k)!)."""
+    x, y, z = p
+    r = np.sqrt(x * x + y * y + z * z)
+    am = abs(m)
+    norm = sqrt((2 * l + 1) / (4 * pi) * factorial(l - am) / factorial(l + am))
+    dleg = _leg.Legendre.basis(l).deriv(am)(z / r)
+    c = complex(x, y) ** am
+    ang = 1.0 if m == 0 else sqrt(2.0) * (c.real if m > 0 else c.imag)
+    return norm * r ** (l - am) * dleg * ang
+
+
+def zlm_table_l34(x, y, z):
+    """Published polynomial forms of the real solid harmonics, l = 3 (m = -3..3) then
+    l = 4 (m = -4..4)."""
+    r2 = x * x + y * y + z * z
+    l3 = [0.25 * sqrt(35 / (2 * pi)) * y * (3 * x * x - y * y), 0.5 * sqrt(105 / pi) * x * y * z,
+          0.25 * sqrt(21 / (2 * pi)) * y * (5 * z * z - r2), 0.25 * sqrt(7 / pi) * z * (5 * z * z - 3 * r2),
+          0.25 * sqrt(21 / (2 * pi)) * x * (5 * z * z - r2), 0.25 * sqrt(105 / pi) * (x * x - y * y) * z,
+          0.25 * sqrt(35 / (2 * pi)) * x * (x * x - 3 * y * y)]
+    l4 = [0.75 * sqrt(35 / pi) * x * y * (x * x - y * y), 0.75 * sqrt(35 / (2 * pi)) * y * z * (3 * x * x - y * y),
+          0.75 * sqrt(5 / pi) * x * y * (7 * z * z - r2), 0.75 * sqrt(5 / (2 * pi)) * y * z * (7 * z * z - 3 * r2),
+          3 / 16 * sqrt(1 / pi) * (35 * z ** 4 - 30 * z * z * r2 + 3 * r2 * r2),
+          0.75 * sqrt(5 / (2 * pi)) * x * z * (7 * z * z - 3 * r2), 3 / 8 * sqrt(5 / pi) * (x * x - y * y) * (7 * z * z - r2),
+          0.75 * sqrt(35 / (2 * pi)) * x * z * (x * x - 3 * y * y),
+          3 / 16 * sqrt(35 / pi) * (x * x * (x * x - 3 * y * y) - y * y * (3 * x * x - y * y))]
+    return l3 + l4
+
+
+def ylm_l4_fixture(seed=20261204, npts=8):
+    rng = np.random.default_rng(seed)
+    pts = rng.normal(0, 1.3, size=(npts, 3))
+    vals = []
+    for p in pts:
+        v = [zlm_definition(l, m, p) for l in range(5) for m in range(-l, l + 1)]
+        tab = zlm_table_l34(*p)
+        scale = max(1.0, float(np.max(np.abs(v))))
+        assert np.max(np.abs(np.array(v[9:]) - tab)) < 1e-13 * scale, "l = 3, 4 tables disagree with the definition"
+        assert np.max(np.abs(np.array(v[:9]) - ylm_closed(*p))) < 1e-13 * scale
+        vals.append(v)
+    return {"points": pts.tolist(), "values": vals}
+
+
 def main():
     rng = np.random.default_rng(20261203)
     out = {"ang2bohr": 1.8897261246257702}
@@ -139,6 +191,7 @@ def main():
                    "A": A.tolist(), "B": B.tolist(), "C": C.tolist(),
                    "out": alias_3c(V, la, lb).reshape(-1, order="F").tolist()})
     out["blocks_3c"] = b3
+    out["ylm_l4"] = ylm_l4_fixture()
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote", os.path.join(HERE, "golden.json"))

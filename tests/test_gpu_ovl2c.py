@@ -334,3 +334,23 @@ def test_inputs_produced_by_torch_just_before_the_call(dev):
         torch.cuda.synchronize()
         b = pb(dS)[4]["Rnl"]["zeta"]
         assert torch.equal(a, b), seed
+
+
+def test_batch_cap_rejected_before_launch(dev):
+    """Batches beyond the documented caps (2^30 pairs for 2c, 2^28 triples for 3c) are
+    refused by argument validation, before any workspace or kernel (positions are then
+    addressed in 64-bit arithmetic everywhere below the cap).  Small real buffers are
+    passed: nothing reads them."""
+    cb = configs.c1_basis(1)
+    log("2c/3c batch caps (argument validation only)")
+    ctx = aok.Context(dev.index or 0)
+    buf = torch.zeros(64, dtype=torch.float32, device=dev)
+    p = buf.data_ptr()
+    ls = aok._lib.i64_array(cb.ls)
+    with pytest.raises(aok.AOKArgumentError):
+        ctx.call("aok_ovl2c", aok._lib.AOK_F32, 0, p, p, None, None, aok._lib.AOK_I64, (1 << 30) + 1, cb.ns, cb.K,
+                 cb.NZ, ls, p, p, 1.0, p)
+    with pytest.raises(aok.AOKArgumentError):
+        ctx.call("aok_ovl3c", aok._lib.AOK_F32, 0, p, p, p, None, None, None, aok._lib.AOK_I64, (1 << 28) + 1, cb.ns,
+                 cb.K, cb.NZ, ls, p, p, 1.0, p)
+    ctx.close()

@@ -309,6 +309,19 @@ def test_ylm_closed_forms_l2():
         assert np.max(np.abs(Z - np.array(vals))) < 1e-14
 
 
+def test_ylm_l3_l4_published_forms():
+    """m-order and sign of every harmonic up to l = 4 against the textbook definition
+    (associated Legendre without the Condon-Shortley phase, sin for m < 0, cos for m > 0),
+    itself checked against the published l = 3, 4 polynomial tables when the fixture is made.
+    A permuted m, a flipped sign or a cos/sin swap at any (l, m) fails here."""
+    g = GOLD["ylm_l4"]
+    for p, vals in zip(g["points"], g["values"]):
+        Z, _ = O.ylm(4, p)
+        ref = np.array(vals)
+        assert Z.shape == ref.shape == (25,)
+        assert np.max(np.abs(Z - ref)) < 1e-13 * max(1.0, np.max(np.abs(ref)))
+
+
 def test_ylm_orthonormal_on_sphere():
     """L2 normalisation and orthogonality for l <= 4 by product quadrature."""
     L = 4
