@@ -99,6 +99,8 @@ _SIGS = {
     "aok_compile_coef": [_vp, _int, _vp, _vp, _i64, _i64, _i64, _i64p, _vp],
     "aok_compile_coef_pb": [_vp, _int, _vp, _vp, _vp, _i64, _i64, _i64, _i64p, _vp, _vp],
     "aok_ovl2c": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp, _dbl, _vp],
+    "aok_ovl2c_noalloc": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp, _dbl,
+                          _vp],
     "aok_ovl2c_ps": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp, _dbl,
                      _vp],
     "aok_ovl2c_pullback": [_vp, _int, _uint, _vp, _vp, _vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _i64p, _vp, _vp,

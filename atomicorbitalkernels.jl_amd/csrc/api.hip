@@ -197,6 +197,10 @@ aok_status aok_create(aok_ctx *out, int device, void *stream)
         return AOK_ERR_DEVICE;
     }
     *reinterpret_cast<volatile unsigned *>(c->h_err) = 0u;  // pinned memory comes uncleared
+    // this device's copy of the allocation-free entry point's scratch array; a failure
+    // here is reported by aok_ovl2c_noalloc, which repeats the lookup
+    (void)ovl2c_noalloc_setup(c);
+    c->last_error.clear();
     *out = c;
     return AOK_OK;
 }
@@ -433,7 +437,8 @@ aok_status aok_compile_coef_pb(aok_ctx ctx, aok_dtype dtype, const void *dcoef, 
 // op: 0 forward, 1 parameter pullback, 2 position cotangents
 static aok_status ovl2c_entry(aok_ctx ctx, aok_dtype dtype, Ovl2cCall &c, int op, const char *what)
 {
-    const bool pullback = op != 0;
+    // op: 0 forward, 1 (coef, alpha) pullback, 2 position cotangents, 3 allocation-free forward
+    const bool pullback = op == 1 || op == 2;
     AOK_REQUIRE(ctx, !bad_dtype(dtype), "dtype must be AOK_F32 or AOK_F64");
     AOK_REQUIRE(ctx, !bad_itype(static_cast<aok_itype>(c.itype)), "itype must be AOK_I64 or AOK_I32");
     AOK_REQUIRE(ctx, (c.flags & ~AOK_PLAIN) == 0u, "unknown flags");
@@ -449,6 +454,8 @@ static aok_status ovl2c_entry(aok_ctx ctx, aok_dtype dtype, Ovl2cCall &c, int op
     if (st != AOK_OK) return st;
     if (op == 2)
         st = dtype == AOK_F32 ? ovl2c_dx_run<float>(ctx, c) : ovl2c_dx_run<double>(ctx, c);
+    else if (op == 3)
+        st = dtype == AOK_F32 ? ovl2c_run_noalloc<float>(ctx, c) : ovl2c_run_noalloc<double>(ctx, c);
     else
         st = dtype == AOK_F32 ? ovl2c_run<float>(ctx, c, pullback) : ovl2c_run<double>(ctx, c, pullback);
     if (st != AOK_OK) return st;
@@ -467,6 +474,20 @@ aok_status aok_ovl2c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *p
     c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.coef = coef, c.alpha = alpha, c.mode = 0, c.scale = lengthscale;
     c.out = out;
     return ovl2c_entry(ctx, dtype, c, 0, "aok_ovl2c");
+}
+
+aok_status aok_ovl2c_noalloc(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *posA, const void *posB,
+                             const void *sidxA, const void *sidxB, aok_itype itype, int64_t B, int64_t ns,
+                             int64_t K, int64_t NZ, const int64_t *ls, const void *coef, const void *alpha,
+                             double lengthscale, void *out)
+{
+    AOK_CHECK_CTX(ctx);
+    Ovl2cCall c{};
+    c.flags = flags;
+    c.posA = posA, c.posB = posB, c.sidxA = sidxA, c.sidxB = sidxB, c.itype = itype, c.B = B;
+    c.ls = ls, c.ns = ns, c.K = K, c.NZ = NZ, c.coef = coef, c.alpha = alpha, c.mode = 0, c.scale = lengthscale;
+    c.out = out;
+    return ovl2c_entry(ctx, dtype, c, 3, "aok_ovl2c_noalloc");
 }
 
 aok_status aok_ovl2c_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *posA, const void *posB,

@@ -20,6 +20,8 @@
 // Plain path (AOK_PLAIN): the reference's own decomposition -- one work item per
 // (b, s_a, s_b), runtime l loops, private E scratch, three exps per primitive pair,
 // float atomics in the pullback -- kept as the baseline the fast path is measured against.
+#include <mutex>
+
 #include "aok_common.hpp"
 #include "ovl_plan.hpp"
 #include "ovl_prep.hpp"
@@ -1608,6 +1610,59 @@ aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback)
 }
 
 // ---------------------------------------------------------------------------------
+// Allocation-free forward (aok_ovl2c_noalloc): the same prep and forward kernels, with
+// the bucketing tables and primitive-pair records in a fixed per-device array instead of
+// the context workspace.  Host code never passes `scratch` itself (its host-side name is
+// not a device address): each device's copy is looked up once with hipGetSymbolAddress,
+// with that device current, and kept in scratch_ptr.
+// ---------------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[AOK_NOALLOC_SCRATCH_BYTES];
+static std::mutex scratch_mu;
+static void *scratch_ptr[NOALLOC_MAX_DEVICES];
+
+aok_status ovl2c_noalloc_setup(aok_ctx ctx)
+{
+    const int d = ctx->device;
+    if (d < 0 || d >= NOALLOC_MAX_DEVICES) return fail(ctx, AOK_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[d]) return AOK_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        return fail(ctx, AOK_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    scratch_ptr[d] = p;
+    return AOK_OK;
+}
+
+template <typename T>
+aok_status ovl2c_run_noalloc(aok_ctx ctx, const Ovl2cCall &c)
+{
+    if (c.flags & AOK_PLAIN) return fail(ctx, AOK_ERR_ARG, "aok_ovl2c_noalloc runs the fast kernels only (flags 0)");
+    OvlShape sh;
+    aok_status st = shape_of(ctx, c, sh, false);
+    if (st != AOK_OK) return st;
+    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), false);
+    if (w.total > AOK_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, AOK_ERR_ARG,
+                    "aok_ovl2c_noalloc: this batch needs " + std::to_string(w.total) +
+                        " bytes of scratch, the fixed maximum is " + std::to_string(size_t(AOK_NOALLOC_SCRATCH_BYTES)) +
+                        ": split the batch");
+    if ((st = ovl2c_noalloc_setup(ctx)) != AOK_OK) return st;
+    char *ws = static_cast<char *>(scratch_ptr[ctx->device]);
+    if (c.B == 0) return AOK_OK;
+    // untimed: timing would draw events from the context pool, which may create some
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    st = fast_prep<T>(ctx, c, sh, w, ws);
+    if (st == AOK_OK) st = fast_forward<T>(ctx, c, sh, w, ws);
+    ctx->timing = timing;
+    return st;
+}
+
+// ---------------------------------------------------------------------------------
 // Plans: the species-pair bucketing and the primitive-pair records of one (sidx, params)
 // set, built once and reused by forward and pullback calls (the rrule's forward and
 // its pullback see the same batch and parameters).
@@ -1750,6 +1805,8 @@ template aok_status ovl2c_dx_run<double>(aok_ctx, const Ovl2cCall &);
 template aok_status ovl2c_plan_dx<float>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &);
 template aok_status ovl2c_plan_dx<double>(aok_ctx, const Ovl2cPlanState &, const Ovl2cCall &);
 template aok_status ovl2c_run<float>(aok_ctx, const Ovl2cCall &, bool);
+template aok_status ovl2c_run_noalloc<float>(aok_ctx, const Ovl2cCall &);
+template aok_status ovl2c_run_noalloc<double>(aok_ctx, const Ovl2cCall &);
 template aok_status ovl2c_run<double>(aok_ctx, const Ovl2cCall &, bool);
 template aok_status compile_coef_run<float>(aok_ctx, const void *, const void *, int64_t, int64_t, int64_t,
                                             const int64_t *, void *);

@@ -251,6 +251,10 @@ struct Ovl2cPlanState {
 template <typename T>
 aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback);
 template <typename T>
+aok_status ovl2c_run_noalloc(aok_ctx ctx, const Ovl2cCall &c);
+// Look up this device's copy of the fixed scratch array (device current); once per device.
+aok_status ovl2c_noalloc_setup(aok_ctx ctx);
+template <typename T>
 aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p);
 template <typename T>
 aok_status ovl2c_plan_refresh(aok_ctx ctx, Ovl2cPlanState &p);

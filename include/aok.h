@@ -150,6 +150,25 @@ aok_status aok_ovl2c_ps(aok_ctx ctx, aok_dtype dtype, unsigned flags,
                         int64_t ns, int64_t K, int64_t NZ, const int64_t *ls,
                         const void *zeta, const void *D, double lengthscale, void *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * forward of aok_ovl2c (fast kernels, flags must be 0), same arguments and results, with
+ * its bucketing tables and primitive-pair records in a fixed per-device array of
+ * AOK_NOALLOC_SCRATCH_BYTES that lives until the process ends, instead of the context
+ * workspace.  It calls no hipMalloc/hipFree.  A batch whose tables exceed the array
+ * (about 8 bytes per pair plus 32 bytes per (species pair, shell pair, primitive pair):
+ * some 4 million pairs of a 3-species, 6-shell, 9-primitive basis) fails with
+ * AOK_ERR_ARG before any launch; split it.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other contexts, other streams, other host
+ * threads) race and must be serialised by the caller.  Untimed (aok_set_timing does not
+ * apply). */
+#define AOK_NOALLOC_SCRATCH_BYTES (32u << 20)
+aok_status aok_ovl2c_noalloc(aok_ctx ctx, aok_dtype dtype, unsigned flags,
+                             const void *posA, const void *posB,
+                             const void *sidxA, const void *sidxB, aok_itype itype, int64_t B,
+                             int64_t ns, int64_t K, int64_t NZ, const int64_t *ls,
+                             const void *coef, const void *alpha, double lengthscale, void *out);
+
 /* Replaces _overlap_pb_coef_alpha(dS, basis, XA, XB) (overlap_2c_grad.jl:218-242):
  * dcoef, dalpha [ns*K*NZ] (dtype) from the cotangent dS [N*N*B].  Every primitive
  * slot receives its cotangent, padded (coef == 0) slots included
