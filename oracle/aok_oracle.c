@@ -4,8 +4,8 @@
  * TEST INFRASTRUCTURE ONLY (the correctness oracle and the timed CPU baseline);
  * the product library never links or loads it.  Each function names the reference
  * file:line whose semantics it restates.  Scalar loops in plain C; the only
- * parallel code is orc_ovl2c_omp, the OpenMP split of the same work items that
- * stands in for KernelAbstractions' CPU backend in the CPU-baseline timing.
+ * parallel code is orc_ovl2c_omp / orc_ovl2c_pb_omp, the OpenMP split of the same work
+ * items that stands in for KernelAbstractions' CPU backend in the CPU-baseline timing.
  */
 #include "aok_oracle.h"
 
@@ -448,9 +448,74 @@ int orc_num_threads(void)
 #endif
 }
 
-/* batch_S_orbital_pb_kernel!  overlap_2c_grad.jl:71-214: every K x K primitive
- * pair (padded ones included), E tables to l+2, contraction against the
- * aliased read-back position of dS, four accumulations per pair. */
+/* batch_S_orbital_pb_kernel!  overlap_2c_grad.jl:71-214, one work item (b, s_a, s_b):
+ * every K x K primitive pair (padded ones included), E tables to l+2, contraction
+ * against the aliased read-back position of dS, four accumulations per pair (into the
+ * given arrays; the reference's atomics). */
+static void ovl2c_pb_item(const double *dS, const double *posA, const double *posB,
+                          const int64_t *sidxA, const int64_t *sidxB, int64_t b, int64_t sa, int64_t sb,
+                          int64_t ns, int64_t K, const int64_t *ls, const int64_t *nbf, const int64_t *off,
+                          int64_t N, const double *coef, const double *alpha, double *Ex,
+                          double *dcoef, double *dalpha)
+{
+    double *Ey = Ex + ORC_T * (ORC_LE + 1) * (ORC_LE + 1), *Ez = Ey + ORC_T * (ORC_LE + 1) * (ORC_LE + 1);
+    const int64_t sgA = sidxA ? sidxA[b] - 1 : 0, sgB = sidxB ? sidxB[b] - 1 : 0;
+    const double *A = posA + 3 * b, *Bv = posB + 3 * b;
+    const double *g_b = dS + N * N * b;
+    const int la = (int)ls[sa], lb = (int)ls[sb];
+    const int64_t nba = nbf[sa];
+    const int N1 = 2 * la + 1;
+    for (int64_t ip = 0; ip < K; ++ip)
+        for (int64_t jp = 0; jp < K; ++jp) {
+            const int64_t qa = sa + ns * (ip + K * sgA), qb = sb + ns * (jp + K * sgB);
+            const double ca = coef[qa], cb = coef[qb];
+            const double aa = alpha[qa], ab = alpha[qb];
+            const double p = aa + ab, mu = aa * ab / p, oo2p = 0.5 / p;
+            const double pref0 = pow(ORC_PI / p, 1.5);
+            double *Es[3] = {Ex, Ey, Ez};
+            for (int ax = 0; ax < 3; ++ax) {
+                const double P = (aa * A[ax] + ab * Bv[ax]) / p;
+                const double AB = A[ax] - Bv[ax];
+                md_etab(Es[ax], la + 2, lb + 2, P - A[ax], P - Bv[ax], oo2p, exp(-mu * AB * AB));
+            }
+            double Bsum = 0.0, Aa = 0.0, Ab = 0.0;
+            int index1 = 0;
+            for (int ll1 = la; ll1 >= 0; --ll1)
+                for (int n1 = 0; n1 <= la - ll1; ++n1) {
+                    const int m1 = la - ll1 - n1;
+                    int index2 = 0;
+                    for (int ll2 = lb; ll2 >= 0; --ll2)
+                        for (int n2 = 0; n2 <= lb - ll2; ++n2) {
+                            const int m2 = lb - ll2 - n2;
+                            const int lin = index1 + N1 * index2;
+                            const int64_t i = lin % nba, j = lin / nba;
+                            const double g = g_b[(off[sa] + i) + N * (off[sb] + j)];
+                            const int LA = la + 2, LB = lb + 2;
+                            const double ex = e0(Ex, LA, LB, ll1, ll2), ey = e0(Ey, LA, LB, m1, m2),
+                                         ez = e0(Ez, LA, LB, n1, n2);
+                            const double eprod = ex * ey * ez;
+                            const double dGa = -(e0(Ex, LA, LB, ll1 + 2, ll2) * ey * ez +
+                                                 ex * e0(Ey, LA, LB, m1 + 2, m2) * ez +
+                                                 ex * ey * e0(Ez, LA, LB, n1 + 2, n2));
+                            const double dGb = -(e0(Ex, LA, LB, ll1, ll2 + 2) * ey * ez +
+                                                 ex * e0(Ey, LA, LB, m1, m2 + 2) * ez +
+                                                 ex * ey * e0(Ez, LA, LB, n1, n2 + 2));
+                            Bsum += g * eprod;
+                            Aa += g * dGa;
+                            Ab += g * dGb;
+                            ++index2;
+                        }
+                    ++index1;
+                }
+            dcoef[qa] += cb * pref0 * Bsum;
+            dcoef[qb] += ca * pref0 * Bsum;
+            dalpha[qa] += ca * cb * pref0 * Aa;
+            dalpha[qb] += ca * cb * pref0 * Ab;
+        }
+}
+
+#define ORC_ETAB_DOUBLES (3 * ORC_T * (ORC_LE + 1) * (ORC_LE + 1))
+
 void orc_ovl2c_pb(const double *dS, const double *posA, const double *posB,
                   const int64_t *sidxA, const int64_t *sidxB, int64_t B, int64_t ns, int64_t K,
                   int64_t NZ, const int64_t *ls, const double *coef, const double *alpha,
@@ -461,69 +526,59 @@ void orc_ovl2c_pb(const double *dS, const double *posA, const double *posB,
     const int64_t N = shell_layout(ns, ls, nbf, off);
     memset(dcoef, 0, sizeof(double) * (size_t)(ns * K * NZ));
     memset(dalpha, 0, sizeof(double) * (size_t)(ns * K * NZ));
-    double *Ex = (double *)malloc(sizeof(double) * 3 * ORC_T * (ORC_LE + 1) * (ORC_LE + 1));
-    double *Ey = Ex + ORC_T * (ORC_LE + 1) * (ORC_LE + 1), *Ez = Ey + ORC_T * (ORC_LE + 1) * (ORC_LE + 1);
-    for (int64_t b = 0; b < B; ++b) {
-        const int64_t sgA = sidxA ? sidxA[b] - 1 : 0, sgB = sidxB ? sidxB[b] - 1 : 0;
-        const double *A = posA + 3 * b, *Bv = posB + 3 * b;
-        const double *g_b = dS + N * N * b;
+    double *Ex = (double *)malloc(sizeof(double) * ORC_ETAB_DOUBLES);
+    for (int64_t b = 0; b < B; ++b)
         for (int64_t sb = 0; sb < ns; ++sb)
-            for (int64_t sa = 0; sa < ns; ++sa) {
-                const int la = (int)ls[sa], lb = (int)ls[sb];
-                const int64_t nba = nbf[sa];
-                const int N1 = 2 * la + 1;
-                for (int64_t ip = 0; ip < K; ++ip)
-                    for (int64_t jp = 0; jp < K; ++jp) {
-                        const int64_t qa = sa + ns * (ip + K * sgA), qb = sb + ns * (jp + K * sgB);
-                        const double ca = coef[qa], cb = coef[qb];
-                        const double aa = alpha[qa], ab = alpha[qb];
-                        const double p = aa + ab, mu = aa * ab / p, oo2p = 0.5 / p;
-                        const double pref0 = pow(ORC_PI / p, 1.5);
-                        double *Es[3] = {Ex, Ey, Ez};
-                        for (int ax = 0; ax < 3; ++ax) {
-                            const double P = (aa * A[ax] + ab * Bv[ax]) / p;
-                            const double AB = A[ax] - Bv[ax];
-                            md_etab(Es[ax], la + 2, lb + 2, P - A[ax], P - Bv[ax], oo2p,
-                                    exp(-mu * AB * AB));
-                        }
-                        double Bsum = 0.0, Aa = 0.0, Ab = 0.0;
-                        int index1 = 0;
-                        for (int ll1 = la; ll1 >= 0; --ll1)
-                            for (int n1 = 0; n1 <= la - ll1; ++n1) {
-                                const int m1 = la - ll1 - n1;
-                                int index2 = 0;
-                                for (int ll2 = lb; ll2 >= 0; --ll2)
-                                    for (int n2 = 0; n2 <= lb - ll2; ++n2) {
-                                        const int m2 = lb - ll2 - n2;
-                                        const int lin = index1 + N1 * index2;
-                                        const int64_t i = lin % nba, j = lin / nba;
-                                        const double g = g_b[(off[sa] + i) + N * (off[sb] + j)];
-                                        const int LA = la + 2, LB = lb + 2;
-                                        const double ex = e0(Ex, LA, LB, ll1, ll2),
-                                                     ey = e0(Ey, LA, LB, m1, m2),
-                                                     ez = e0(Ez, LA, LB, n1, n2);
-                                        const double eprod = ex * ey * ez;
-                                        const double dGa = -(e0(Ex, LA, LB, ll1 + 2, ll2) * ey * ez +
-                                                             ex * e0(Ey, LA, LB, m1 + 2, m2) * ez +
-                                                             ex * ey * e0(Ez, LA, LB, n1 + 2, n2));
-                                        const double dGb = -(e0(Ex, LA, LB, ll1, ll2 + 2) * ey * ez +
-                                                             ex * e0(Ey, LA, LB, m1, m2 + 2) * ez +
-                                                             ex * ey * e0(Ez, LA, LB, n1, n2 + 2));
-                                        Bsum += g * eprod;
-                                        Aa += g * dGa;
-                                        Ab += g * dGb;
-                                        ++index2;
-                                    }
-                                ++index1;
-                            }
-                        dcoef[qa] += cb * pref0 * Bsum;
-                        dcoef[qb] += ca * pref0 * Bsum;
-                        dalpha[qa] += ca * cb * pref0 * Aa;
-                        dalpha[qb] += ca * cb * pref0 * Ab;
-                    }
-            }
-    }
+            for (int64_t sa = 0; sa < ns; ++sa)
+                ovl2c_pb_item(dS, posA, posB, sidxA, sidxB, b, sa, sb, ns, K, ls, nbf, off, N, coef, alpha, Ex,
+                              dcoef, dalpha);
     free(Ex);
+    free(nbf);
+}
+
+/* The same work items split statically across OpenMP threads over the linearised
+ * (b, s_a, s_b) ndrange (KernelAbstractions' CPU backend splits the ndrange across
+ * Julia threads); the reference's atomics become per-thread partial arrays, summed in
+ * thread order at the end.  CPU-baseline timing only. */
+void orc_ovl2c_pb_omp(const double *dS, const double *posA, const double *posB,
+                      const int64_t *sidxA, const int64_t *sidxB, int64_t B, int64_t ns, int64_t K,
+                      int64_t NZ, const int64_t *ls, const double *coef, const double *alpha,
+                      double *dcoef, double *dalpha)
+{
+    int64_t *nbf = (int64_t *)malloc(sizeof(int64_t) * (size_t)(2 * ns + 1));
+    int64_t *off = nbf + ns;
+    const int64_t N = shell_layout(ns, ls, nbf, off);
+    const int64_t npar = ns * K * NZ;
+    const int nt = orc_num_threads();
+    double *part = (double *)calloc((size_t)(2 * npar * nt), sizeof(double));
+    const int64_t items = B * ns * ns;
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+#else
+        const int t = 0;
+#endif
+        double *Ex = (double *)malloc(sizeof(double) * ORC_ETAB_DOUBLES);
+        double *pc = part + 2 * npar * t, *pa = pc + npar;
+#pragma omp for schedule(static)
+        for (int64_t it = 0; it < items; ++it) {
+            const int64_t b = it % B, sa = (it / B) % ns, sb = it / (B * ns);
+            ovl2c_pb_item(dS, posA, posB, sidxA, sidxB, b, sa, sb, ns, K, ls, nbf, off, N, coef, alpha, Ex, pc,
+                          pa);
+        }
+        free(Ex);
+    }
+    for (int64_t q = 0; q < npar; ++q) {
+        double c = 0.0, a = 0.0;
+        for (int t = 0; t < nt; ++t) {
+            c += part[2 * npar * t + q];
+            a += part[2 * npar * t + npar + q];
+        }
+        dcoef[q] = c;
+        dalpha[q] = a;
+    }
+    free(part);
     free(nbf);
 }
 

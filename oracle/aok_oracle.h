@@ -74,6 +74,11 @@ void orc_ovl2c_pb(const double *dS, const double *posA, const double *posB,
                   const int64_t *sidxA, const int64_t *sidxB, int64_t B, int64_t ns,
                   int64_t K, int64_t NZ, const int64_t *ls, const double *coef,
                   const double *alpha, double *dcoef, double *dalpha);
+/* The same split over OpenMP threads (per-thread partials summed in thread order). */
+void orc_ovl2c_pb_omp(const double *dS, const double *posA, const double *posB,
+                      const int64_t *sidxA, const int64_t *sidxB, int64_t B, int64_t ns,
+                      int64_t K, int64_t NZ, const int64_t *ls, const double *coef,
+                      const double *alpha, double *dcoef, double *dalpha);
 
 /* Position cotangents of the 2-centre overlap (no reference counterpart): dA, dB
  * [3*B] in Bohr, dB = -dA; dB may be NULL. */

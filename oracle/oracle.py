@@ -53,8 +53,9 @@ def lib():
             f.argtypes = [_D, _D, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                           ctypes.c_int64, _I, _D, _D, _D]
             f.restype = ctypes.c_int64
-        L.orc_ovl2c_pb.argtypes = [_D, _D, _D, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
-                                   ctypes.c_int64, _I, _D, _D, _D, _D]
+        for name in ("orc_ovl2c_pb", "orc_ovl2c_pb_omp"):
+            getattr(L, name).argtypes = [_D, _D, _D, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int64, _I, _D, _D, _D, _D]
         L.orc_ovl2c_dx.argtypes = [_D, _D, _D, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                    ctypes.c_int64, _I, _D, _D, _D, _D]
         L.orc_ovl3c.argtypes = [_D, _D, _D, _I, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
@@ -198,7 +199,7 @@ def ovl2c(posA, posB, sidxA, sidxB, ls, coef, alpha, omp=False):
     return out.reshape((N, N, B), order="F")
 
 
-def ovl2c_pb(dS, posA, posB, sidxA, sidxB, ls, coef, alpha):
+def ovl2c_pb(dS, posA, posB, sidxA, sidxB, ls, coef, alpha, omp=False):
     posA = np.asarray(posA, np.float64)
     B = posA.shape[0]
     ns, K, NZ = coef.shape
@@ -212,7 +213,8 @@ def ovl2c_pb(dS, posA, posB, sidxA, sidxB, ls, coef, alpha):
     aa, ap = _d(_fortran_flat(alpha))
     dc = np.zeros(ns * K * NZ)
     da = np.zeros(ns * K * NZ)
-    lib().orc_ovl2c_pb(gp, Ap, Bp, sap, sbp, B, ns, K, NZ, lp, cp, ap, dc.ctypes.data_as(_D),
+    f = lib().orc_ovl2c_pb_omp if omp else lib().orc_ovl2c_pb
+    f(gp, Ap, Bp, sap, sbp, B, ns, K, NZ, lp, cp, ap, dc.ctypes.data_as(_D),
                        da.ctypes.data_as(_D))
     shp = (ns, K, NZ)
     return dc.reshape(shp, order="F"), da.reshape(shp, order="F")
@@ -290,9 +292,9 @@ def pullback_ps_3c(dT, posA, posB, posC, sidxA, sidxB, sidxC, ls, zeta, D):
     return da + dzn, dD
 
 
-def pullback_ps_2c(dS, posA, posB, sidxA, sidxB, ls, zeta, D):
+def pullback_ps_2c(dS, posA, posB, sidxA, sidxB, ls, zeta, D, omp=False):
     """Full (dzeta, dD) of the Lux 2c overlap (overlap_2c_grad.jl:296-304)."""
     coef = compile_coef(zeta, D, ls)
-    dcoef, dalpha = ovl2c_pb(dS, posA, posB, sidxA, sidxB, ls, coef, zeta)
+    dcoef, dalpha = ovl2c_pb(dS, posA, posB, sidxA, sidxB, ls, coef, zeta, omp=omp)
     dD, dz_norm = compile_coef_pb(dcoef, zeta, D, ls)
     return dalpha + dz_norm, dD

@@ -443,3 +443,18 @@ def test_ao_absent_species_zero_gradient():
     dP = np.ones((len(c["X"]), len(c["iR"])))
     _, dz, dD = O.ao_pullback(dP=dP, **c)
     assert np.all(dz[:, :, 2] == 0) and np.all(dD[:, :, 2] == 0)
+
+
+def test_ovl2c_pb_openmp_matches_serial():
+    """The OpenMP split of the 2c pullback (per-thread partials; the CPU-baseline form)
+    gives the serial oracle's cotangents."""
+    from aok import configs
+
+    d = configs.c1(1, B=96)
+    cb = d["basis"]
+    dS = np.random.default_rng(4).standard_normal((cb.N, cb.N, 96))
+    coef = O.compile_coef(cb.zeta, cb.D, cb.ls)
+    a = O.ovl2c_pb(dS, d["posA"], d["posB"], d["sidxA"], d["sidxB"], cb.ls, coef, cb.zeta)
+    b = O.ovl2c_pb(dS, d["posA"], d["posB"], d["sidxA"], d["sidxB"], cb.ls, coef, cb.zeta, omp=True)
+    for x, y in zip(a, b):
+        assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(x))
