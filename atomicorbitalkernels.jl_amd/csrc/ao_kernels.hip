@@ -11,11 +11,12 @@
 
 namespace aok {
 
-// (a, d) pair of the LDS parameter table: a = zeta*log2(e) for float (the exponential
-// runs on v_exp_f32 in base 2), a = zeta for double.
+// One primitive of the LDS parameter table: a = zeta*log2(e) for float (the exponential
+// runs on v_exp_f32 in base 2), a = zeta for double; d = D; ad = a * d (the radial's
+// derivative sum, so each primitive costs two FMAs after its exponential).
 template <typename T>
-struct ParamPair {
-    T a, d;
+struct ParamSet {
+    T a, d, ad, pad;
 };
 
 template <typename T>
@@ -41,7 +42,7 @@ __device__ __forceinline__ T a_to_zeta()
 // built once per call (FP64 products rounded once) and copied into each block's LDS.
 template <typename T>
 __global__ void ao_param_kernel(const T *__restrict__ zeta, const T *__restrict__ D, int nRad, int K, int NZ,
-                                ParamPair<T> *__restrict__ tab)
+                                ParamSet<T> *__restrict__ tab)
 {
     const int n = NZ * nRad * K;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,14 +53,17 @@ __global__ void ao_param_kernel(const T *__restrict__ zeta, const T *__restrict_
     const int m = rem - k * K;
     const int64_t src = k + static_cast<int64_t>(nRad) * (m + static_cast<int64_t>(K) * sg);
     const double z = static_cast<double>(zeta[src]);
-    ParamPair<T> pp;
-    pp.a = std::is_same<T, float>::value ? T(z * 1.4426950408889634) : T(z);
+    ParamSet<T> pp;
+    const double a = std::is_same<T, float>::value ? z * 1.4426950408889634 : z;
+    pp.a = T(a);
     pp.d = D[src];
+    pp.ad = T(a * static_cast<double>(D[src]));
+    pp.pad = T(0);
     tab[q] = pp;
 }
 
 template <typename T>
-__device__ __forceinline__ void load_param_table(ParamPair<T> *tab, const ParamPair<T> *__restrict__ gtab, int n)
+__device__ __forceinline__ void load_param_table(ParamSet<T> *tab, const ParamSet<T> *__restrict__ gtab, int n)
 {
     for (int q = threadIdx.x; q < n; q += blockDim.x) tab[q] = gtab[q];
 }
@@ -76,15 +80,15 @@ __device__ __forceinline__ T int_pow(T r, int p)
 // Gaussian with p = 0 uses the closed form G = -2 sum zeta D e (finite at r = 0);
 // otherwise the reference's dR/dr * (1/r) (NaN at r = 0 like _aorb_ed_ka!).
 template <typename T, bool GRAD>
-__device__ __forceinline__ void radial(const ParamPair<T> *row, int K, T f, T r, int p, int decay, T &R,
+__device__ __forceinline__ void radial(const ParamSet<T> *row, int K, T f, T r, int p, int decay, T &R,
                                        T &G)
 {
     T s = T(0), ds = T(0);
     for (int m = 0; m < K; ++m) {
-        const ParamPair<T> pp = row[m];
-        const T e = exp_param<T>(pp.a, f) * pp.d;
-        s += e;
-        if (GRAD) ds += pp.a * e;
+        const ParamSet<T> pp = row[m];
+        const T ex = exp_param<T>(pp.a, f);
+        s = fma(ex, pp.d, s);
+        if (GRAD) ds = fma(ex, pp.ad, ds);
     }
     if (p == 0) {
         R = s;
@@ -136,13 +140,13 @@ __device__ __forceinline__ T pick(const T *v, int y)
 
 template <typename T, int L, bool GRAD, bool GENERIC>
 __global__ __launch_bounds__(256) void ao_eval_kernel(const T *__restrict__ X, const void *__restrict__ sidx,
-                                                      int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
+                                                      int itype, int64_t nX, const ParamSet<T> *__restrict__ gtab,
                                                        int nRad, int K, int NZ, T scale,
                                                       int decay, AoRunTable runs, T *__restrict__ P,
                                                       T *__restrict__ dP, unsigned *__restrict__ err)
 {
     extern __shared__ unsigned char smem_raw[];
-    ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw);
+    ParamSet<T> *tab = reinterpret_cast<ParamSet<T> *>(smem_raw);
     load_param_table(tab, gtab, NZ * nRad * K);
     __syncthreads();
     constexpr int NY = (L + 1) * (L + 1);
@@ -229,13 +233,13 @@ __device__ __forceinline__ void store_shell_vec(const T (*Z)[(L + 1) * (L + 1)],
 
 template <typename T, int L, int PPL>
 __global__ __launch_bounds__(256) void ao_eval_vec_kernel(const T *__restrict__ X, const void *__restrict__ sidx,
-                                                          int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
+                                                          int itype, int64_t nX, const ParamSet<T> *__restrict__ gtab,
                                                            int nRad, int K, int NZ, T scale,
                                                           int decay, AoRunTable runs, T *__restrict__ P,
                                                           unsigned *__restrict__ err)
 {
     extern __shared__ unsigned char smem_raw[];
-    ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw);
+    ParamSet<T> *tab = reinterpret_cast<ParamSet<T> *>(smem_raw);
     load_param_table(tab, gtab, NZ * nRad * K);
     __syncthreads();
     constexpr int NY = (L + 1) * (L + 1);
@@ -283,20 +287,47 @@ __global__ __launch_bounds__(256) void ao_eval_vec_kernel(const T *__restrict__ 
 
 // Pullback contraction of one full shell run: dr += sum_j w_j Z_j and W_y += w_j R for
 // its harmonics y (the spatial gradient is contracted once per point from W).
+// The point's cotangent of column o is src[cs * o] (src = dPin + i, cs = nX).
 template <int LL, int L, typename T>
-__device__ __forceinline__ void contract_shell(const T *Z, T *W, T R, int64_t i, int64_t nX, int o0,
-                                               const T *__restrict__ dPin, T &dr)
+__device__ __forceinline__ void contract_shell(const T *Z, T *W, T R, const T *src, int64_t cs, int o0, T &dr)
 {
     if constexpr (LL <= L) {
 #pragma unroll
         for (int j = 0; j <= 2 * LL; ++j) {
             const int yy = LL * LL + j;
-            const T w = dPin[i + nX * (o0 + j)];
+            const T w = src[cs * (o0 + j)];
             dr += w * Z[yy];
             W[yy] += w * R;
         }
     }
 }
+
+// Prefetched shell runs of the pullback: the cotangents of a group of runs are loaded
+// into registers (load_run) before their radials are formed, and contracted after
+// (use_run), so the loads overlap the exponentials.  Measured on nX = 2^24, 10 radials
+// x 4 primitives, L = 3 (FP32): one run per group at 6 waves/SIMD 0.89 ms; at 7 waves
+// the compiler serialises the loads (0.92 ms); two or three runs per group at 4-6 waves
+// 0.94-1.67 ms.
+template <int LL, int MJ, typename T>
+__device__ __forceinline__ void load_run(T (&v)[MJ], const T *src, int64_t cs, int o0)
+{
+    if constexpr (2 * LL + 1 <= MJ) {
+#pragma unroll
+        for (int j = 0; j <= 2 * LL; ++j) v[j] = src[cs * (o0 + j)];
+    }
+}
+template <int LL, int L, int MJ, typename T>
+__device__ __forceinline__ void use_run(const T *Z, T *W, T R, const T (&v)[MJ], T &dr)
+{
+    if constexpr (LL <= L && 2 * LL + 1 <= MJ) {
+#pragma unroll
+        for (int j = 0; j <= 2 * LL; ++j) {
+            dr += v[j] * Z[LL * LL + j];
+            W[LL * LL + j] += v[j] * R;
+        }
+    }
+}
+constexpr int AO_PB_RUN_GROUP = 1;
 
 // ---------------------------------------------------------------------------------
 // Pullback, one 256-point tile per block iteration.
@@ -315,7 +346,7 @@ constexpr int AO_PB_THREADS = 256;
 
 struct AoPbLayout {
     int nslot, nsub, nitem, nRadP;
-    size_t tab, dr, f, cnt, acc, total;
+    size_t tab, dr, f, cnt, rng, dec, acc, total;
 };
 
 // LDS plan of one pullback block.
@@ -333,29 +364,32 @@ __host__ __device__ inline AoPbLayout ao_pb_layout(int nRad, int K, int NZ)
         o = (o + bytes + 15) & ~size_t(15);
         return at;
     };
-    y.tab = take(sizeof(ParamPair<T>) * y.nslot);
+    y.tab = take(sizeof(ParamSet<T>) * y.nslot);
     y.dr = take(sizeof(T) * AO_PB_THREADS * y.nRadP);
     y.f = take(sizeof(T) * AO_PB_THREADS);
     y.cnt = take(sizeof(int) * (4 * NZ + NZ + 1));
+    y.rng = take(sizeof(int) * NZ * (y.nsub + 1));
+    y.dec = take(sizeof(int2) * y.nitem);
     y.acc = take(sizeof(double) * 2 * y.nitem);
     y.total = o;
     return y;
 }
 
-// Waves per SIMD the FP32 pullback is compiled for: at most 72 VGPRs, no spills for
-// L <= 3 (measured on nX = 2^24, 10 radials x 4 primitives, L = 3: 4 waves/SIMD at the
-// compiler's own 99 VGPRs 1.06-1.11 ms, 7 waves 0.89 ms with the grid matched to the
-// resident blocks).  FP64 and arbitrary orbital maps keep the compiler's choice.
+// Waves per SIMD the FP32 pullback is compiled for: at most 84 VGPRs for L <= 3
+// (measured on nX = 2^24, 10 radials x 4 primitives, L = 3: 4 waves/SIMD at the
+// compiler's own 99 VGPRs 1.06-1.11 ms, 7 waves 0.89-0.92 ms, 6 waves with the run's
+// loads issued before its radial 0.89 ms, the grid matched to the resident blocks).
+// FP64 and arbitrary orbital maps keep the compiler's choice.
 template <typename T, int L, bool GENERIC>
 constexpr int ao_pb_waves()
 {
-    return std::is_same<T, float>::value && L <= 3 && !GENERIC ? 7 : 1;
+    return std::is_same<T, float>::value && L <= 3 && !GENERIC ? 6 : 1;
 }
 
 template <typename T, int L, bool GENERIC>
 __global__ __launch_bounds__(AO_PB_THREADS) __attribute__((amdgpu_waves_per_eu(ao_pb_waves<T, L, GENERIC>(), 8)))
 void ao_pullback_kernel(
-    const T *__restrict__ X, const void *__restrict__ sidx, int itype, int64_t nX, const ParamPair<T> *__restrict__ gtab,
+    const T *__restrict__ X, const void *__restrict__ sidx, int itype, int64_t nX, const ParamSet<T> *__restrict__ gtab,
     int nRad, int K, int NZ, T scale, int decay, AoRunTable runs, const T *__restrict__ dPin, T *__restrict__ dX,
     double *__restrict__ partial, int want_params, unsigned *__restrict__ err)
 {
@@ -363,15 +397,24 @@ void ao_pullback_kernel(
     constexpr int TILE = AO_PB_THREADS;
     extern __shared__ unsigned char smem_raw[];
     const AoPbLayout y = ao_pb_layout<T>(nRad, K, NZ);
-    ParamPair<T> *tab = reinterpret_cast<ParamPair<T> *>(smem_raw + y.tab);
+    ParamSet<T> *tab = reinterpret_cast<ParamSet<T> *>(smem_raw + y.tab);
     T *dru = reinterpret_cast<T *>(smem_raw + y.dr);  // [row][nRadP], rows in species order
     T *fu = reinterpret_cast<T *>(smem_raw + y.f);
     int *whist = reinterpret_cast<int *>(smem_raw + y.cnt);  // [4 waves][NZ]
     int *soff = whist + 4 * NZ;                              // [NZ + 1]
+    int *rng = reinterpret_cast<int *>(smem_raw + y.rng);    // [NZ][nsub + 1] row bounds of the subsets
+    int2 *idec = reinterpret_cast<int2 *>(smem_raw + y.dec);  // item -> (u | p << 8 | g << 16, k)
     double *acc = reinterpret_cast<double *>(smem_raw + y.acc);
 
     load_param_table(tab, gtab, y.nslot);
-    for (int q = threadIdx.x; q < 2 * y.nitem; q += AO_PB_THREADS) acc[q] = 0.0;
+    // parameter items decoded once per block: item = (sub u, slot), slot = (g * nRad + k) * K + m
+    for (int q = threadIdx.x; q < y.nitem; q += AO_PB_THREADS) {
+        acc[2 * q] = 0.0;
+        acc[2 * q + 1] = 0.0;
+        const int u = q / y.nslot, slot = q - u * y.nslot;
+        const int g = slot / (nRad * K), k = (slot - g * nRad * K) / K;
+        idec[q] = make_int2(u | (run_poly(runs, k) << 8) | (g << 16), k);
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
@@ -417,9 +460,18 @@ void ao_pullback_kernel(
                 pos = soff[sg] + rk;
                 for (int w = 0; w < wave; ++w) pos += whist[w * NZ + sg];
             }
+            // bounds of the nsub row subsets of every species
+            const int nb = y.nsub + 1;
+            for (int q = threadIdx.x; q < NZ * nb; q += AO_PB_THREADS) {
+                const int g = q / nb, u = q - g * nb;
+                const int n0 = soff[g], cnt = soff[g + 1] - n0;
+                rng[q] = n0 + (cnt * u) / y.nsub;
+            }
         }
         // ---- phase A ----
         if (i < nX) {
+            const T *src = dPin + i;
+            const int64_t cs = nX;
             T *myrow = dru + pos * y.nRadP;
             for (int k = 0; k < nRad; ++k) myrow[k] = T(0);
             const T x = scale * X[3 * i], y3 = scale * X[3 * i + 1], z = scale * X[3 * i + 2];
@@ -432,22 +484,59 @@ void ao_pullback_kernel(
 #pragma unroll
             for (int q = 0; q < NY; ++q) W[q] = T(0);
             T sgd = T(0);
-            for (int q = 0; q < runs.nrun; ++q) {
+            if constexpr (!GENERIC) {
+                constexpr int MJ = 2 * L + 1, RG = AO_PB_RUN_GROUP;
+                for (int q = 0; q < runs.nrun; q += RG) {
+                    T v[RG][MJ];
+#pragma unroll
+                    for (int g = 0; g < RG; ++g) {
+                        if (q + g >= runs.nrun) break;
+                        const uint32_t rw = runs.run[q + g];
+                        const int o0 = run_o0(rw);
+                        switch (run_l(rw)) {
+                            case 0: load_run<0>(v[g], src, cs, o0); break;
+                            case 1: load_run<1>(v[g], src, cs, o0); break;
+                            case 2: load_run<2>(v[g], src, cs, o0); break;
+                            case 3: load_run<3>(v[g], src, cs, o0); break;
+                            case 4: load_run<4>(v[g], src, cs, o0); break;
+                        }
+                    }
+#pragma unroll
+                    for (int g = 0; g < RG; ++g) {
+                        if (q + g >= runs.nrun) break;
+                        const uint32_t rw = runs.run[q + g];
+                        const int k = run_k(rw);
+                        T R, G;
+                        radial<T, true>(tab + (sg * nRad + k) * K, K, f, r, run_poly(runs, k), decay, R, G);
+                        T dr = T(0);
+                        switch (run_l(rw)) {
+                            case 0: use_run<0, L>(Z, W, R, v[g], dr); break;
+                            case 1: use_run<1, L>(Z, W, R, v[g], dr); break;
+                            case 2: use_run<2, L>(Z, W, R, v[g], dr); break;
+                            case 3: use_run<3, L>(Z, W, R, v[g], dr); break;
+                            case 4: use_run<4, L>(Z, W, R, v[g], dr); break;
+                        }
+                        sgd += G * dr;
+                        myrow[k] += dr;
+                    }
+                }
+            }
+            for (int q = 0; GENERIC && q < runs.nrun; ++q) {
                 const uint32_t rw = runs.run[q];
-            const int k = run_k(rw), lk = run_l(rw), o0 = run_o0(rw);
+                const int k = run_k(rw), lk = run_l(rw), o0 = run_o0(rw);
                 T R, G;
                 radial<T, true>(tab + (sg * nRad + k) * K, K, f, r, run_poly(runs, k), decay, R, G);
                 T dr = T(0);
                 switch (lk) {
-                    case 0: contract_shell<0, L>(Z, W, R, i, nX, o0, dPin, dr); break;
-                    case 1: contract_shell<1, L>(Z, W, R, i, nX, o0, dPin, dr); break;
-                    case 2: contract_shell<2, L>(Z, W, R, i, nX, o0, dPin, dr); break;
-                    case 3: contract_shell<3, L>(Z, W, R, i, nX, o0, dPin, dr); break;
-                    case 4: contract_shell<4, L>(Z, W, R, i, nX, o0, dPin, dr); break;
+                    case 0: contract_shell<0, L>(Z, W, R, src, cs, o0, dr); break;
+                    case 1: contract_shell<1, L>(Z, W, R, src, cs, o0, dr); break;
+                    case 2: contract_shell<2, L>(Z, W, R, src, cs, o0, dr); break;
+                    case 3: contract_shell<3, L>(Z, W, R, src, cs, o0, dr); break;
+                    case 4: contract_shell<4, L>(Z, W, R, src, cs, o0, dr); break;
                     default:
                         if constexpr (GENERIC) {
                             const int yy = -1 - lk;
-                            const T w = dPin[i + nX * o0];
+                            const T w = src[cs * o0];
                             dr += w * pick<NY>(Z, yy);
 #pragma unroll
                             for (int q2 = 0; q2 < NY; ++q2) W[q2] += (q2 == yy) ? w * R : T(0);
@@ -469,12 +558,10 @@ void ao_pullback_kernel(
         __syncthreads();
         // ---- phase B ----
         for (int it = threadIdx.x; it < y.nitem; it += AO_PB_THREADS) {
-            const int u = it / y.nslot, slot = it - u * y.nslot;  // slot = (g*nRad + k)*K + m
-            const int g = slot / (nRad * K), rem = slot - g * nRad * K, k = rem / K;
-            const ParamPair<T> pp = tab[slot];
-            const int pk = run_poly(runs, k);
-            const int n0 = soff[g], cnt = soff[g + 1] - n0;
-            const int j0 = n0 + (cnt * u) / y.nsub, j1 = n0 + (cnt * (u + 1)) / y.nsub;
+            const int2 dc = idec[it];
+            const int u = dc.x & 0xff, pk = (dc.x >> 8) & 0xff, g = dc.x >> 16, k = dc.y;
+            const ParamSet<T> pp = tab[it - u * y.nslot];
+            const int j0 = rng[g * (y.nsub + 1) + u], j1 = rng[g * (y.nsub + 1) + u + 1];
             const T *col = dru + k;
             T sD = T(0), sZ = T(0);
             auto term = [&](int j) {
@@ -483,7 +570,7 @@ void ao_pullback_kernel(
                 if (pk != 0) dr *= int_pow((decay == AOK_DECAY_GAUSSIAN) ? sqrt(f) : f, pk);
                 const T w = dr * exp_param<T>(pp.a, f);
                 sD += w;
-                sZ -= w * pp.d * f;
+                sZ = fma(w, f, sZ);
             };
             int j = j0;
             for (; j + 4 <= j1; j += 4) {
@@ -494,7 +581,7 @@ void ao_pullback_kernel(
             }
             for (; j < j1; ++j) term(j);
             acc[2 * it] += static_cast<double>(sD);
-            acc[2 * it + 1] += static_cast<double>(sZ);
+            acc[2 * it + 1] -= static_cast<double>(pp.d) * static_cast<double>(sZ);  // dzeta = -D sum w f
         }
         __syncthreads();  // LDS rows are rewritten by the next tile
     }
@@ -546,12 +633,12 @@ __global__ __launch_bounds__(256) void ao_pullback_finalize(const double *__rest
 // ---------------------------------------------------------------------------------
 // The per-call parameter table, at byte offset `at` of the context workspace.
 template <typename T>
-static aok_status param_table(aok_ctx ctx, const AoArgs &a, size_t at, const ParamPair<T> *&gtab)
+static aok_status param_table(aok_ctx ctx, const AoArgs &a, size_t at, const ParamSet<T> *&gtab)
 {
     const int n = a.NZ * a.nRad * a.K;
-    aok_status st = ensure_ws(ctx, at + sizeof(ParamPair<T>) * n);
+    aok_status st = ensure_ws(ctx, at + sizeof(ParamSet<T>) * n);
     if (st != AOK_OK) return st;
-    ParamPair<T> *t = reinterpret_cast<ParamPair<T> *>(static_cast<char *>(ctx->ws) + at);
+    ParamSet<T> *t = reinterpret_cast<ParamSet<T> *>(static_cast<char *>(ctx->ws) + at);
     hipLaunchKernelGGL((ao_param_kernel<T>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
                        static_cast<const T *>(a.zeta), static_cast<const T *>(a.D), a.nRad, a.K, a.NZ, t);
     AOK_HIP(ctx, hipGetLastError());
@@ -562,9 +649,9 @@ static aok_status param_table(aok_ctx ctx, const AoArgs &a, size_t at, const Par
 template <typename T, int L, bool GENERIC>
 static aok_status launch_eval_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &runs)
 {
-    const size_t shm = sizeof(ParamPair<T>) * a.NZ * a.nRad * a.K;
+    const size_t shm = sizeof(ParamSet<T>) * a.NZ * a.nRad * a.K;
     if (a.nX <= 0) return AOK_OK;
-    const ParamPair<T> *gtab = nullptr;
+    const ParamSet<T> *gtab = nullptr;
     aok_status st = param_table<T>(ctx, a, 0, gtab);
     if (st != AOK_OK) return st;
     // launch shape: points per lane (values-only, canonical maps) and the grid cap.  The
@@ -664,7 +751,7 @@ static aok_status launch_pb_L(aok_ctx ctx, const AoArgs &a, const AoRunTable &ru
     if (shm > 160 * 1024)
         return fail(ctx, AOK_ERR_UNSUPPORTED, "ao_pullback: basis too large for the LDS tile");
     const size_t part_bytes = align256(sizeof(double) * 2 * static_cast<size_t>(y.nitem) * grid);
-    const ParamPair<T> *gtab = nullptr;
+    const ParamSet<T> *gtab = nullptr;
     aok_status st = param_table<T>(ctx, a, part_bytes, gtab);
     if (st != AOK_OK) return st;
     double *partial = static_cast<double *>(ctx->ws);
