@@ -24,7 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "atomicorbitalkernels.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, packed FMA
+FP32_UNPACKED_TFLOPS = 78.6       # the same without packed FP32 (the kernels are built without it)
 # HBM bytes per launch of the C1 kernels: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
 # runs of this bench's C1 step (--no-extra; KiB per dispatch, mean over the launches), each
 # scaled by what the same counter reads for the bench's 2 GiB device copy (FETCH_SIZE reads
@@ -32,35 +33,60 @@ FP32_PEAK_TFLOPS = 157.3
 PMC_KIB = {"ovl2c_bwd_fast": (126480.0, 10333.0), "ovl2c_fwd_fast": (8939.0, 114411.0)}
 PMC_SCALE = (2.0, 1.0)
 # FLOPs per pair the C1 kernels execute: rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32 and
-# SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes), per launch / 65536 pairs.
-# The kernels' Obara-Saika 1-D recurrences need far fewer operations than the Hermite
-# (McMurchie-Davidson) count of the reference algorithm (algorithmic_flops below), so the
-# roofline prices these executed FLOPs; the Hermite-count rate is reported beside it.
-# VALU busy: SQ_ACTIVE_INST_VALU * 4 / (1024 SIMDs * GRBM_GUI_ACTIVE / 8 XCDs).
+# SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes), per launch / 65536 pairs;
+# VALU busy: SQ_ACTIVE_INST_VALU * 4 / (1024 SIMDs * GRBM_GUI_ACTIVE / 8 XCDs).  Reported
+# beside the algorithmic rate, never priced.
 EXEC_FLOP_PER_PAIR = {"ovl2c_bwd_fast": (3.177e9 + 3.224e8) / 65536, "ovl2c_fwd_fast": (4.032e8 + 1.287e8) / 65536}
 VALU_BUSY = {"ovl2c_bwd_fast": 0.70, "ovl2c_fwd_fast": 0.42}
 METRIC = "2-ctr overlap pairs/s + (ζ,D)-grad pairs/s, FP32@1e-6 vs plain FP64; AO GB/s"
 
 
-def e2(la, lb):
-    return sum(i + j + 1 for i in range(la + 1) for j in range(lb + 1)) - 1
-
-
-def nbf(l):
+# ---- algorithmic FLOP model of the Obara-Saika 2-centre overlap -----------------------------
+# Counted per primitive pair (a, b) of a shell pair (la, lb), FMA = 2 FLOPs, exp = 1:
+#  * geometry and prefactor: mu R^2, exp, prefactor product (3); PA, PB per axis when
+#    la + lb > 0 (6);
+#  * 1-D tables s_ij (i <= la, j <= lb) per axis: vertical recursion
+#    s_{i+1,0} = PA s_i0 + i/(2p) s_{i-1,0} (3 FLOPs per entry beyond s_10 = PA) to i = la + lb,
+#    horizontal transfer s_{i,j+1} = s_{i+1,j} + AB s_ij (one FMA per entry);
+#  * forward: the prefactor folded into the x table ((la+1)(lb+1) products), then per output
+#    element s_x s_y (1) and an FMA into the accumulator (2): 3 FLOPs;
+#  * pullback (real x real): tables to (la + 2, lb) and (la, lb + 2) from one vertical
+#    recursion to la + lb + 2 and the horizontal transfer to lb + 2; per element the value
+#    sum g s_x s_y s_z (4) and the bra- and ket-raised sums (8 each); 8 FLOPs of epilogue
+#    (the four cotangent updates); s-s pairs use the closed form of the raised sums
+#    (|PA|^2 + 3/(2p), 4 FLOPs each) instead of tables;
+#  * pullback, padded representative x real: the value sum only (4 per element) on the
+#    forward's tables, 2 FLOPs of epilogue.
+def _nbf(l):
     return (l + 1) * (l + 2) // 2
 
 
-def f2(la, lb):
-    return 30 + 3 * (5 * e2(la, lb) + 3) + 4 * nbf(la) * nbf(lb)
+def _os_tables(la, lb):
+    return 3 * max(la + lb - 1, 0) + 2 * sum(la + lb - j + 1 for j in range(1, lb + 1))
 
 
-def g2(la, lb):
-    return 40 + 3 * (5 * e2(la + 2, lb + 2) + 3) + 25 * nbf(la) * nbf(lb)
+def os_flops_fwd(la, lb):
+    geo = 3 + (6 if la + lb > 0 else 0)
+    pref = (la + 1) * (lb + 1) if la + lb > 0 else 0
+    return geo + 3 * _os_tables(la, lb) + pref + 3 * _nbf(la) * _nbf(lb)
 
 
-def algorithmic_flops(basis, sidxA, sidxB):
-    """Counting model: forward F2 per non-skipped primitive pair; pullback G2 per
-    real x real pair plus F2 per (distinct padded exponent x real) pair."""
+def os_flops_bwd(la, lb):
+    if la == 0 and lb == 0:
+        return 3 + 2 * 4 + 4 + 8 + 8
+    vrr = 3 * (la + lb + 1)
+    hrr = 2 * sum(la + lb + 3 - j for j in range(1, lb + 3))
+    return 9 + 3 * (vrr + hrr) + 20 * _nbf(la) * _nbf(lb) + 8
+
+
+def os_flops_pad(la, lb):
+    return os_flops_fwd(la, lb) + _nbf(la) * _nbf(lb) + 2
+
+
+def os_model_flops(basis, sidxA, sidxB):
+    """Algorithmic FLOPs of one forward and one (zeta, D) pullback launch over the batch:
+    forward per real x real primitive pair; pullback per real x real pair plus per (distinct
+    padded exponent x real) pair, the padded slots' coefficient cotangents."""
     import numpy as np
 
     NZ = basis.NZ
@@ -78,11 +104,23 @@ def algorithmic_flops(basis, sidxA, sidxB):
                 for sb in range(basis.ns):
                     la, lb = basis.ls[sa], basis.ls[sb]
                     rr = nreal[sa, a] * nreal[sb, b]
-                    fwd[a, b] += rr * f2(la, lb)
-                    bwd[a, b] += rr * g2(la, lb) + (npad[sa, a] * nreal[sb, b] + nreal[sa, a] * npad[sb, b]) * f2(la, lb)
+                    fwd[a, b] += rr * os_flops_fwd(la, lb)
+                    bwd[a, b] += rr * os_flops_bwd(la, lb) + \
+                        (npad[sa, a] * nreal[sb, b] + nreal[sa, a] * npad[sb, b]) * os_flops_pad(la, lb)
     cnt = np.zeros((NZ, NZ))
     np.add.at(cnt, (np.asarray(sidxA) - 1, np.asarray(sidxB) - 1), 1)
     return float((fwd * cnt).sum()), float((bwd * cnt).sum())
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -167,11 +205,11 @@ def main():
     value = B / (ms_step * 1e-3)
     fwd_ms = float(np.mean(kern["ovl2c_fwd_fast"]))
     bwd_ms = float(np.mean(kern["ovl2c_bwd_fast"]))
-    flops_fwd, flops_bwd = algorithmic_flops(cb, d["sidxA"], d["sidxB"])
+    flops_fwd, flops_bwd = os_model_flops(cb, d["sidxA"], d["sidxB"])
     dom = "ovl2c_bwd_fast" if bwd_ms >= fwd_ms else "ovl2c_fwd_fast"
     dom_ms = max(fwd_ms, bwd_ms)
-    dom_flops = EXEC_FLOP_PER_PAIR[dom] * B
-    ref_flops = flops_bwd if dom == "ovl2c_bwd_fast" else flops_fwd
+    dom_flops = flops_bwd if dom == "ovl2c_bwd_fast" else flops_fwd
+    exec_flops = EXEC_FLOP_PER_PAIR[dom] * B
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
     fk, wk = PMC_KIB[dom]
     traffic = (fk * PMC_SCALE[0] + wk * PMC_SCALE[1]) * 1024.0
@@ -182,15 +220,19 @@ def main():
         "config": {"workload": "2c overlap forward + (zeta,D) pullback, B=65536 pairs, 3 species 9s4p1d->3s2p1d "
                                "(N=15, K=9), zeta log-uniform [0.1,7000], FP32", "global_batch": B,
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic, "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_flop_per_launch": dom_flops,
-                     "flop_count": "FP32+FP64 VALU FLOPs the kernel executes (rocprofv3 counters)",
+                     "algorithmic_flop_per_pair": dom_flops / B,
+                     "flop_model": "Obara-Saika count per primitive pair (bench.py os_flops_*): geometry + "
+                                   "1-D tables + per-element contraction, FMA = 2",
+                     "frac_of_unpacked_fp32": achieved / FP32_UNPACKED_TFLOPS,
+                     "executed_flop_per_launch": exec_flops,
+                     "executed_TFLOPs": exec_flops / (dom_ms * 1e-3) / 1e12,
                      "valu_busy": VALU_BUSY[dom],
-                     "hermite_model_flop_per_launch": ref_flops,
-                     "hermite_model_TFLOPs": ref_flops / (dom_ms * 1e-3) / 1e12,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "note": "VALU FP32 kernel priced against the published FP32 peak"},
+                     "note": "FP32 VALU kernel (no MFMA: the primitive contraction has no operand shared "
+                             "across pairs); peak = published packed FP32 rate"},
         "fwd_kernel_ms": fwd_ms, "bwd_kernel_ms": bwd_ms,
         "fwd_kernel_pairs_per_s": B / (fwd_ms * 1e-3), "grad_kernel_pairs_per_s": B / (bwd_ms * 1e-3),
     }
@@ -236,21 +278,29 @@ def main():
         log("C1 FP64 reference run for the FP32 error")
         orb64 = orbital(torch.float64)
         XA64, XB64 = inputs(torch.float64)
-        z32 = torch.as_tensor(cb.zeta.astype(np.float32), dtype=torch.float64)
-        # feed FP64 the FP32-rounded inputs
+        # the FP64 kernels (pinned to the oracle by the tests) fed the FP32-rounded inputs,
+        # so the difference is the FP32 arithmetic alone
         ps64 = {"Rnl": {"zeta": aok.as_fortran(orb32.Rnl.zeta.double()), "D": aok.as_fortran(orb32.Rnl.D.double())},
                 "Ylm": {}}
         XA64r = (XA32[0].double().contiguous(), XA32[1])
         XB64r = (XB32[0].double().contiguous(), XB32[1])
-        S32, _ = step()
-        S64 = aok.batch_overlap(orb64, XA64r, XB64r, ps64, ctx=ctx)
+        S32, dps32 = step()
+        S64, pb64 = aok.rrule_batch_overlap(orb64, XA64r, XB64r, ps64, ctx=ctx)
+        dps64 = pb64(aok.as_fortran(dS32.double()))[4]
+        fp32 = {"definition": "max|x32 - x64| / max|x64| per output (overlap blocks: max over pairs of the "
+                              "block's value), FP64 kernels on the same FP32-rounded inputs"}
         num = (S32.double() - S64).abs().amax(dim=(0, 1))
         den = S64.abs().amax(dim=(0, 1)).clamp_min(1e-300)
-        out["fp32_blockwise_max_rel_err"] = float((num / den).max())
+        fp32["c1_S_blockwise"] = float((num / den).max())
         big = S64.abs() >= 1e-3 * S64.abs().amax(dim=(0, 1), keepdim=True)
         rel = ((S32.double() - S64).abs() / S64.abs().clamp_min(1e-300))[big]
-        out["fp32_elementwise_max_rel_err_tau1e-3"] = float(rel.max())
-        del z32
+        fp32["c1_S_elementwise_tau1e-3"] = float(rel.max())
+        for k in ("zeta", "D"):
+            g32, g64 = dps32["Rnl"][k].double(), dps64["Rnl"][k]
+            fp32[f"c1_d{k}"] = float((g32 - g64).abs().max() / g64.abs().max())
+        out["fp32_blockwise_max_rel_err"] = fp32["c1_S_blockwise"]
+        out["fp32_error"] = fp32
+        del S64, pb64
         # plain FP64 GPU form (reference decomposition) -- the >= 4x denominator
         log("C1 plain FP64 forward/pullback")
         dS64 = dS32.double()
@@ -351,7 +401,27 @@ def main():
                          "GB_per_s": nbytes / (kms * 1e-3) / 1e9,
                          "frac_of_hbm_peak": nbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "algorithmic_bytes": nbytes}
-        del P, X, dPin
+        # FP32 errors of every AO output against the FP64 kernels on the same rounded inputs
+        log("AO FP64 reference run for the FP32 error")
+        R4d = aok.GaussianTypeRadials(aok.as_fortran(orb4.Rnl.zeta.double()), aok.as_fortran(orb4.Rnl.D.double()),
+                                      spec4, None, (1, 2, 3))
+        orb4d = aok.AtomicOrbitals(R4d, 3, length_unit="bohr")
+        Xd = X.double()
+
+        def rel_err(a, b):
+            return float((a.double() - b).abs().max() / b.abs().max())
+
+        P32, dP32 = aok.evaluate_ed(orb4, X, sidx=sidx, ctx=ctx)
+        P64, dP64 = aok.evaluate_ed(orb4d, Xd, sidx=sidx, ctx=ctx)
+        fp32["ao_c4b_P"] = rel_err(P32, P64)
+        fp32["ao_c4b_dP"] = rel_err(dP32, dP64)
+        del P32, dP32, P64, dP64
+        _, _, dX32, dq32, _ = aok.rrule_evaluate(orb4, X, sidx=sidx, ctx=ctx)[1](dPin)
+        _, _, dX64, dq64, _ = aok.rrule_evaluate(orb4d, Xd, sidx=sidx, ctx=ctx)[1](aok.as_fortran(dPin.double()))
+        fp32["ao_c4b_dX"] = rel_err(dX32, dX64)
+        fp32["ao_c4b_dzeta"] = rel_err(dq32["Rnl"]["zeta"], dq64["Rnl"]["zeta"])
+        fp32["ao_c4b_dD"] = rel_err(dq32["Rnl"]["D"], dq64["Rnl"]["D"])
+        del P, X, Xd, dPin, dX32, dX64
         # C2: uneven batch, 6 species (union spec N = 50, K = 10, l <= 3), Zipf species pairs
         log("C2 B=2^18 rrule step")
         d2 = configs.c2(2, B=2 ** 18)
@@ -392,7 +462,7 @@ def main():
         wall2 = (time.perf_counter() - ts) * 1e3 / 5
         log("C2 graph-replayed step")
         gwall2 = graph_step_ms(o2, X2A, X2B, p2, dS2, 5, tknobs)
-        f2f, f2b = algorithmic_flops(cb2, d2["sidxA"], d2["sidxB"])
+        f2f, f2b = os_model_flops(cb2, d2["sidxA"], d2["sidxB"])
         o2d = orbital2(torch.float64)
         S2d = aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
                                 {"Rnl": {"zeta": aok.as_fortran(o2.Rnl.zeta.double()),
@@ -400,6 +470,7 @@ def main():
         num = (S2.double() - S2d).abs().amax(dim=(0, 1))
         den = S2d.abs().amax(dim=(0, 1)).clamp_min(1e-300)
         err2 = float((num / den).max())
+        fp32["c2_S_blockwise"] = err2
         del S2d
         aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
                           o2d.static_params(), mode="plain", ctx=ctx)
@@ -411,7 +482,7 @@ def main():
                          "candidates": [[k["ovl_cap"], k["ovl_split"], k["ovl_order"], round(m, 4)] for k, m in tres],
                          "candidate_fields": ["ovl_cap", "ovl_split", "ovl_order", "step_ms"]},
             "fwd_kernel_ms": float(np.mean(fk)), "bwd_kernel_ms": float(np.mean(bk)),
-            "fwd_hermite_model_TFLOPs": f2f / (np.mean(fk) * 1e-3) / 1e12, "bwd_hermite_model_TFLOPs": f2b / (np.mean(bk) * 1e-3) / 1e12,
+            "fwd_model_TFLOPs": f2f / (np.mean(fk) * 1e-3) / 1e12, "bwd_model_TFLOPs": f2b / (np.mean(bk) * 1e-3) / 1e12,
             "fp32_blockwise_max_rel_err": err2, "plain_f64_fwd_kernel_ms": plain2,
             "fwd_speedup_vs_plain_f64": plain2 / float(np.mean(fk))}
         del S2, dS2, pb2
@@ -441,6 +512,14 @@ def main():
         B3 = c4["posA"].shape[0]
         out["ovl3c_c4a"] = {"B": B3, "kernel_ms": float(np.mean(ks)), "wall_ms": wall3,
                             "triples_per_s": B3 / (wall3 * 1e-3), "kernel_triples_per_s": B3 / (np.mean(ks) * 1e-3)}
+        basis3d = aok.CartesianGTOBasis(2, cb.ns, cb.K, cb.ls, nbf3, off3, off3[-1],
+                                        aok.as_fortran(orb32.Rnl.zeta.double()), aok.as_fortran(coef3.double()),
+                                        zlist, 1.0)
+        X3d = [(x[0].double().contiguous(), x[1]) for x in X3]
+        V64 = aok.batch_overlap_3c(basis3d, *X3d, ctx=ctx)
+        num3 = (V.double() - V64).abs().amax(dim=(0, 1, 2))
+        fp32["c4a_V_blockwise"] = float((num3 / V64.abs().amax(dim=(0, 1, 2)).clamp_min(1e-300)).max())
+        del V64
         # the 3-centre (coef, alpha) pullback of a random cotangent at the same size
         log("3c pullback B=2^14")
         g3 = torch.Generator(device=dev).manual_seed(11)
@@ -457,34 +536,63 @@ def main():
         # FP32 against the FP64 pullback of the same FP32-rounded inputs, relative to the
         # largest |cotangent| of each output
         dc32, da32 = aok.overlap_3c_pb_coef_alpha(dT, basis3, *X3, ctx=ctx)
-        basis3d = aok.CartesianGTOBasis(2, cb.ns, cb.K, cb.ls, nbf3, off3, off3[-1],
-                                        aok.as_fortran(orb32.Rnl.zeta.double()), aok.as_fortran(coef3.double()),
-                                        zlist, 1.0)
-        X3d = [(x[0].double().contiguous(), x[1]) for x in X3]
         dc64, da64 = aok.overlap_3c_pb_coef_alpha(aok.as_fortran(dT.double()), basis3d, *X3d, ctx=ctx)
         err3 = max(float((dc32.double() - dc64).abs().max() / dc64.abs().max()),
                    float((da32.double() - da64).abs().max() / da64.abs().max()))
         out["ovl3c_c4a_pullback"] = {"B": B3, "kernel": "ovl3c_bwd_rows", "kernel_ms": float(np.mean(ks)),
                                      "wall_ms": wall3b, "triples_per_s": B3 / (wall3b * 1e-3),
                                      "fp32_max_rel_err": err3}
+        fp32["c4a_dcoef_dalpha"] = err3
         del V, dT, dc64, da64
 
     if not args.no_cpu:
-        log("CPU baseline (C oracle, FP64, 1 thread)")
+        # The reference's KernelAbstractions CPU backend cannot run here (Julia is absent); the
+        # baseline is the FP64 C restatement of the same kernels, timed on the host cores with
+        # OpenMP over the (b, s_a, s_b) work items (KA's CPU split) and on one thread.
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
 
-        nb = 16384
-        coef = O.compile_coef(cb.zeta, cb.D, cb.ls)
-        dSn = np.random.default_rng(0).standard_normal((cb.N, cb.N, nb))
-        ts = time.perf_counter()
-        O.ovl2c(d["posA"][:nb], d["posB"][:nb], d["sidxA"][:nb], d["sidxB"][:nb], cb.ls, coef, cb.zeta)
-        O.pullback_ps_2c(dSn, d["posA"][:nb], d["posB"][:nb], d["sidxA"][:nb], d["sidxB"][:nb], cb.ls, cb.zeta,
-                         cb.D)
-        tcpu = time.perf_counter() - ts
-        out["cpu_baseline"] = {"value": nb / tcpu, "unit": "pairs/s", "cores": 1, "kind": "port",
-                               "sample": f"first {nb} of the {B} C1 pairs: forward + (zeta,D) pullback, FP64 C "
-                                         f"restatement of the reference kernels ({tcpu:.1f} s)"}
+        nthr = O.num_threads()
+        log(f"CPU baseline (C oracle, FP64, {nthr} threads and 1 thread)")
+
+        def r32(a):
+            return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+        zr, Dr = r32(cb.zeta), r32(cb.D)
+        pAr, pBr = r32(d["posA"]), r32(d["posB"])
+        coef = O.compile_coef(zr, Dr, cb.ls)
+
+        def cpu_step(nb, omp):
+            dSn = r32(np.random.default_rng(0).standard_normal((cb.N, cb.N, nb)))
+            ts = time.perf_counter()
+            Sc = O.ovl2c(pAr[:nb], pBr[:nb], d["sidxA"][:nb], d["sidxB"][:nb], cb.ls, coef, zr, omp=omp)
+            gc = O.pullback_ps_2c(dSn, pAr[:nb], pBr[:nb], d["sidxA"][:nb], d["sidxB"][:nb], cb.ls, zr, Dr, omp=omp)
+            return time.perf_counter() - ts, Sc, gc, dSn
+
+        nb_mt, nb_st = 32768, 4096
+        t_mt, S_c, g_c, dS_c = cpu_step(nb_mt, True)
+        t_st, _, _, _ = cpu_step(nb_st, False)
+        out["cpu_baseline"] = {
+            "value": nb_mt / t_mt, "unit": "pairs/s", "cores": nthr, "kind": "port",
+            "cpu_model": _cpu_model(),
+            "sample": f"first {nb_mt} of the {B} C1 pairs: forward + (zeta,D) pullback, FP64 C restatement of "
+                      f"the reference kernels, OpenMP static over (b, s_a, s_b) on {nthr} threads ({t_mt:.1f} s)",
+            "single_thread": {"value": nb_st / t_st, "unit": "pairs/s", "cores": 1,
+                              "sample": f"first {nb_st} pairs, same work, serial ({t_st:.1f} s)"}}
+        # the same pairs on the GPU (FP32 fast path) against these oracle outputs: the bench's
+        # oracle-anchored error check (the FP64-kernel comparisons above cover the full sizes)
+        sub = slice(0, nb_mt)
+        XAs = (XA32[0][sub].contiguous(), XA32[1][sub].contiguous())
+        XBs = (XB32[0][sub].contiguous(), XB32[1][sub].contiguous())
+        Ss, pbs = aok.rrule_batch_overlap(orb32, XAs, XBs, ps32, ctx=ctx)
+        gs = pbs(aok.fortran_from_numpy(dS_c, torch.float32, dev))[4]["Rnl"]
+        Sh = Ss.double().cpu().numpy()
+        errS = float(np.max(np.max(np.abs(Sh - S_c), axis=(0, 1)) / np.maximum(np.max(np.abs(S_c), axis=(0, 1)),
+                                                                                    1e-300)))
+        errz = float(np.max(np.abs(gs["zeta"].double().cpu().numpy() - g_c[0])) / np.max(np.abs(g_c[0])))
+        errD = float(np.max(np.abs(gs["D"].double().cpu().numpy() - g_c[1])) / np.max(np.abs(g_c[1])))
+        out.setdefault("fp32_error", {})["c1_vs_oracle_subset"] = {
+            "pairs": nb_mt, "S_blockwise": errS, "dzeta": errz, "dD": errD}
     print(json.dumps(out), flush=True)
 
 

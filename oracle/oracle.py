@@ -68,6 +68,11 @@ def lib():
     return _LIB
 
 
+def num_threads() -> int:
+    """OpenMP threads the oracle's parallel forms use (OMP_NUM_THREADS or the machine's)."""
+    return int(lib().orc_num_threads())
+
+
 def _d(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(_D)

@@ -79,7 +79,7 @@ def test_c3_overlap_loss_gradient_vs_fd(dev):
 
 def test_c3_fp32_gradients_vs_fp64(dev):
     """FP32 rrule cotangents vs FP64 ones on the same FP32-rounded inputs, full batch.
-    Bound: 1e-5 of the largest |cotangent| (FP32 terms, FP64 reductions)."""
+    Bound: 1e-6 of the largest |cotangent| (FP32 terms, FP64 reductions; measured 4.9e-7)."""
     d, cb, XA, XB, S_tgt = _loss_setup(dev, torch.float32)
     log("C3 FP32 vs FP64 cotangents")
     o32 = _orb(cb, cb.zeta, cb.D, torch.float32, dev)
@@ -91,8 +91,8 @@ def test_c3_fp32_gradients_vs_fp64(dev):
     X64B = (XB[0].double().contiguous(), XB[1])
     _, pb64 = aok.rrule_batch_overlap(o64, X64A, X64B, o64.static_params(), None)
     g64 = pb64(aok.as_fortran(dS.double()))[4]["Rnl"]
-    assert rel_max(host(g32["zeta"]), host(g64["zeta"])) < 1e-5
-    assert rel_max(host(g32["D"]), host(g64["D"])) < 1e-5
+    assert rel_max(host(g32["zeta"]), host(g64["zeta"])) < 1e-6
+    assert rel_max(host(g32["D"]), host(g64["D"])) < 1e-6
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])

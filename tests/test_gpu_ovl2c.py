@@ -4,7 +4,8 @@ Tolerances (written per assertion):
   FP64 kernels (fast and plain) vs the FP64 oracle: 1e-12 relative to the block max
   (values), 1e-10 (gradients, which sum many terms);
   FP32 fast kernel vs the FP64 oracle on the same FP32-rounded inputs: blockwise
-  max_b max_ij |dS| / max_ij |S64_b| <= 1e-6 (the target metric), gradients 1e-5;
+  max_b max_ij |dS| / max_ij |S64_b| <= 1e-6 (the target metric), gradients 1e-6 of the
+  largest |cotangent| for the C1 basis (measured 1.0e-7 - 8.5e-7), 1e-5 for the uneven batch;
   FP32 plain kernel (the reference's own FP32 algorithm): 1e-4 like
   test/integrals/test_compile_orbitals.jl:138."""
 import json
@@ -119,7 +120,7 @@ def test_c1_pullback_vs_oracle(dev, mode, dtype):
     _, _, _, _, dps, _ = pb(dev_f(dS, dtype, dev))
     dzr, dDr = O.pullback_ps_2c(rounded(dS, dtype), rounded(d["posA"], dtype), rounded(d["posB"], dtype),
                                 d["sidxA"], d["sidxB"], cb.ls, rounded(cb.zeta, dtype), rounded(cb.D, dtype))
-    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    tol = 1e-10 if dtype == torch.float64 else 1e-6
     assert rel_max(host(dps["Rnl"]["zeta"]), dzr) < tol
     assert rel_max(host(dps["Rnl"]["D"]), dDr) < tol
     # kernel-level (dcoef, dalpha) against the oracle's batch_S_orbital_pb_kernel!
@@ -128,8 +129,8 @@ def test_c1_pullback_vs_oracle(dev, mode, dtype):
     coef = O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls)
     dcr, dar = O.ovl2c_pb(rounded(dS, dtype), rounded(d["posA"], dtype), rounded(d["posB"], dtype), d["sidxA"],
                           d["sidxB"], cb.ls, coef, rounded(cb.zeta, dtype))
-    assert rel_max(host(dc), dcr) < (1e-10 if dtype == torch.float64 else 1e-5)
-    assert rel_max(host(da), dar) < (1e-10 if dtype == torch.float64 else 1e-5)
+    assert rel_max(host(dc), dcr) < (1e-10 if dtype == torch.float64 else 1e-6)
+    assert rel_max(host(da), dar) < (1e-10 if dtype == torch.float64 else 1e-6)
     if mode == "fast":  # fixed-order FP64 reductions: bitwise reproducible
         _, _, _, _, dps2, _ = pb(dev_f(dS, dtype, dev))
         assert torch.equal(dps["Rnl"]["zeta"], dps2["Rnl"]["zeta"])
@@ -284,10 +285,14 @@ def test_c2_uneven_six_species(dev, dtype):
     N = 50, K = 10), Zipf-distributed species pairs; forward on a 2500-pair subset and the
     (zeta, D) pullback on 300 pairs against the oracle.  Tolerances: FP64 1e-12 / 1e-10;
     FP32 blockwise 3e-6 / gradients 1e-5.  The FP32 bound is looser than C1's 1e-6: the
-    worst blocks pair 9-10-primitive f shells (mixed-sign coefficients, zeta in
-    [0.1, 7000]) with 6-primitive s shells, where primitive-pair terms cancel to a sum
-    much smaller than themselves and every FP32 term carries a few ulp; measured worst
-    block 1.6e-6 over 20000 pairs."""
+    worst blocks are s-s (and s-p, s-d, d-s, f-s) blocks of contractions of up to 10
+    primitives with mixed-sign coefficients and zeta in [0.1, 7000], where up to 100
+    primitive-pair terms cancel to a sum several times smaller than themselves.  Each FP32
+    source of a term -- the rounded record constant c_a c_b (pi/p)^(3/2), v_exp_f32, the
+    1-D tables and products, the FP32 sum -- alone gives 0.5-1.5e-6 on such a block
+    (numpy emulation of the kernel arithmetic, one source at a time); no single one
+    dominates, so only FP64 terms (the FP64 kernels) close the gap.  Measured worst block
+    2.1e-6 over the 2^18-pair batch."""
     d = configs.c2(2, B=2500)
     cb = d["basis"]
     log(f"2c C2 subset dtype={dtype}")
