@@ -10,9 +10,11 @@ module AOKHip
 
 using AMDGPU: ROCArray, ROCVector, ROCMatrix, stream
 using StaticArrays: SVector
+using DecoratedParticles: PState
 import Unitful
+import ChainRulesCore: rrule, NoTangent, unthunk
 import AtomicOrbitalKernels as AOK
-import AtomicOrbitalKernels: evaluate, evaluate_ed, batch_overlap!, pullback_ps
+import AtomicOrbitalKernels: evaluate, evaluate_ed, batch_overlap!, batch_overlap_3c!, pullback_ps
 
 const libaok = joinpath(@__DIR__, "..", "lib", "libaok.so")
 
@@ -79,11 +81,14 @@ evaluate_ed(basis::AOK.AtomicOrbitals, Rs::ROCVector{<:SVector{3}}, sidx::ROCVec
     _ao_eval(basis, Rs, sidx, ps, st, true)
 
 # ---- H1 pullback: pullback_ps + _aorb_pullback_x (atomicorbitals.jl:254-299) ----------
-function pullback_ps(∂P::ROCMatrix{T}, basis::AOK.AtomicOrbitals, Rs::ROCVector{SVector{3, T}},
-                     sidx::ROCVector{<:Integer}, ps::NamedTuple, st) where {T}
+# One aok_ao_pullback call gives the parameter cotangents and, with want_x, the position
+# cotangent sum_o dP[i,o] grad P[i,o] that _aorb_pullback_x forms from a stored dP.
+function _ao_pullback(∂P::ROCMatrix{T}, basis::AOK.AtomicOrbitals, Rs::ROCVector{SVector{3, T}},
+                      sidx::ROCVector{<:Integer}, ps, st, want_x::Bool) where {T}
     ζ = ps.Rnl.ζ; D = ps.Rnl.D
     nRad, K, NZ = size(ζ)
     ∂ζ = similar(ζ); ∂D = similar(D)
+    ∂Rs = want_x ? similar(Rs) : nothing
     rc = ccall((:aok_ao_pullback, libaok), Cint,
                (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64,
                 Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Int64, Cint, Cdouble, Cint, Ptr{Cvoid}, Ptr{Cvoid},
@@ -91,15 +96,33 @@ function pullback_ps(∂P::ROCMatrix{T}, basis::AOK.AtomicOrbitals, Rs::ROCVecto
                ctx().h, _dt(T), pointer(Rs), pointer(sidx), eltype(sidx) == Int32 ? AOK_I32 : AOK_I64,
                length(Rs), pointer(ζ), pointer(D), nRad, K, NZ, _ip(st.Rnl.poly), _ip(st.iR), _ip(st.iY),
                length(basis), AOK._ylm_degree(basis.Ylm), Float64(basis.lengthscale),
-               basis.Rnl isa AOK.GaussianTypeRadials ? Cint(0) : Cint(1), pointer(∂P), C_NULL,
-               pointer(∂ζ), pointer(∂D))
-    check(rc, "pullback_ps")
-    return (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple())
+               basis.Rnl isa AOK.GaussianTypeRadials ? Cint(0) : Cint(1), pointer(∂P),
+               want_x ? pointer(∂Rs) : C_NULL, pointer(∂ζ), pointer(∂D))
+    check(rc, want_x ? "rrule(evaluate) pullback" : "pullback_ps")
+    return ∂Rs, (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple())
+end
+
+pullback_ps(∂P::ROCMatrix{T}, basis::AOK.AtomicOrbitals, Rs::ROCVector{SVector{3, T}},
+            sidx::ROCVector{<:Integer}, ps::NamedTuple, st) where {T} =
+    _ao_pullback(∂P, basis, Rs, sidx, ps, st, false)[2]
+
+# rrule(evaluate, basis, X, ps, st) (atomicorbitals.jl:307-328) dispatches to _aorb_rrule
+# with the extracted device positions; the forward keeps no dP (the pullback kernel forms
+# the spatial Jacobian in registers), the cotangent mirrors the input kind (_xtangent).
+function AOK._aorb_rrule(basis::AOK.AtomicOrbitals, X, Rs::ROCVector{SVector{3, T}},
+                         sidx::ROCVector{<:Integer}, ps, st) where {T}
+    P = _ao_eval(basis, Rs, sidx, ps, st, false)
+    function _aorb_pullback(_∂)
+        ∂P = ROCArray{T}(unthunk(_∂))
+        ∂Rs, ∂ps = _ao_pullback(∂P, basis, Rs, sidx, ps, st, true)
+        return (NoTangent(), NoTangent(), AOK._xtangent(X, ∂Rs), ∂ps, NoTangent())
+    end
+    return P, _aorb_pullback
 end
 
 # ---- H2: batch_overlap!(out, ::CartesianGTOBasis, XA, XB) (overlap_2c.jl:171-197) ----------
-function batch_overlap!(out::ROCArray{FT, 3}, basis::AOK.CartesianGTOBasis, XA::AbstractVector,
-                        XB::AbstractVector; kwargs...) where {FT}
+function batch_overlap!(out::ROCArray{FT, 3}, basis::AOK.CartesianGTOBasis, XA::AbstractVector{<:PState},
+                        XB::AbstractVector{<:PState}; kwargs...) where {FT}
     B = length(XA)
     length(XB) == B || error("XA and XB must have equal length; got $(length(XA)) and $(length(XB))")
     posA = ROCArray(AOK._cgto_posmat(XA, FT)); posB = ROCArray(AOK._cgto_posmat(XB, FT))
@@ -115,9 +138,30 @@ function batch_overlap!(out::ROCArray{FT, 3}, basis::AOK.CartesianGTOBasis, XA::
     return out
 end
 
+# ---- H2 kernel pullback: _overlap_pb_coef_alpha(∂S, basis, XA, XB) (overlap_2c_grad.jl:218-242)
+function AOK._overlap_pb_coef_alpha(∂S::ROCArray{FT, 3}, basis::AOK.CartesianGTOBasis,
+                                    XA::AbstractVector{<:PState}, XB::AbstractVector{<:PState};
+                                    kwargs...) where {FT}
+    B = length(XA)
+    posA = ROCArray(AOK._cgto_posmat(XA, FT)); posB = ROCArray(AOK._cgto_posmat(XB, FT))
+    sA = ROCArray(AOK._cgto_sidx(basis, XA)); sB = ROCArray(AOK._cgto_sidx(basis, XB))
+    coef = ROCArray(FT.(basis.coef)); α = ROCArray(FT.(basis.ζ))
+    ns, K, NZ = size(basis.coef)
+    ∂c = similar(coef); ∂α = similar(α)
+    rc = ccall((:aok_ovl2c_pullback, libaok), Cint,
+               (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64,
+                Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}),
+               ctx().h, _dt(FT), AOK_FAST, pointer(∂S), pointer(posA), pointer(posB), pointer(sA), pointer(sB),
+               AOK_I64, B, ns, K, NZ, _ip(basis.ls), pointer(coef), pointer(α), Float64(basis.lengthscale),
+               pointer(∂c), pointer(∂α))
+    check(rc, "_overlap_pb_coef_alpha")
+    return ∂c, ∂α
+end
+
 # ---- H2 pullback: pullback_ps(∂S, orb, XA, XB, ps, st) (overlap_2c_grad.jl:296-304) ------
-function pullback_ps(∂S::ROCArray{FT, 3}, orb::AOK.GaussianTypeOrbitals, XA::AbstractVector,
-                     XB::AbstractVector, ps, st) where {FT}
+# kernel pullback + normalisation VJP in one call
+function pullback_ps(∂S::ROCArray{FT, 3}, orb::AOK.GaussianTypeOrbitals, XA::AbstractVector{<:PState},
+                     XB::AbstractVector{<:PState}, ps, st) where {FT}
     str = AOK._compile_struct(orb)
     B = length(XA)
     posA = ROCArray(AOK._cgto_posmat(XA, FT)); posB = ROCArray(AOK._cgto_posmat(XB, FT))
@@ -137,8 +181,10 @@ function pullback_ps(∂S::ROCArray{FT, 3}, orb::AOK.GaussianTypeOrbitals, XA::A
 end
 
 # ---- H2 rrule: forward + pullback on one plan (overlap_2c_grad.jl:309-318) -------------
-import ChainRulesCore: rrule, NoTangent
+# Taken when the parameters live on the device (ps.Rnl.ζ, ps.Rnl.D ROCArrays): strictly
+# more specific than the reference's method, which stays in charge of host parameters.
 const AOK_PARAMS_PS = Cint(1)
+const RocPS = NamedTuple{(:Rnl, :Ylm), <:Tuple{NamedTuple{(:ζ, :D), <:Tuple{ROCArray, ROCArray}}, Any}}
 
 mutable struct Plan2c
     h::Ptr{Cvoid}
@@ -178,13 +224,13 @@ _plan_pullback(plan, ∂S, posA, posB, ∂ζ, ∂D) =
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
                 plan.h, pointer(∂S), pointer(posA), pointer(posB), pointer(∂ζ), pointer(∂D)), "pullback_ps")
 
-function rrule(::typeof(AOK.batch_overlap), orb::AOK.GaussianTypeOrbitals, XA::AbstractVector,
-               XB::AbstractVector, ps, st)
+function rrule(::typeof(AOK.batch_overlap), orb::AOK.GaussianTypeOrbitals, XA::AbstractVector{<:PState},
+               XB::AbstractVector{<:PState}, ps::RocPS, st)
     plan, posA, posB, ζ, D, FT, N, B = _plan2c(orb, XA, XB, ps)
     S = similar(posA, FT, N, N, B)
     _plan_forward(plan, posA, posB, S)
-    function batch_overlap_pullback(∂S)
-        ∂Sd = ROCArray{FT}(∂S); ∂ζ = similar(ζ); ∂D = similar(D)
+    function batch_overlap_pullback(_∂)
+        ∂Sd = ROCArray{FT}(unthunk(_∂)); ∂ζ = similar(ζ); ∂D = similar(D)
         _plan_pullback(plan, ∂Sd, posA, posB, ∂ζ, ∂D)
         return NoTangent(), NoTangent(), NoTangent(), NoTangent(), (Rnl = (ζ = ∂ζ, D = ∂D), Ylm = NamedTuple()),
                NoTangent()
@@ -236,18 +282,22 @@ function replay!(sg::StepGraph)
     return sg.S, (Rnl = (ζ = sg.∂ζ, D = sg.∂D), Ylm = NamedTuple())
 end
 
-# ---- H2 3-centre: batch_overlap_3c!(out, basis, posA, posB, posC) (kernels_3c.jl:183-200) --
-# A CompiledBasis without species maps onto one species (NZ = 1, sidx = NULL) whose
-# primitive slots are zero-padded to the longest contraction.
-function batch_overlap_3c!(out::ROCArray{FT, 4}, ls::Vector{Int}, coef::ROCArray{FT, 3}, α::ROCArray{FT, 3},
-                           posA::ROCMatrix{FT}, posB::ROCMatrix{FT}, posC::ROCMatrix{FT}) where {FT}
-    ns, K, NZ = size(coef)
-    B = size(posA, 2)
+# ---- H2 3-centre: batch_overlap_3c!(out, ::CompiledBasis, posA, posB, posC) (kernels_3c.jl:183-200) --
+# Unitful (3, B) positions stripped by the reference's own to_bohr; the CompiledBasis
+# (no species) becomes one species whose primitive slots are zero-padded to the longest
+# contraction (_padded, as the 2-centre geometry-in method below).
+function batch_overlap_3c!(out::ROCArray{FT, 4}, basis::AOK.CompiledBasis,
+                           posA::AbstractMatrix{<:Unitful.Length}, posB::AbstractMatrix{<:Unitful.Length},
+                           posC::AbstractMatrix{<:Unitful.Length}; kwargs...) where {FT}
+    A = ROCArray(AOK.to_bohr(posA, FT)); Bm = ROCArray(AOK.to_bohr(posB, FT)); C = ROCArray(AOK.to_bohr(posC, FT))
+    coef, α = _padded(basis, FT)
+    ns, K, _ = size(coef)
     rc = ccall((:aok_ovl3c, libaok), Cint,
                (Ptr{Cvoid}, Cint, Cuint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint,
                 Int64, Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Ptr{Cvoid}),
-               ctx().h, _dt(FT), AOK_FAST, pointer(posA), pointer(posB), pointer(posC), C_NULL, C_NULL, C_NULL,
-               AOK_I64, B, ns, K, NZ, ls, pointer(coef), pointer(α), 1.0, pointer(out))
+               ctx().h, _dt(FT), AOK_FAST, pointer(A), pointer(Bm), pointer(C), C_NULL, C_NULL, C_NULL,
+               AOK_I64, size(posA, 2), ns, K, 1, _ip(Vector{Int}(Array(basis.ls))), pointer(coef), pointer(α), 1.0,
+               pointer(out))
     check(rc, "batch_overlap_3c!")
     return out
 end
