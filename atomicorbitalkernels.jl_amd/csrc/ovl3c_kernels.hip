@@ -46,7 +46,7 @@ struct Rec3Of<double> {
 
 struct Ovl3Ws {
     size_t coef64, alpha64, nreal, lidx, grp_off, rec;
-    size_t blk_hist, rank, blk_off, bcnt, boff, toff, perm, tile_key;
+    size_t blk_hist, rank, blk_off, bcnt, boff, toff, perm, tile_key, tri;
     size_t total;
     int nblk_h, NK, maxtiles;
     int64_t ngrp, maxrec;
@@ -58,7 +58,9 @@ static Ovl3Ws ovl3_ws_map(int64_t B, int ns, int K, int NZ, size_t rec_bytes)
     const size_t npar = static_cast<size_t>(ns) * K * NZ;
     w.NK = NZ * NZ * NZ;
     w.nblk_h = static_cast<int>((B + 255) / 256);
-    w.maxtiles = static_cast<int>((B + OVL_TILE - 1) / OVL_TILE) + w.NK;
+    // tiles padded to a multiple of 8: tile t's workgroups (linear id t + y * maxtiles) all
+    // run on XCD t mod 8, so its blocks' partial lines meet in that XCD's L2
+    w.maxtiles = ((static_cast<int>((B + OVL_TILE - 1) / OVL_TILE) + w.NK) + 7) & ~7;
     w.ngrp = static_cast<int64_t>(w.NK) * ns * ns * ns;
     w.maxrec = w.ngrp * K * K * K;
     size_t o = 0;
@@ -81,6 +83,7 @@ static Ovl3Ws ovl3_ws_map(int64_t B, int ns, int K, int NZ, size_t rec_bytes)
     w.toff = take(sizeof(int) * (w.NK + 1));
     w.perm = take(sizeof(int) * static_cast<size_t>(B > 0 ? B : 1));
     w.tile_key = take(sizeof(int) * static_cast<size_t>(w.maxtiles));
+    w.tri = take(sizeof(int) * static_cast<size_t>(ns) * ns * ns);
     w.total = o;
     return w;
 }
@@ -119,7 +122,8 @@ __global__ __launch_bounds__(256) void ovl3_plan_kernel(const OvlShape sh, const
                                                         int *__restrict__ lidx, int64_t *__restrict__ grp_off,
                                                         int nblk_h, const int *__restrict__ blk_hist,
                                                         int *__restrict__ blk_off, int *__restrict__ bcnt,
-                                                        int *__restrict__ boff, int *__restrict__ toff)
+                                                        int *__restrict__ boff, int *__restrict__ toff,
+                                                        int *__restrict__ tri, int light_l)
 {
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, NK = NZ * NZ * NZ;
     for (int idx = threadIdx.x; idx < NZ * ns; idx += blockDim.x) {
@@ -135,6 +139,48 @@ __global__ __launch_bounds__(256) void ovl3_plan_kernel(const OvlShape sh, const
         nreal[idx] = nr;
     }
     bucket_offsets(NK, nblk_h, blk_hist, blk_off, bcnt, boff, toff);
+    // the forward's shell triples: those of light shells only (l <= light_l) first, then
+    // the others, each part in (c, b, a) order (measured on C4a: 0.427 ms, against 0.44-0.45
+    // with the heavy part ordered by its number of d shells).  Stream compaction in rounds
+    // of 256 triples: wave ballots give each triple its rank within its part, the running
+    // part totals sit in LDS.
+    {
+        __shared__ int part_tot[2], wcnt[4][2];
+        const int ns3 = ns * ns * ns;
+        int nL = 0;
+        for (int q = 0; q < ns; ++q) nL += sh.ls[q] <= light_l;
+        const int poff[2] = {0, nL * nL * nL};
+        if (threadIdx.x < 2) part_tot[threadIdx.x] = 0;
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int q0 = 0; q0 < ns3; q0 += 256) {
+            const int q = q0 + static_cast<int>(threadIdx.x);
+            int m = -1, sa = 0, sb = 0, sc = 0;
+            if (q < ns3) {
+                sa = q % ns;
+                sb = (q / ns) % ns;
+                sc = q / (ns * ns);
+                m = (sh.ls[sa] > light_l || sh.ls[sb] > light_l || sh.ls[sc] > light_l) ? 1 : 0;
+            }
+            int rank = 0;
+            for (int k = 0; k < 2; ++k) {
+                const unsigned long long bal = __ballot(m == k);
+                if (m == k) rank = __popcll(bal & ((1ull << lane) - 1ull));
+                if (lane == 0) wcnt[wave][k] = __popcll(bal);
+            }
+            __syncthreads();
+            if (m >= 0) {
+                int pos = poff[m] + part_tot[m] + rank;
+                for (int w = 0; w < wave; ++w) pos += wcnt[w][m];
+                tri[pos] = sa | (sb << 8) | (sc << 16);
+            }
+            __syncthreads();
+            if (threadIdx.x < 2)
+                part_tot[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] + wcnt[2][threadIdx.x] +
+                                         wcnt[3][threadIdx.x];
+            __syncthreads();
+        }
+    }
     // record offsets: contiguous ranges of groups per thread, then a scan of the partials
     __shared__ int64_t wtot[4];
     const int64_t ngrp = static_cast<int64_t>(NK) * ns * ns * ns;
@@ -225,6 +271,16 @@ __global__ __launch_bounds__(256) void ovl3_fill_kernel(const OvlShape sh, const
     rec[grp_off[g] + (static_cast<int64_t>(i) * nb + j) * nc + k] = r;
 }
 
+// Unroll factor of the forward's primitive-triple loop (records of several triples
+// loaded together).
+#ifndef AOK_F3_UNROLL
+#define AOK_F3_UNROLL 2
+#endif
+#ifndef AOK_PRAGMA
+#define AOK_PRAGMA_(x) _Pragma(#x)
+#define AOK_PRAGMA(x) AOK_PRAGMA_(x)
+#endif
+
 template <int L>
 struct Sh3 {
     static constexpr int nbf = (L + 1) * (L + 2) / 2;
@@ -288,6 +344,7 @@ __device__ __forceinline__ void triple_fwd(const typename Rec3Of<T>::type *__res
 #pragma unroll
     for (int q = 0; q < NLIN; ++q) acc[q] = T(0);
     const int ntrip = na * nb * nc;
+    AOK_PRAGMA(unroll AOK_F3_UNROLL)
     for (int t = 0; t < ntrip; ++t) {
         const typename Rec3Of<T>::type r = recs[t];
         const T G = r.Kabc * trip_exp(r, geo.rab, geo.rac, geo.rbc);
@@ -353,14 +410,17 @@ __device__ __forceinline__ void triple_fwd(const typename Rec3Of<T>::type *__res
         X(1, 0, 1) X(1, 0, 2) X(1, 1, 0) X(1, 1, 1) X(1, 1, 2) X(1, 2, 0) X(1, 2, 1) X(1, 2, 2) X(2, 0, 0)       \
             X(2, 0, 1) X(2, 0, 2) X(2, 1, 0) X(2, 1, 1) X(2, 1, 2) X(2, 2, 0) X(2, 2, 1) X(2, 2, 2)
 
-// The FP32 l <= 2 instance is compiled for 2 waves/SIMD (256 VGPRs and a 12-byte spill
-// instead of 1 wave: C4a 0.795 -> 0.761 ms); the others keep the compiler's choice.
+// The FP32 l <= 2 instance is compiled for 2 waves/SIMD (256 VGPRs instead of 1 wave).
 template <typename T, int LMAXB>
 constexpr int ovl3_fwd_waves()
 {
     return std::is_same<T, float>::value && LMAXB == 2 ? 2 : 1;
 }
 
+// Grid (tile, shell triple): workgroup (x, y) takes tile x and the shell triple
+// tri[tri0 + y] of this launch's range, so a heavy triple (9 x 9 x 9 primitive triples) and
+// a light one (1 x 1 x 1) are separate workgroups that the dispatcher balances; the tiles
+// of one triple are consecutive (their records are shared through the caches).
 template <typename T, int LMAXB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_waves<T, LMAXB>(), 8))) void ovl3c_fwd_fast(const OvlShape sh, const T *__restrict__ posA,
                                                      const T *__restrict__ posB, const T *__restrict__ posC, T scale,
@@ -369,13 +429,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_wav
                                                      const int *__restrict__ toff,
                                                      const typename Rec3Of<T>::type *__restrict__ rec,
                                                      const int64_t *__restrict__ grp_off,
-                                                     const int *__restrict__ nreal, T *__restrict__ out, int maxtiles,
-                                                     int part)
+                                                     const int *__restrict__ nreal, const int *__restrict__ tri,
+                                                     int tri0, T *__restrict__ out)
 {
     const int ns = sh.ns, NZ = sh.NZ, N = sh.N, NK = NZ * NZ * NZ;
-    // (tile, shell pair (s_c, s_b)) of the block, tile fastest
-    const int t = static_cast<int>(blockIdx.x) % maxtiles, cb = static_cast<int>(blockIdx.x) / maxtiles;
+    const int t = static_cast<int>(blockIdx.x);
     if (t >= toff[NK]) return;
+    const int code = tri[tri0 + static_cast<int>(blockIdx.y)];
+    const int sa = code & 0xff, sb = (code >> 8) & 0xff, sc = code >> 16;
     const int key = tile_key[t];
     const int first = boff[key] + (t - toff[key]) * OVL_TILE;
     const int nact = min(OVL_TILE, boff[key] + bcnt[key] - first);
@@ -383,6 +444,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_wav
     const bool active = lane < nact;
     const int64_t b = perm[first + (active ? lane : 0)];
     const int sgA = key / (NZ * NZ), sgB = (key / NZ) % NZ, sgC = key % NZ;
+    const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
+    const int na = nreal[sgA * ns + sa], nb = nreal[sgB * ns + sb], nc = nreal[sgC * ns + sc];
     T A[3], Bv[3], C[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
@@ -404,27 +467,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_wav
         ac[ax] = static_cast<T>(dac);
     }
     T *ob = out + static_cast<int64_t>(N) * N * N * b;
-    const int sc = cb / ns, sb = cb - sc * ns;
-    const int lc = sh.ls[sc], nc = nreal[sgC * ns + sc];
-    const int lb = sh.ls[sb], nb = nreal[sgB * ns + sb];
-    // part 1: only triples of s and p shells; part 2: only triples with a d shell; 0: all
-    const bool light_bc = lb <= 1 && lc <= 1;
-    if (part == 1 && !light_bc) return;
-    for (int sa = 0; sa < ns; ++sa) {
-        const int la = sh.ls[sa], na = nreal[sgA * ns + sa];
-        if ((part == 1 && la > 1) || (part == 2 && light_bc && la <= 1)) continue;
-        const int64_t g = ((static_cast<int64_t>(key) * ns + sc) * ns + sb) * ns + sa;
-        const typename Rec3Of<T>::type *recs = rec + grp_off[g];
+    const int64_t g = ((static_cast<int64_t>(key) * ns + sc) * ns + sb) * ns + sa;
+    const typename Rec3Of<T>::type *recs = rec + grp_off[g];
 #define AOK_F3_CASE(A_, B_, C_)                                                                                 \
     if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB)                                                    \
         if (la == A_ && lb == B_ && lc == C_) {                                                                 \
             triple_fwd<T, A_, B_, C_>(recs, na, nb, nc, geo, ab, ac, ob, N, sh.off[sa], sh.off[sb], sh.off[sc], \
                                       active);                                                                  \
-            continue;                                                                                           \
+            return;                                                                                             \
         }
-        AOK_L3_CASES(AOK_F3_CASE)
+    AOK_L3_CASES(AOK_F3_CASE)
 #undef AOK_F3_CASE
-    }
 }
 
 // ---- plain path: one item per (b, s_a, s_b, s_c) --------------------------------------------
@@ -929,6 +982,10 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
                        c.itype, c.B, sh.NZ, reinterpret_cast<int *>(ws + w.rank),
                        reinterpret_cast<int *>(ws + w.blk_hist), ctx->d_err);
     AOK_HIP(ctx, hipGetLastError());
+    // light shells: those the l <= 1 instance takes (all shells when the basis has no d)
+    const int light_l = LMAXB == 2 ? 1 : LMAXB;
+    int nL = 0;
+    for (int q = 0; q < sh.ns; ++q) nL += sh.ls[q] <= light_l;
     hipLaunchKernelGGL((ovl3_plan_kernel<T>), dim3(1), dim3(256), 0, ctx->stream, sh,
                        static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha),
                        reinterpret_cast<double *>(ws + w.coef64), reinterpret_cast<double *>(ws + w.alpha64),
@@ -936,7 +993,7 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
                        reinterpret_cast<int64_t *>(ws + w.grp_off), w.nblk_h,
                        reinterpret_cast<const int *>(ws + w.blk_hist), reinterpret_cast<int *>(ws + w.blk_off),
                        reinterpret_cast<int *>(ws + w.bcnt), reinterpret_cast<int *>(ws + w.boff),
-                       reinterpret_cast<int *>(ws + w.toff));
+                       reinterpret_cast<int *>(ws + w.toff), reinterpret_cast<int *>(ws + w.tri), light_l);
     AOK_HIP(ctx, hipGetLastError());
     const int nblk_tile = (w.maxtiles + 255) / 256;
     const int nblk_rec = static_cast<int>((w.maxrec + 255) / 256);
@@ -949,25 +1006,29 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.lidx),
                        reinterpret_cast<const int64_t *>(ws + w.grp_off), reinterpret_cast<R *>(ws + w.rec));
     AOK_HIP(ctx, hipGetLastError());
-    // With d shells, the triples of s and p shells alone run in the l <= 1 instance (62
+    // With d shells, the triples of s and p shells alone run in the l <= 1 instance (64
     // VGPRs, 8 waves/SIMD) and the rest in the l <= 2 one (its (d, d, d) case holds 216
-    // accumulators: 256 VGPRs, 1 wave/SIMD).
+    // accumulators: 250 VGPRs, 2 waves/SIMD).  Measured on C4a: one launch per number of d
+    // shells (126 / 170 / 250 VGPRs) 0.44-0.45 ms against 0.43 ms for these two.
     time_begin(ctx, "ovl3c_fwd_fast");
-    const int grid = w.maxtiles * sh.ns * sh.ns;
-    auto launch = [&](auto kern, int part) {
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(OVL_TILE), 0, ctx->stream, sh, static_cast<const T *>(c.posA),
-                           static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
+    const int ns3 = sh.ns * sh.ns * sh.ns, nL3 = nL * nL * nL;
+    auto launch = [&](auto kern, int t0, int count) {
+        if (count <= 0) return;
+        hipLaunchKernelGGL(kern, dim3(w.maxtiles, count), dim3(OVL_TILE), 0, ctx->stream, sh,
+                           static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
+                           static_cast<const T *>(c.posC), static_cast<T>(c.scale),
                            reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                            reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                            reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                            reinterpret_cast<const int64_t *>(ws + w.grp_off),
-                           reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), w.maxtiles, part);
+                           reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.tri), t0,
+                           static_cast<T *>(c.out));
     };
     if constexpr (LMAXB == 2) {
-        launch(&ovl3c_fwd_fast<T, 1>, 1);
-        launch(&ovl3c_fwd_fast<T, 2>, 2);
+        launch(&ovl3c_fwd_fast<T, 1>, 0, nL3);
+        launch(&ovl3c_fwd_fast<T, 2>, nL3, ns3 - nL3);
     } else {
-        launch(&ovl3c_fwd_fast<T, LMAXB>, 0);
+        launch(&ovl3c_fwd_fast<T, LMAXB>, 0, ns3);
     }
     time_end(ctx);
     AOK_HIP(ctx, hipGetLastError());
