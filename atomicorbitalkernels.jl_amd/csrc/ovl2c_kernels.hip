@@ -289,15 +289,17 @@ __device__ __forceinline__ float pair_exp(const PairRecF &r, double R2)
     return ldexpf(__builtin_amdgcn_exp2f(-f), -static_cast<int>(n));
 }
 __device__ __forceinline__ double pair_exp(const PairRecD &r, double R2) { return exp(-r.mu2 * R2); }
-// Pullback: FP32 argument.  Every pullback term is summed over the batch and judged
-// against the largest cotangent, and an argument rounded in FP32 perturbs a term of size
-// e^{-y} by about y e^{-y} 2^-23 relative to its prefactor, so the large-y terms whose
-// per-block accuracy needs the FP64 argument in the forward are negligible here.  C1
-// (B = 65536): pullback 114 -> 110 us; FP32 vs FP64 cotangents 4.1e-7 (dzeta) and 1.1e-7
-// (dD) of the largest entry, against 4.9e-7 and 1.1e-7 with the FP64 argument.
-__device__ __forceinline__ float pair_exp_pb(const PairRecF &r, double, float r2f)
+// Pullback: the argument mu |AB|^2 log2(e) formed in FP64 and rounded once to FP32, then
+// v_exp_f32 (3 instructions instead of the forward's 7-instruction range reduction).  The
+// rounding perturbs a term e^{-y} by about y 2^-24 ln 2 of itself; pullback terms are
+// summed over the batch and judged against the largest cotangent, so the large-y terms
+// (whose per-block accuracy needs the forward's exact reduction) are negligible here.
+// C1 (B = 65536): pullback 110 -> 107 us; FP32 cotangents against the FP64 oracle on
+// 32768 pairs 4.7e-7 (dzeta) and 1.9e-7 (dD) of the largest entry, against 4.5e-7 and
+// 1.6e-7 with the exact reduction (with the product also in FP32: 106 us, 7.8e-7).
+__device__ __forceinline__ float pair_exp_pb(const PairRecF &r, double R2, float)
 {
-    return __builtin_amdgcn_exp2f(-(static_cast<float>(r.mu2) * r2f));
+    return __builtin_amdgcn_exp2f(-static_cast<float>(r.mu2 * R2));
 }
 __device__ __forceinline__ double pair_exp_pb(const PairRecD &r, double R2, double) { return pair_exp(r, R2); }
 
