@@ -58,8 +58,8 @@ static Ovl3Ws ovl3_ws_map(int64_t B, int ns, int K, int NZ, size_t rec_bytes)
     const size_t npar = static_cast<size_t>(ns) * K * NZ;
     w.NK = NZ * NZ * NZ;
     w.nblk_h = static_cast<int>((B + 255) / 256);
-    // tiles padded to a multiple of 8: tile t's workgroups (linear id t + y * maxtiles) all
-    // run on XCD t mod 8, so its blocks' partial lines meet in that XCD's L2
+    // tiles padded to a multiple of 8: the forward's grid maps tile t to XCD t mod 8 (both
+    // grid forms), so its blocks' partial lines meet in that XCD's L2
     w.maxtiles = ((static_cast<int>((B + OVL_TILE - 1) / OVL_TILE) + w.NK) + 7) & ~7;
     w.ngrp = static_cast<int64_t>(w.NK) * ns * ns * ns;
     w.maxrec = w.ngrp * K * K * K;
@@ -417,10 +417,9 @@ constexpr int ovl3_fwd_waves()
     return std::is_same<T, float>::value && LMAXB == 2 ? 2 : 1;
 }
 
-// Grid (tile, shell triple): workgroup (x, y) takes tile x and the shell triple
-// tri[tri0 + y] of this launch's range, so a heavy triple (9 x 9 x 9 primitive triples) and
-// a light one (1 x 1 x 1) are separate workgroups that the dispatcher balances; the tiles
-// of one triple are consecutive (their records are shared through the caches).
+// One workgroup per (tile, shell triple of this launch's range tri[tri0 ...]), so a heavy
+// triple (9 x 9 x 9 primitive triples) and a light one (1 x 1 x 1) are separate workgroups
+// that the dispatcher balances.
 template <typename T, int LMAXB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_waves<T, LMAXB>(), 8))) void ovl3c_fwd_fast(const OvlShape sh, const T *__restrict__ posA,
                                                      const T *__restrict__ posB, const T *__restrict__ posC, T scale,
@@ -430,12 +429,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_fwd_wav
                                                      const typename Rec3Of<T>::type *__restrict__ rec,
                                                      const int64_t *__restrict__ grp_off,
                                                      const int *__restrict__ nreal, const int *__restrict__ tri,
-                                                     int tri0, T *__restrict__ out)
+                                                     int tri0, int tiles8, T *__restrict__ out)
 {
     const int ns = sh.ns, NZ = sh.NZ, N = sh.N, NK = NZ * NZ * NZ;
-    const int t = static_cast<int>(blockIdx.x);
+    // 1-D grid, XCD-affine and triple-fastest: workgroup w runs on XCD w % 8; its sequence
+    // number s = w / 8 on that XCD walks tile q * 8 + w % 8 (q = s / count) through all
+    // `count` triples of the launch before the XCD's next tile, so a tile's output lines are
+    // completed in one L2
+    const int count = static_cast<int>(gridDim.x >> 3) / tiles8;
+    const int sq = static_cast<int>(blockIdx.x >> 3), q = sq / count;
+    const int t = q * 8 + static_cast<int>(blockIdx.x & 7u), j = sq - q * count;
     if (t >= toff[NK]) return;
-    const int code = tri[tri0 + static_cast<int>(blockIdx.y)];
+    const int code = tri[tri0 + j];
     const int sa = code & 0xff, sb = (code >> 8) & 0xff, sc = code >> 16;
     const int key = tile_key[t];
     const int first = boff[key] + (t - toff[key]) * OVL_TILE;
@@ -1030,7 +1035,11 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
     const int ns3 = sh.ns * sh.ns * sh.ns, nL3 = nL * nL * nL;
     auto launch = [&](auto kern, int t0, int count) {
         if (count <= 0) return;
-        hipLaunchKernelGGL(kern, dim3(w.maxtiles, count), dim3(OVL_TILE), 0, ctx->stream, sh,
+        // XCD-affine, triple-fastest 1-D grid (C4a: 0.423 -> 0.409 ms, writes 778 -> 586 MB for
+        // the 222 MB output; what remains are lines shared by the light and heavy launches).
+        // maxtiles * count < 2^31 for every output that fits in device memory (N >= ns, so
+        // the output holds at least 64 * ns^3 floats per tile); checked in ovl3c_run.
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(w.maxtiles * count)), dim3(OVL_TILE), 0, ctx->stream, sh,
                            static_cast<const T *>(c.posA), static_cast<const T *>(c.posB),
                            static_cast<const T *>(c.posC), static_cast<T>(c.scale),
                            reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
@@ -1038,7 +1047,7 @@ static aok_status fast3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &
                            reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                            reinterpret_cast<const int64_t *>(ws + w.grp_off),
                            reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.tri), t0,
-                           static_cast<T *>(c.out));
+                           w.maxtiles / 8, static_cast<T *>(c.out));
     };
     if constexpr (LMAXB == 2) {
         launch(&ovl3c_fwd_fast<T, 1>, 0, nL3);
@@ -1073,9 +1082,10 @@ aok_status ovl3c_run(aok_ctx ctx, const Ovl3cCall &c)
     const std::string msg = build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, 1 << 20, sh);
     if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
     if (c.B < 0 || c.B > (int64_t(1) << 28)) return fail(ctx, AOK_ERR_ARG, "B out of range");
-    // one workgroup per (64-triple tile, shell pair), or 128 items of (triple, shell triple)
+    // one workgroup per (64-triple tile, shell triple) (tiles padded by up to 8 x 8 x 8 species
+    // triples and to a multiple of 8), or 128 items of (triple, shell triple)
     const int64_t ns = sh.ns;
-    if (((c.B + OVL_TILE - 1) / OVL_TILE + 512) * ns * ns > INT32_MAX || (c.B * ns * ns * ns + 127) / 128 > INT32_MAX)
+    if (((c.B + OVL_TILE - 1) / OVL_TILE + 520) * ns * ns * ns > INT32_MAX || (c.B * ns * ns * ns + 127) / 128 > INT32_MAX)
         return fail(ctx, AOK_ERR_ARG, "B x shells^3 too large for one call: split the batch");
     int lmax = 0;
     for (int q = 0; q < sh.ns; ++q) lmax = std::max<int>(lmax, sh.ls[q]);
