@@ -270,6 +270,7 @@ def main():
         gctx.close()
         return ms
 
+    subsets = {}  # small GPU FP32 results for the CPU leg's oracle-anchored checks
     if not args.no_extra:
         # position cotangents of the same step (rrule with wrt_positions): their kernel alone
         log("C1 position cotangents")
@@ -432,6 +433,17 @@ def main():
         fp32["ao_c4b_dX"] = rel_err(dX32, dX64)
         fp32["ao_c4b_dzeta"] = rel_err(dq32["Rnl"]["zeta"], dq64["Rnl"]["zeta"])
         fp32["ao_c4b_dD"] = rel_err(dq32["Rnl"]["D"], dq64["Rnl"]["D"])
+        # the first 20000 points through the FP32 kernels, kept on the host for the CPU leg's
+        # oracle-anchored check
+        ns_ = 20000
+        Xs, ss = X[:ns_].contiguous(), sidx[:ns_].contiguous()
+        dPs = dPin[:ns_].t().contiguous().t()
+        Ps, dPgs = aok.evaluate_ed(orb4, Xs, sidx=ss, ctx=ctx)
+        _, _, dXs, dqs, _ = aok.rrule_evaluate(orb4, Xs, sidx=ss, ctx=ctx)[1](dPs)
+        host = lambda t: t.double().cpu().numpy()
+        subsets["ao"] = {"basis": c4, "X": host(Xs), "sidx": ss.cpu().numpy(), "dP": host(dPs), "P32": host(Ps),
+                         "dP32": host(dPgs), "dX32": host(dXs), "dz32": host(dqs["Rnl"]["zeta"]),
+                         "dD32": host(dqs["Rnl"]["D"])}
         del P, X, Xd, dPin, dX32, dX64
         # C2: uneven batch, 6 species (union spec N = 50, K = 10, l <= 3), Zipf species pairs
         log("C2 B=2^18 rrule step")
@@ -554,6 +566,16 @@ def main():
                                      "wall_ms": wall3b, "triples_per_s": B3 / (wall3b * 1e-3),
                                      "fp32_max_rel_err": err3}
         fp32["c4a_dcoef_dalpha"] = err3
+        # the first 256 triples through the FP32 kernels, kept for the CPU leg's oracle check
+        n3 = 256
+        X3s = [(x[0][:n3].contiguous(), x[1][:n3].contiguous()) for x in X3]
+        dTs = dT[..., :n3]
+        V3s = aok.batch_overlap_3c(basis3, *X3s, ctx=ctx)
+        dc3s, da3s = aok.overlap_3c_pb_coef_alpha(dTs, basis3, *X3s, ctx=ctx)
+        h = lambda t: t.double().cpu().numpy()
+        subsets["c4a"] = {"pos": [h(x[0]) for x in X3s], "sidx": [x[1].cpu().numpy() for x in X3s], "ls": cb.ls,
+                          "coef": h(coef3), "alpha": h(orb32.Rnl.zeta), "dT": h(dTs), "V32": h(V3s),
+                          "dc32": h(dc3s), "da32": h(da3s)}
         del V, dT, dc64, da64
 
     if not args.no_cpu:
@@ -604,6 +626,33 @@ def main():
         errD = float(np.max(np.abs(gs["D"].double().cpu().numpy() - g_c[1])) / np.max(np.abs(g_c[1])))
         out.setdefault("fp32_error", {})["c1_vs_oracle_subset"] = {
             "pairs": nb_mt, "S_blockwise": errS, "dzeta": errz, "dD": errD}
+
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+        if "ao" in subsets:  # AO C4b: first 20000 points, FP32 GPU against the FP64 oracle
+            a = subsets["ao"]
+            c4o = a["basis"]
+            _, iR4, iY4 = c4o.ao_maps()
+            z4, D4 = r32(c4o.zeta), r32(c4o.D)
+            poly4 = np.zeros(c4o.ns, np.int64)
+            Pr, dPr = O.ao_eval(a["X"], a["sidx"], z4, D4, poly4, np.array(iR4), np.array(iY4), 3, grad=True)
+            dXr, dzr, dDr = O.ao_pullback(a["X"], a["sidx"], z4, D4, poly4, np.array(iR4), np.array(iY4), 3, a["dP"])
+            out["fp32_error"]["ao_c4b_vs_oracle_subset"] = {
+                "points": len(a["sidx"]), "P": rel(a["P32"], Pr),
+                "dP": max(rel(a["dP32"][:, :, c], dPr[:, :, c]) for c in range(3)),
+                "dX": rel(a["dX32"], dXr), "dzeta": rel(a["dz32"], dzr), "dD": rel(a["dD32"], dDr)}
+        if "c4a" in subsets:  # 3c C4a: first 256 triples, FP32 GPU against the FP64 oracle
+            t = subsets["c4a"]
+            pa, pb, pc = t["pos"]
+            sa_, sb_, sc_ = t["sidx"]
+            Vr = O.ovl3c(pa, pb, pc, sa_, sb_, sc_, t["ls"], t["coef"], t["alpha"])
+            dcr, dar = O.ovl3c_pb(t["dT"], pa, pb, pc, sa_, sb_, sc_, t["ls"], t["coef"], t["alpha"])
+            num = np.max(np.abs(t["V32"] - Vr), axis=(0, 1, 2))
+            den = np.maximum(np.max(np.abs(Vr), axis=(0, 1, 2)), 1e-300)
+            out["fp32_error"]["c4a_vs_oracle_subset"] = {
+                "triples": len(sa_), "V_blockwise": float(np.max(num / den)),
+                "dcoef": rel(t["dc32"], dcr), "dalpha": rel(t["da32"], dar)}
     print(json.dumps(out), flush=True)
 
 
