@@ -699,7 +699,7 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                                            const Geo3 &geo, const T (&ab)[3],
                                            const T (&ac)[3], const T *__restrict__ g, double *__restrict__ rA,
                                            double *__restrict__ rB, double *__restrict__ rC, int64_t rs,
-                                           double *__restrict__ cacc, double *__restrict__ cval)
+                                           double *__restrict__ cacc, double *__restrict__ cval, T *__restrict__ gls)
 {
     constexpr int NBA = Sh3<LA>::nbf, NBB = Sh3<LB>::nbf;
     constexpr int N1 = 2 * LA + 1, N2 = 2 * LB + 1;
@@ -711,16 +711,28 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     constexpr int NBC = Sh3<LC>::nbf, NG = NBA * NBB * NBC;
     constexpr bool GREG = NG <= 27;
     T gr[GREG ? NG : 1];
-    if constexpr (GREG) {
+    // larger blocks: staged once per shell triple into the lane's LDS column (element e at
+    // gls[64 e]) instead of re-read from global memory per primitive triple, where 64 lanes'
+    // blocks 13.5 KB apart do not stay in the caches (C4a two-d triples 769 -> 255 us; the
+    // (d, p, p) triples, the only one-d blocks above 27 values, run in their own launch so
+    // the other one-d triples keep the occupancy the staging area would cost)
+    // (FP64 (d, d, d) blocks stay in global memory: 110 KB of staging plus the 2 KB x K
+    // accumulator columns would not fit the 160 KB LDS for K > 24)
+    constexpr bool GLDS = !GREG && (std::is_same<T, float>::value || NG <= 108);
+    if constexpr (GREG || GLDS) {
 #pragma unroll
         for (int kk = 0; kk < NBC; ++kk)
 #pragma unroll
             for (int jj = 0; jj < NBB; ++jj)
 #pragma unroll
-                for (int ii = 0; ii < NBA; ++ii)
-                    gr[ii + NBA * (jj + NBB * kk)] =
-                        g[(sh.off[sa] + ii) +
-                          static_cast<int64_t>(N) * ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
+                for (int ii = 0; ii < NBA; ++ii) {
+                    const T v = g[(sh.off[sa] + ii) +
+                                  static_cast<int64_t>(N) * ((sh.off[sb] + jj) + static_cast<int64_t>(N) * (sh.off[sc] + kk))];
+                    if constexpr (GREG)
+                        gr[ii + NBA * (jj + NBB * kk)] = v;
+                    else
+                        gls[64 * (ii + NBA * (jj + NBB * kk))] = v;
+                }
     }
     // bra sums over (j, k) and ket-B sums over k in registers, ket-C sums over (i, j) in the
     // lane's LDS column (cacc[(2 k + c) * 64]); one row update each
@@ -793,6 +805,8 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                                         T gv;
                                         if constexpr (GREG)
                                             gv = gr[ii + NBA * (jj + NBB * kk)];
+                                        else if constexpr (GLDS)
+                                            gv = gls[64 * (ii + NBA * (jj + NBB * kk))];
                                         else
                                             gv = g[(sh.off[sa] + ii) +
                                                    static_cast<int64_t>(N) *
@@ -838,17 +852,27 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     for (int k = 0; k < 2 * nlc; ++k) rC[k * rs] += cacc[64 * k];
 }
 
-// CLS >= 0: only the shell triples with CLS d shells (one instance per class, so that
-// each is compiled for the registers of its own cases; CLS 0 clears the rows and runs
-// first), CLS < 0: every triple.
+// Class of a shell triple (shells up to d): its number of d shells, except that the
+// one-d triples whose cotangent block exceeds 27 values (d, p, p) form class 4, so that
+// only their launch carries the LDS staging area.
+__host__ __device__ constexpr int triple_class(int la, int lb, int lc)
+{
+    const int nd = (la == 2) + (lb == 2) + (lc == 2);
+    const int ng = (la + 1) * (la + 2) / 2 * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2);
+    return nd == 1 && ng > 27 ? 4 : nd;
+}
+
+// CLS >= 0: only the shell triples of class CLS (one instance per class, so that each is
+// compiled for the registers of its own cases; CLS 0 clears the rows and runs first),
+// CLS < 0: every triple.
 // Waves per SIMD the FP32 instances are compiled for: 2 for the triples with no d shell
-// (222 VGPRs) and with one (256 VGPRs and an 88-byte spill instead of 288 registers at
-// 1 wave: C4a rows 4.58 -> 4.32 ms); forcing 3 on the first (212-byte spill) or 2 on the
-// two-d class measured slower.  Other instances keep the compiler's choice.
+// (224 VGPRs), with one d (189) and for the (d, p, p) class (248); forcing 3 on the
+// no-d class (212-byte spill) or the one-d class (84-byte spill: 852 -> 967 us), or 2 on
+// the two-d class measured slower.  Other instances keep the compiler's choice.
 template <typename T, int LMAXB, int CLS>
 constexpr int ovl3_pb_waves()
 {
-    return std::is_same<T, float>::value && LMAXB == 2 && (CLS == 0 || CLS == 1) ? 2 : 1;
+    return std::is_same<T, float>::value && LMAXB == 2 && (CLS == 0 || CLS == 1 || CLS == 4) ? 2 : 1;
 }
 
 template <typename T, int LMAXB, int CLS>
@@ -900,8 +924,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_pb_wave
         ac[ax] = static_cast<T>(dac);
     }
     const T *g = dT + static_cast<int64_t>(N) * N * N * b;
-    extern __shared__ double cacc_all[];  // [2 K][64] sums, then [2 K][64] C-list values
+    extern __shared__ double cacc_all[];  // [2 K][64] sums, [2 K][64] C-list values, [NG][64] T cotangents
     double *cacc = cacc_all + threadIdx.x, *cval = cacc_all + 64 * 2 * K + threadIdx.x;
+    T *gls = reinterpret_cast<T *>(cacc_all + 64 * 4 * K) + threadIdx.x;
     const int sc0 = splitC ? pc : 0, sc1 = splitC ? pc + 1 : ns;
     const int sb0 = splitB ? pb : 0, sb1 = splitB ? pb + 1 : ns;
     const int64_t K2 = 2 * static_cast<int64_t>(K);
@@ -910,16 +935,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_pb_wave
             for (int sa = 0; sa < ns; ++sa) {
                 const int la = sh.ls[sa], lb = sh.ls[sb], lc = sh.ls[sc];
                 if constexpr (CLS >= 0)
-                    if ((la == 2) + (lb == 2) + (lc == 2) != CLS) continue;
+                    if (triple_class(la, lb, lc) != CLS) continue;
                 double *rA = row + K2 * sa * Bc;
                 double *rB = row + K2 * (ns + (splitB ? 0 : sb)) * Bc;
                 double *rC = row + K2 * (ns + nB + (splitC ? 0 : sc)) * Bc;
 #define AOK_B3_CASE(A_, B_, C_)                                                                                  \
     if constexpr (A_ <= LMAXB && B_ <= LMAXB && C_ <= LMAXB &&                                                   \
-                  (CLS < 0 || (A_ == 2) + (B_ == 2) + (C_ == 2) == CLS))                                         \
+                  (CLS < 0 || triple_class(A_, B_, C_) == CLS))                                                  \
         if (la == A_ && lb == B_ && lc == C_) {                                                                  \
             triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, lcoef, lalpha, nreal, nlist, geo, ab, ac, g, rA, rB,  \
-                                      rC, Bc, cacc, cval);                                                        \
+                                      rC, Bc, cacc, cval, gls);                                                   \
             continue;                                                                                            \
         }
                 AOK_L3_CASES(AOK_B3_CASE)
@@ -1163,24 +1188,41 @@ static aok_status bwd3_launch(aok_ctx ctx, const Ovl3cCall &c, const OvlShape &s
     for (int64_t b0 = 0; b0 < c.B; b0 += Bc) {
         const int64_t n = std::min<int64_t>(Bc, c.B - b0);
         time_begin(ctx, "ovl3c_bwd_rows");
-        auto rows_launch = [&](auto kern) {
+        // ngl: the largest cotangent block the instance stages in LDS (more than 27 values;
+        // K <= 32, so the total stays within 160 KB)
+        auto rows_launch = [&](auto kern, int ngl) -> aok_status {
+            const size_t shm = sizeof(double) * 64 * 4 * sh.K + sizeof(T) * 64 * ngl;
+            if (shm > 64 * 1024)
+                AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)));
             hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((n + 63) / 64), static_cast<unsigned>(nparts)),
-                               dim3(64), sizeof(double) * 64 * 4 * sh.K, ctx->stream, sh, b0, n,
+                               dim3(64), shm, ctx->stream, sh, b0, n,
                                static_cast<const T *>(c.dT), static_cast<const T *>(c.posA),
                                static_cast<const T *>(c.posB), static_cast<const T *>(c.posC), static_cast<T>(c.scale),
                                c.sidxA, c.sidxB, c.sidxC, c.itype, reinterpret_cast<const double *>(ws + o_lc),
                                reinterpret_cast<const double *>(ws + o_la), reinterpret_cast<const int *>(ws + o_nr),
                                reinterpret_cast<const int *>(ws + o_nl), splitB, splitC,
                                reinterpret_cast<double *>(ws + o_rows), ctx->d_err);
+            return AOK_OK;
         };
         // with d shells: one launch per number of d shells in the triple (0 first: it clears)
         if constexpr (LMAXB == 2) {
-            rows_launch(&ovl3c_bwd_rows<T, 2, 0>);
-            rows_launch(&ovl3c_bwd_rows<T, 2, 1>);
-            rows_launch(&ovl3c_bwd_rows<T, 2, 2>);
-            rows_launch(&ovl3c_bwd_rows<T, 2, 3>);
+            const int ngl3 = std::is_same<T, float>::value ? 216 : 0;
+            aok_status rs = rows_launch(&ovl3c_bwd_rows<T, 2, 0>, 0);
+            if (rs == AOK_OK) rs = rows_launch(&ovl3c_bwd_rows<T, 2, 1>, 0);
+            if (rs == AOK_OK) rs = rows_launch(&ovl3c_bwd_rows<T, 2, 4>, 54);
+            if (rs == AOK_OK) rs = rows_launch(&ovl3c_bwd_rows<T, 2, 2>, 108);
+            if (rs == AOK_OK) rs = rows_launch(&ovl3c_bwd_rows<T, 2, 3>, ngl3);
+            if (rs != AOK_OK) {
+                time_end(ctx);
+                return rs;
+            }
         } else {
-            rows_launch(&ovl3c_bwd_rows<T, LMAXB, -1>);
+            const aok_status rs = rows_launch(&ovl3c_bwd_rows<T, LMAXB, -1>, 0);  // l <= 1: blocks in registers
+            if (rs != AOK_OK) {
+                time_end(ctx);
+                return rs;
+            }
         }
         time_end(ctx);
         AOK_HIP(ctx, hipGetLastError());
