@@ -699,7 +699,8 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                                            const Geo3 &geo, const T (&ab)[3],
                                            const T (&ac)[3], const T *__restrict__ g, double *__restrict__ rA,
                                            double *__restrict__ rB, double *__restrict__ rC, int64_t rs,
-                                           double *__restrict__ cacc, double *__restrict__ cval, T *__restrict__ gls)
+                                           double *__restrict__ cacc, double *__restrict__ cval, float *__restrict__ caccf,
+                                           float *__restrict__ cvalf, T *__restrict__ gls)
 {
     constexpr int NBA = Sh3<LA>::nbf, NBB = Sh3<LB>::nbf;
     constexpr int N1 = 2 * LA + 1, N2 = 2 * LB + 1;
@@ -719,6 +720,12 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     // (FP64 (d, d, d) blocks stay in global memory: 110 KB of staging plus the 2 KB x K
     // accumulator columns would not fit the 160 KB LDS for K > 24)
     constexpr bool GLDS = !GREG && (std::is_same<T, float>::value || NG <= 108);
+    // FP32 instance: prefactor, exponential and per-k sums in FP32 (C4a rows 3.56 -> 3.27 ms),
+    // and the C-side sums and C-list values in LDS in FP32 (half the LDS traffic of FP64
+    // ones: 3.27 -> 3.17 ms); FP32 error against FP64 9.8e-8 -> 1.2e-7 of the largest
+    // cotangent.  The (i, j) sums and the rows stay FP64.
+    constexpr bool F32P = std::is_same<T, float>::value;
+    constexpr bool F32L = F32P;
     if constexpr (GREG || GLDS) {
 #pragma unroll
         for (int kk = 0; kk < NBC; ++kk)
@@ -738,10 +745,17 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
     // lane's LDS column (cacc[(2 k + c) * 64]); one row update each
     // the C shell's (coef, alpha) list in the lane's LDS column too (cval[(2 k + c) * 64])
     for (int k = 0; k < nlc; ++k) {
-        cacc[64 * (2 * k)] = 0.0;
-        cacc[64 * (2 * k + 1)] = 0.0;
-        cval[64 * (2 * k)] = lcoef[static_cast<int64_t>(ic) * K + k];
-        cval[64 * (2 * k + 1)] = lalpha[static_cast<int64_t>(ic) * K + k];
+        if constexpr (F32L) {
+            caccf[64 * (2 * k)] = 0.f;
+            caccf[64 * (2 * k + 1)] = 0.f;
+            cvalf[64 * (2 * k)] = static_cast<float>(lcoef[static_cast<int64_t>(ic) * K + k]);
+            cvalf[64 * (2 * k + 1)] = static_cast<float>(lalpha[static_cast<int64_t>(ic) * K + k]);
+        } else {
+            cacc[64 * (2 * k)] = 0.0;
+            cacc[64 * (2 * k + 1)] = 0.0;
+            cval[64 * (2 * k)] = lcoef[static_cast<int64_t>(ic) * K + k];
+            cval[64 * (2 * k + 1)] = lalpha[static_cast<int64_t>(ic) * K + k];
+        }
     }
     for (int i = 0; i < nla; ++i) {
         double sA0 = 0.0, sA1 = 0.0;
@@ -751,16 +765,50 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
             const double cb = lcoef[static_cast<int64_t>(ib) * K + j], be = lalpha[static_cast<int64_t>(ib) * K + j];
             const double p = al + be, ip = 1.0 / p, eab = al * be * geo.rab;
             const T aop = static_cast<T>(al * ip), bop = static_cast<T>(be * ip);
+            // FP32 instance: the per-(i, j) pieces of the exponent argument (times log2 e) and
+            // per-k FP32 sums t0 = sum_k cc w H, ta = sum_k cc w Ha, tb = sum_k cc w Hb
+            [[maybe_unused]] float pf = 0.f, E0 = 0.f, E1 = 0.f, t0 = 0.f, ta = 0.f, tb = 0.f;
+            const double cab = ca * cb;
+            [[maybe_unused]] const float cabf = static_cast<float>(cab);
+            if constexpr (F32P) {
+                pf = static_cast<float>(p);
+                E0 = static_cast<float>(eab * 1.4426950408889634074);
+                E1 = static_cast<float>((al * geo.rac + be * geo.rbc) * 1.4426950408889634074);
+            }
             for (int k = 0; k < nlc; ++k) {
                 if ((i >= nra) + (j >= nrb) + (k >= nrc) > 1) continue;
-                const double cc = cval[64 * (2 * k)], ga = cval[64 * (2 * k + 1)];
-                double w0;
-                const double iq = inv_q<T>(p + ga, w0);
-                // exp(-(ab rab + ag rac + bg rbc) / q): the argument in FP64, the exponential
-                // in the kernel precision (FP32: range reduction in FP64, then v_exp_f32)
-                const double w = w0 * exp_neg3<T>((eab + ga * (al * geo.rac + be * geo.rbc)) * iq);
-                const T gq = static_cast<T>(ga * iq);
-                const T pq = static_cast<T>(p * iq), i2q = static_cast<T>(0.5 * iq);
+                double cc, ga;
+                if constexpr (F32L) {
+                    cc = cvalf[64 * (2 * k)];
+                    ga = cvalf[64 * (2 * k + 1)];
+                } else {
+                    cc = cval[64 * (2 * k)];
+                    ga = cval[64 * (2 * k + 1)];
+                }
+                T gq, pq, i2q;
+                [[maybe_unused]] double w = 0.0;
+                [[maybe_unused]] float wf = 0.f;
+                if constexpr (F32P) {
+                    // pullback terms are summed over the batch and judged against the largest
+                    // cotangent, so 1/q, q^{-3/2} and exp(-(ab rab + ag rac + bg rbc)/q) in FP32
+                    // suffice here (the forward keeps its FP64 argument for per-block accuracy)
+                    const float gaf = static_cast<float>(ga);
+                    const float iqf = __builtin_amdgcn_rcpf(pf + gaf);
+                    wf = 5.568327997f * iqf * __builtin_sqrtf(iqf) *
+                         __builtin_amdgcn_exp2f(-(fmaf(gaf, E1, E0) * iqf));
+                    gq = gaf * iqf;
+                    pq = pf * iqf;
+                    i2q = 0.5f * iqf;
+                } else {
+                    double w0;
+                    const double iq = inv_q<T>(p + ga, w0);
+                    // exp(-(ab rab + ag rac + bg rbc) / q): the argument in FP64, the
+                    // exponential in the kernel precision
+                    w = w0 * exp_neg3<T>((eab + ga * (al * geo.rac + be * geo.rbc)) * iq);
+                    gq = static_cast<T>(ga * iq);
+                    pq = static_cast<T>(p * iq);
+                    i2q = static_cast<T>(0.5 * iq);
+                }
                 T sx[LA + 3][LB + 3][LC + 3], sy[LA + 3][LB + 3][LC + 3], sz[LA + 3][LB + 3][LC + 3];
                 {
                     const T PA = -bop * ab[0], PB = aop * ab[0], PC = PA + ac[0];
@@ -834,14 +882,34 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
                     }
                     base1 += LA - ll1 + 1;
                 }
-                const double wH = w * static_cast<double>(H);
-                const double cw = ca * cb * cc * w;
-                sA0 += cb * cc * wH;
-                sB0 += ca * cc * wH;
-                cacc[64 * (2 * k)] += ca * cb * wH;
-                sA1 -= cw * static_cast<double>(Ha);
-                sB1 -= cw * static_cast<double>(Hb);
-                cacc[64 * (2 * k + 1)] -= cw * static_cast<double>(Hc);
+                if constexpr (F32P) {
+                    const float wc = static_cast<float>(cc) * wf;
+                    t0 = fmaf(wc, H, t0);
+                    ta = fmaf(wc, Ha, ta);
+                    tb = fmaf(wc, Hb, tb);
+                    if constexpr (F32L) {
+                        caccf[64 * (2 * k)] = fmaf(cabf, wf * H, caccf[64 * (2 * k)]);
+                        caccf[64 * (2 * k + 1)] = fmaf(-cabf, wc * Hc, caccf[64 * (2 * k + 1)]);
+                    } else {
+                        cacc[64 * (2 * k)] += cab * static_cast<double>(wf * H);
+                        cacc[64 * (2 * k + 1)] -= cab * static_cast<double>(wc * Hc);
+                    }
+                } else {
+                    const double wH = w * static_cast<double>(H);
+                    const double cw = cab * cc * w;
+                    sA0 += cb * cc * wH;
+                    sB0 += ca * cc * wH;
+                    cacc[64 * (2 * k)] += cab * wH;
+                    sA1 -= cw * static_cast<double>(Ha);
+                    sB1 -= cw * static_cast<double>(Hb);
+                    cacc[64 * (2 * k + 1)] -= cw * static_cast<double>(Hc);
+                }
+            }
+            if constexpr (F32P) {
+                sA0 += cb * static_cast<double>(t0);
+                sB0 += ca * static_cast<double>(t0);
+                sA1 -= cab * static_cast<double>(ta);
+                sB1 -= cab * static_cast<double>(tb);
             }
             rB[(2 * j) * rs] += sB0;
             rB[(2 * j + 1) * rs] += sB1;
@@ -849,7 +917,7 @@ __device__ __forceinline__ void triple_bwd(const OvlShape &sh, int sa, int sb, i
         rA[(2 * i) * rs] += sA0;
         rA[(2 * i + 1) * rs] += sA1;
     }
-    for (int k = 0; k < 2 * nlc; ++k) rC[k * rs] += cacc[64 * k];
+    for (int k = 0; k < 2 * nlc; ++k) rC[k * rs] += F32L ? static_cast<double>(caccf[64 * k]) : cacc[64 * k];
 }
 
 // Class of a shell triple (shells up to d): its number of d shells, except that the
@@ -926,6 +994,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_pb_wave
     const T *g = dT + static_cast<int64_t>(N) * N * N * b;
     extern __shared__ double cacc_all[];  // [2 K][64] sums, [2 K][64] C-list values, [NG][64] T cotangents
     double *cacc = cacc_all + threadIdx.x, *cval = cacc_all + 64 * 2 * K + threadIdx.x;
+    float *caccf = reinterpret_cast<float *>(cacc_all) + threadIdx.x;  // FP32 layout of the same area
+    float *cvalf = reinterpret_cast<float *>(cacc_all) + 64 * 2 * K + threadIdx.x;
     T *gls = reinterpret_cast<T *>(cacc_all + 64 * 4 * K) + threadIdx.x;
     const int sc0 = splitC ? pc : 0, sc1 = splitC ? pc + 1 : ns;
     const int sb0 = splitB ? pb : 0, sb1 = splitB ? pb + 1 : ns;
@@ -944,7 +1014,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ovl3_pb_wave
                   (CLS < 0 || triple_class(A_, B_, C_) == CLS))                                                  \
         if (la == A_ && lb == B_ && lc == C_) {                                                                  \
             triple_bwd<T, A_, B_, C_>(sh, sa, sb, sc, sg, lcoef, lalpha, nreal, nlist, geo, ab, ac, g, rA, rB,  \
-                                      rC, Bc, cacc, cval, gls);                                                   \
+                                      rC, Bc, cacc, cval, caccf, cvalf, gls);                                     \
             continue;                                                                                            \
         }
                 AOK_L3_CASES(AOK_B3_CASE)
