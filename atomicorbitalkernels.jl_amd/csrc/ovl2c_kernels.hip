@@ -716,6 +716,36 @@ struct StripeWalk {
     }
 };
 
+// Stage the lanes' cotangent stripe (rows row0.., the chunk's nbb ket columns) into LDS
+// (zeros for inactive lanes), STAGE_BATCH elements per lane in flight at a time: one load
+// at a time paid the full memory latency per element (C2, B = 2^18: the l <= 3 pullback
+// instance spent 3.7 of its 5.0 ms staging; batches of 8: pullback 6.12 -> 4.48 ms,
+// batches of 16: 4.64 ms).
+constexpr int STAGE_BATCH = 8;
+template <typename T>
+__device__ __forceinline__ void stage_stripe(T *stripe, int str, const T *__restrict__ dS, const int *bl_b,
+                                             int64_t NN, int64_t colbase, int N, int nrows, int nbb, int nact,
+                                             int lane)
+{
+    const int total = OVL_TILE * nrows * nbb;
+    StripeWalk w(lane, nrows, nbb);
+    constexpr int SB = STAGE_BATCH;
+    for (int e0 = lane; e0 < total; e0 += OVL_TILE * SB) {
+        T v[SB];
+        int dst[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const bool in = e0 + OVL_TILE * u < total;
+            dst[u] = w.bl * str + w.j * nrows + w.i;
+            v[u] = (in && w.bl < nact) ? dS[static_cast<int64_t>(bl_b[w.bl]) * NN + colbase + N * w.j + w.i] : T(0);
+            w.next(nrows, nbb);
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+            if (e0 + OVL_TILE * u < total) stripe[dst[u]] = v[u];
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Tile bookkeeping shared by the fast kernels
 // ---------------------------------------------------------------------------------
@@ -830,8 +860,21 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
 // ---------------------------------------------------------------------------------
 // Fast pullback
 // ---------------------------------------------------------------------------------
+// FP32 instances that stage the cotangent stripe in LDS copy each shell pair's block of at
+// least BWD_GREG_MIN values into registers before its primitive loop (instead of LDS reads
+// per primitive pair), and the l <= 3 instance is compiled for one wave per SIMD (the
+// registers to hold an f-f block): C2 pullback 4.48 -> 4.22 ms.  Thresholds 20-60 and two
+// waves per SIMD measured within 1%.
+constexpr int BWD_GREG_MIN = 30;
 template <typename T, int LMAXB>
-__global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
+constexpr int bwd_max_waves()
+{
+    return std::is_same<T, float>::value && LMAXB == 3 ? 1 : 8;
+}
+
+template <typename T, int LMAXB>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, bwd_max_waves<T, LMAXB>())))
+void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
                                                      const T *__restrict__ posA, const T *__restrict__ posB, T scale,
                                                      const int *__restrict__ perm, const int *__restrict__ tile_key,
                                                      const int *__restrict__ boff, const int *__restrict__ bcnt,
@@ -873,14 +916,10 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
     const OvlChunk ch = unpack_chunk(sh.ch[chunk]);
     const int sb = ch.sb, lb = sh.ls[sb];
     const int nbb = nbf_of(lb), nrows = ch.nrows;
-    const int len = nrows * nbb, total = OVL_TILE * len;
     const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
     // stage the cotangent stripe (zeros for inactive lanes)
     if constexpr (!DIRECT) {
-        StripeWalk w(lane, nrows, nbb);
-        for (int e = lane; e < total; e += OVL_TILE, w.next(nrows, nbb))
-            stripe[w.bl * sh.str + w.j * nrows + w.i] =
-                w.bl < ti.nact ? dS[static_cast<int64_t>(bl_b[w.bl]) * NN + colbase + N * w.j + w.i] : T(0);
+        stage_stripe(stripe, sh.str, dS, bl_b, NN, colbase, N, nrows, nbb, ti.nact, lane);
     }
     __syncthreads();
     const bool active = lane < ti.nact;
@@ -903,6 +942,13 @@ __global__ __launch_bounds__(64) void ovl2c_bwd_fast(const OvlShape sh, const T 
                 _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
                     _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
                         gr.v[i + ShellC<A>::nbf * j] = active ? src[i + static_cast<int64_t>(N) * j] : T(0); \
+                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
+            } else if constexpr (std::is_same<T, float>::value &&                                  \
+                                 ShellC<A>::nbf * ShellC<B_>::nbf >= BWD_GREG_MIN) {                \
+                GRegs<T, ShellC<A>::nbf, ShellC<B_>::nbf> gr;                                      \
+                _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
+                    _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
+                        gr.v[i + ShellC<A>::nbf * j] = gsl[i + nrows * j];                         \
                 pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
             } else {                                                                               \
                 pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, GStripe<T>{gsl, nrows}, aA, aB, \
@@ -961,13 +1007,9 @@ __global__ __launch_bounds__(64) void ovl2c_dx_fast(const OvlShape sh, int64_t B
     const OvlChunk ch = unpack_chunk(sh.ch[chunk]);
     const int sb = ch.sb, lb = sh.ls[sb];
     const int nbb = nbf_of(lb), nrows = ch.nrows;
-    const int len = nrows * nbb, total = OVL_TILE * len;
     const int64_t colbase = static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
     if constexpr (!DIRECT) {
-        StripeWalk w(lane, nrows, nbb);
-        for (int e = lane; e < total; e += OVL_TILE, w.next(nrows, nbb))
-            stripe[w.bl * sh.str + w.j * nrows + w.i] =
-                w.bl < ti.nact ? dS[static_cast<int64_t>(bl_b[w.bl]) * NN + colbase + N * w.j + w.i] : T(0);
+        stage_stripe(stripe, sh.str, dS, bl_b, NN, colbase, N, nrows, nbb, ti.nact, lane);
     }
     __syncthreads();
     const bool active = lane < ti.nact;
