@@ -222,6 +222,15 @@ __device__ __forceinline__ void make_record(double a, double b, double ca, doubl
     r.w1 = static_cast<T>(cb * pref);
     r.w2 = static_cast<T>(ca * pref);
 }
+template <typename T>
+__device__ __forceinline__ void make_pb_record(double a, double b, PairPb<T> &r)
+{
+    const double p = a + b, d = a - b;
+    r.ca = static_cast<T>(0.25 * d / (a * p));
+    r.cb = static_cast<T>(-0.25 * d / (b * p));
+    r.sa = static_cast<T>(0.5 * b / (a * p));
+    r.sb = static_cast<T>(0.5 * a / (b * p));
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void ovl_fill_kernel(const OvlShape sh, const void *__restrict__ sidxA,
@@ -232,7 +241,8 @@ __global__ __launch_bounds__(256) void ovl_fill_kernel(const OvlShape sh, const 
                                                        int maxtiles, const double *__restrict__ coef64,
                                                        const double *__restrict__ alpha64,
                                                        const int *__restrict__ nlist, const int *__restrict__ lidx,
-                                                       typename RecOf<T>::type *__restrict__ rec)
+                                                       typename RecOf<T>::type *__restrict__ rec,
+                                                       PairPb<T> *__restrict__ rec2)
 {
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, NK = NZ * NZ;
     const int bid = blockIdx.x;
@@ -276,6 +286,11 @@ __global__ __launch_bounds__(256) void ovl_fill_kernel(const OvlShape sh, const 
     typename RecOf<T>::type r;
     make_record<T>(alpha64[qa], alpha64[qb], coef64[qa], coef64[qb], r);
     rec[gid] = r;
+    if (rec2) {
+        PairPb<T> r2;
+        make_pb_record<T>(alpha64[qa], alpha64[qb], r2);
+        rec2[gid] = r2;
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -391,7 +406,16 @@ __device__ __forceinline__ void pair_fwd(const typename RecOf<T>::type *__restri
 
 // Contraction of one primitive pair against the aliased cotangent block g (LDS stripe
 // slice of the lane, column stride nrows).  H = sum g s_x s_y s_z; with RAISED also the
-// bra- and ket-raised sums Ha, Hb (d/dalpha x^l e^{-alpha x^2} = -x^{l+2} e^{-alpha x^2}).
+// exponent sums Ha, Hb of the bra and the ket.  d/dalpha x^i e^{-alpha x^2} = -x^{i+2} e^{..}
+// raises a 1-D factor by two; the sums here take each raised factor minus (2i+1)/(4 alpha)
+// times itself, u(i, j) = s(i+2, j) - (2i+1)/(4 alpha) s(i, j) per axis, which over the
+// three axes removes n_a H with n_a = (2 l_a + 3)/(4 alpha) (the logarithmic derivative of
+// a primitive's L2 norm).  By the vertical recursion
+//   u(i, j) = PA s(i+1, j) + (alpha - (2i+1) beta)/(4 alpha p) s(i, j) + j/(2p) s(i+1, j-1),
+// so the n_a H that the raised moment nearly cancels (diffuse primitives: the raised moment
+// and n_a H agree to 5-15x in C1's p shells) never appears as a difference of rounded
+// values; the reduction adds n_a coef_a dcoef_a back in FP64 (ovl2c_bwd_reduce).  The ket
+// side likewise with v(i, j) = s(i, j+2) - (2j+1)/(4 beta) s(i, j).
 // Cotangent accessors for pair_contract: the lane's slice of the LDS stripe (column stride
 // nrows), or the shell pair's block held in registers (column stride NBA).
 template <typename T>
@@ -413,34 +437,105 @@ struct GRegs {
 #ifndef AOK_FACTOR_L
 #define AOK_FACTOR_L 3
 #endif
+// Exponent tables of one axis from the base table s(i, j), i <= LA, j <= LB, extended
+// here by the one row s(LA+1, j) (bra) or column s(i, LB+1) (ket) they need: the bra's
+// u(i, j) (KET = false; c[i] = (alpha - (2i+1) beta)/(4 alpha p)) or the ket's v(i, j)
+// (KET = true; c[j] = (beta - (2j+1) alpha)/(4 beta p)), i <= LA, j <= LB.
+template <typename T, int LA, int LB, bool KET>
+__device__ __forceinline__ void os1d_exp(T PA, T PB, T i2p, const T (&c)[(KET ? LB : LA) + 1],
+                                         const T (&s)[LA + 1][LB + 1], T (&u)[LA + 1][LB + 1])
+{
+    if constexpr (!KET) {
+        T t[LB + 1];  // s(LA+1, j)
+        t[0] = PA * s[LA][0];
+        if (LA > 0) t[0] = fma(T(LA) * i2p, s[LA > 0 ? LA - 1 : 0][0], t[0]);
+#pragma unroll
+        for (int j = 1; j <= LB; ++j) {
+            T v = fma(PB, t[j - 1], T(LA + 1) * i2p * s[LA][j - 1]);
+            if (j > 1) v = fma(T(j - 1) * i2p, t[j > 1 ? j - 2 : 0], v);
+            t[j] = v;
+        }
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB; ++j) {
+                const T up = i < LA ? s[i < LA ? i + 1 : 0][j] : t[j];
+                T x = fma(PA, up, c[i] * s[i][j]);
+                if (j > 0) x = fma(T(j) * i2p, i < LA ? s[i < LA ? i + 1 : 0][j > 0 ? j - 1 : 0] : t[j > 0 ? j - 1 : 0], x);
+                u[i][j] = x;
+            }
+    } else {
+        T t[LA + 1];  // s(i, LB+1)
+#pragma unroll
+        for (int i = 0; i <= LA; ++i) {
+            T v = PB * s[i][LB];
+            if (i > 0) v = fma(T(i) * i2p, s[i > 0 ? i - 1 : 0][LB], v);
+            if (LB > 0) v = fma(T(LB) * i2p, s[i][LB > 0 ? LB - 1 : 0], v);
+            t[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+            for (int j = 0; j <= LB; ++j) {
+                const T up = j < LB ? s[i][j < LB ? j + 1 : 0] : t[i];
+                T x = fma(PB, up, c[j] * s[i][j]);
+                if (i > 0) x = fma(T(i) * i2p, j < LB ? s[i > 0 ? i - 1 : 0][j < LB ? j + 1 : 0] : t[i > 0 ? i - 1 : 0], x);
+                u[i][j] = x;
+            }
+    }
+}
+// The exponent coefficients c[i] = c0 - i s (PairPb).
+template <typename T, int L>
+__device__ __forceinline__ void exp_coefs(T c0, T st, T (&c)[L + 1])
+{
+    c[0] = c0;
+#pragma unroll
+    for (int i = 1; i <= L; ++i) c[i] = fma(T(-i), st, c0);
+}
+
 template <typename T, int LA, int LB, bool RAISED, typename G>
-__device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, const T (&ab)[3], T r2, const G &g,
-                                              T &H, T &Ha, T &Hb)
+__device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, const PairPb<T> &q, const T (&ab)[3],
+                                              T r2, const G &g, T &H, T &Ha, T &Hb)
 {
     if constexpr (LA == 0 && LB == 0) {
-        // s-s: s_x s_y s_z = 1 and the raised sums close over the axes,
-        // sum_ax s_ax(2, 0) = sum_ax (PA_ax^2 + 1/(2p)) = (beta/p)^2 |AB|^2 + 3/(2p), likewise
-        // the ket with alpha/p
+        // s-s: s_x s_y s_z = 1 and the exponent sums close over the axes,
+        // sum_ax u_ax(0, 0) = sum_ax (PA_ax^2 + (alpha - beta)/(4 alpha p))
+        //                   = (beta/p)^2 |AB|^2 + 3 (alpha - beta)/(4 alpha p), likewise the ket
         const T gv = g(0, 0);
         H = gv;
-        Ha = RAISED ? gv * (r.bop * r.bop * r2 + T(3) * r.i2p) : T(0);
-        Hb = RAISED ? gv * (r.aop * r.aop * r2 + T(3) * r.i2p) : T(0);
+        Ha = RAISED ? gv * fma(r.bop * r.bop, r2, T(3) * q.ca) : T(0);
+        Hb = RAISED ? gv * fma(r.aop * r.aop, r2, T(3) * q.cb) : T(0);
         return;
     }
     constexpr int NBA = ShellC<LA>::nbf, N1 = 2 * LA + 1;
-    constexpr int RA = RAISED ? LA + 2 : LA, RB = RAISED ? LB + 2 : LB;
-    T sx[RA + 1][RB + 1], sy[RA + 1][RB + 1], sz[RA + 1][RB + 1];
-    os1d<T, RA, RB>(-r.bop * ab[0], r.aop * ab[0], r.i2p, sx);
-    os1d<T, RA, RB>(-r.bop * ab[1], r.aop * ab[1], r.i2p, sy);
-    os1d<T, RA, RB>(-r.bop * ab[2], r.aop * ab[2], r.i2p, sz);
+    T sx[LA + 1][LB + 1], sy[LA + 1][LB + 1], sz[LA + 1][LB + 1];
+    const T PA[3] = {-r.bop * ab[0], -r.bop * ab[1], -r.bop * ab[2]};
+    const T PB[3] = {r.aop * ab[0], r.aop * ab[1], r.aop * ab[2]};
+    os1d<T, LA, LB>(PA[0], PB[0], r.i2p, sx);
+    os1d<T, LA, LB>(PA[1], PB[1], r.i2p, sy);
+    os1d<T, LA, LB>(PA[2], PB[2], r.i2p, sz);
+    // exponent tables (RAISED only; 1 x 1 and unused otherwise)
+    constexpr int UA = RAISED ? LA + 1 : 1, UB = RAISED ? LB + 1 : 1;
     H = T(0);
     Ha = T(0);
     Hb = T(0);
     if constexpr (LA >= AOK_FACTOR_L || LB >= AOK_FACTOR_L) {
+        T ux[UA][UB], uy[UA][UB], uz[UA][UB], vx[UA][UB], vy[UA][UB], vz[UA][UB];
+        if constexpr (RAISED) {
+            T ca[LA + 1], cb[LB + 1];
+            exp_coefs<T, LA>(q.ca, q.sa, ca);
+            exp_coefs<T, LB>(q.cb, q.sb, cb);
+            os1d_exp<T, LA, LB, false>(PA[0], PB[0], r.i2p, ca, sx, ux);
+            os1d_exp<T, LA, LB, false>(PA[1], PB[1], r.i2p, ca, sy, uy);
+            os1d_exp<T, LA, LB, false>(PA[2], PB[2], r.i2p, ca, sz, uz);
+            os1d_exp<T, LA, LB, true>(PA[0], PB[0], r.i2p, cb, sx, vx);
+            os1d_exp<T, LA, LB, true>(PA[1], PB[1], r.i2p, cb, sy, vy);
+            os1d_exp<T, LA, LB, true>(PA[2], PB[2], r.i2p, cb, sz, vz);
+        }
         // Factored over the x exponents: for each (lx_a, lx_b) the y-z sums
-        //   t = sum g s_y s_z,  ty = sum g s_y' s_z,  tz = sum g s_y s_z'   (' = raised by 2)
+        //   t = sum g s_y s_z,  ty = sum g u_y s_z,  tz = sum g s_y u_z   (u: exponent table)
         // run over the elements sharing those x exponents, then
-        //   H += s_x t,  Ha += s_x' t + s_x (ty + tz)  (bra raised), Hb likewise (ket raised):
+        //   H += s_x t,  Ha += u_x t + s_x (ty + tz)  (bra), Hb likewise with v (ket):
         // 2 mul + 5 fma per element instead of about 20 operations.
         int base1 = 0;
 #pragma unroll
@@ -461,48 +556,72 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
                         t = fma(u, sy[m1][m2], t);
                         if constexpr (RAISED) {
                             const T v = gv * sy[m1][m2];
-                            tya = fma(u, sy[m1 + 2][m2], tya);
-                            tyb = fma(u, sy[m1][m2 + 2], tyb);
-                            tza = fma(v, sz[n1 + 2][n2], tza);
-                            tzb = fma(v, sz[n1][n2 + 2], tzb);
+                            tya = fma(u, uy[m1][m2], tya);
+                            tyb = fma(u, vy[m1][m2], tyb);
+                            tza = fma(v, uz[n1][n2], tza);
+                            tzb = fma(v, vz[n1][n2], tzb);
                         }
                     }
                 }
                 const T x0 = sx[ll1][ll2];
                 H = fma(x0, t, H);
                 if constexpr (RAISED) {
-                    Ha = fma(sx[ll1 + 2][ll2], t, fma(x0, tya + tza, Ha));
-                    Hb = fma(sx[ll1][ll2 + 2], t, fma(x0, tyb + tzb, Hb));
+                    Ha = fma(ux[ll1][ll2], t, fma(x0, tya + tza, Ha));
+                    Hb = fma(vx[ll1][ll2], t, fma(x0, tyb + tzb, Hb));
                 }
                 base2 += LB - ll2 + 1;
             }
             base1 += LA - ll1 + 1;
         }
     } else {
-        int index1 = 0;
+        // two passes over the elements, the bra's exponent tables in the first and the
+        // ket's in the second, so only one side's tables are live at a time
 #pragma unroll
-        for (int ll1 = LA; ll1 >= 0; --ll1)
-#pragma unroll
-            for (int n1 = 0; n1 <= LA - ll1; ++n1) {
-                const int m1 = LA - ll1 - n1;
-                int index2 = 0;
-#pragma unroll
-                for (int ll2 = LB; ll2 >= 0; --ll2)
-#pragma unroll
-                    for (int n2 = 0; n2 <= LB - ll2; ++n2) {
-                        const int m2 = LB - ll2 - n2;
-                        const int lin = index1 + N1 * index2;
-                        const T gv = g(lin % NBA, lin / NBA);
-                        const T x0 = sx[ll1][ll2], y0 = sy[m1][m2], z0 = sz[n1][n2];
-                        H += gv * (x0 * y0 * z0);
-                        if constexpr (RAISED) {
-                            Ha += gv * (sx[ll1 + 2][ll2] * y0 * z0 + x0 * sy[m1 + 2][m2] * z0 + x0 * y0 * sz[n1 + 2][n2]);
-                            Hb += gv * (sx[ll1][ll2 + 2] * y0 * z0 + x0 * sy[m1][m2 + 2] * z0 + x0 * y0 * sz[n1][n2 + 2]);
-                        }
-                        ++index2;
-                    }
-                ++index1;
+        for (int side = 0; side < (RAISED ? 2 : 1); ++side) {
+            T ux[UA][UB], uy[UA][UB], uz[UA][UB];
+            if constexpr (RAISED) {
+                if (side == 0) {
+                    T ca[LA + 1];
+                    exp_coefs<T, LA>(q.ca, q.sa, ca);
+                    os1d_exp<T, LA, LB, false>(PA[0], PB[0], r.i2p, ca, sx, ux);
+                    os1d_exp<T, LA, LB, false>(PA[1], PB[1], r.i2p, ca, sy, uy);
+                    os1d_exp<T, LA, LB, false>(PA[2], PB[2], r.i2p, ca, sz, uz);
+                } else {
+                    T cb[LB + 1];
+                    exp_coefs<T, LB>(q.cb, q.sb, cb);
+                    os1d_exp<T, LA, LB, true>(PA[0], PB[0], r.i2p, cb, sx, ux);
+                    os1d_exp<T, LA, LB, true>(PA[1], PB[1], r.i2p, cb, sy, uy);
+                    os1d_exp<T, LA, LB, true>(PA[2], PB[2], r.i2p, cb, sz, uz);
+                }
             }
+            T E = T(0);
+            int index1 = 0;
+#pragma unroll
+            for (int ll1 = LA; ll1 >= 0; --ll1)
+#pragma unroll
+                for (int n1 = 0; n1 <= LA - ll1; ++n1) {
+                    const int m1 = LA - ll1 - n1;
+                    int index2 = 0;
+#pragma unroll
+                    for (int ll2 = LB; ll2 >= 0; --ll2)
+#pragma unroll
+                        for (int n2 = 0; n2 <= LB - ll2; ++n2) {
+                            const int m2 = LB - ll2 - n2;
+                            const int lin = index1 + N1 * index2;
+                            const T gv = g(lin % NBA, lin / NBA);
+                            const T x0 = sx[ll1][ll2], y0 = sy[m1][m2], z0 = sz[n1][n2];
+                            if (side == 0) H += gv * (x0 * y0 * z0);
+                            if constexpr (RAISED)
+                                E += gv * (ux[ll1][ll2] * y0 * z0 + x0 * uy[m1][m2] * z0 + x0 * y0 * uz[n1][n2]);
+                            ++index2;
+                        }
+                    ++index1;
+                }
+            if (side == 0)
+                Ha = E;
+            else
+                Hb = E;
+        }
     }
 }
 
@@ -516,7 +635,8 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
 constexpr int LROW = OVL_TILE + 1;
 
 template <typename T, int LA, int LB, typename G>
-__device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restrict__ recs, int nra, int nla, int nrb,
+__device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restrict__ recs,
+                                         const PairPb<T> *__restrict__ recs2, int nra, int nla, int nrb,
                                          int nlb, int K, double R2, const T (&ab)[3], const G &g, double *accA,
                                          double *accB, T *lac, int lane)
 {
@@ -543,7 +663,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
                     const R r = recs[(i0 + u) * K + j];
                     const T e = pair_exp_pb(r, R2, r2f);
                     T H, Ha, Hb;
-                    pair_contract<T, LA, LB, true>(r, ab, r2f, g, H, Ha, Hb);
+                    pair_contract<T, LA, LB, true>(r, recs2[(i0 + u) * K + j], ab, r2f, g, H, Ha, Hb);
                     const T eH = e * H;
                     cA[u] += r.w1 * eH;
                     aA[u] -= r.Kab * e * Ha;
@@ -567,7 +687,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             const R r = recs[i * K + j];
             const T e = pair_exp_pb(r, R2, r2f);
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, true>(r, ab, r2f, g, H, Ha, Hb);
+            pair_contract<T, LA, LB, true>(r, recs2[i * K + j], ab, r2f, g, H, Ha, Hb);
             const T eH = e * H;
             cA += r.w1 * eH;
             aA -= r.Kab * e * Ha;
@@ -583,7 +703,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
         for (int j = 0; j < nrb; ++j) {
             const R r = recs[i * K + j];
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, false>(r, ab, r2f, g, H, Ha, Hb);
+            pair_contract<T, LA, LB, false>(r, PairPb<T>{}, ab, r2f, g, H, Ha, Hb);
             cA += r.w1 * pair_exp_pb(r, R2, r2f) * H;
         }
         lac[2 * i * LROW + lane] = cA;
@@ -594,7 +714,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
         for (int i = 0; i < nra; ++i) {
             const R r = recs[i * K + j];
             T H, Ha, Hb;
-            pair_contract<T, LA, LB, false>(r, ab, r2f, g, H, Ha, Hb);
+            pair_contract<T, LA, LB, false>(r, PairPb<T>{}, ab, r2f, g, H, Ha, Hb);
             cB += r.w2 * pair_exp_pb(r, R2, r2f) * H;
         }
         ket[2 * j * LROW + lane] = cB;
@@ -871,15 +991,21 @@ constexpr int bwd_max_waves()
 {
     return std::is_same<T, float>::value && LMAXB == 3 ? 1 : 8;
 }
+template <typename T, int LMAXB>
+constexpr int bwd_min_waves()
+{
+    return std::is_same<T, float>::value && LMAXB == 2 ? 4 : 1;
+}
 
 template <typename T, int LMAXB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, bwd_max_waves<T, LMAXB>())))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(bwd_min_waves<T, LMAXB>(), bwd_max_waves<T, LMAXB>())))
 void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
                                                      const T *__restrict__ posA, const T *__restrict__ posB, T scale,
                                                      const int *__restrict__ perm, const int *__restrict__ tile_key,
                                                      const int *__restrict__ boff, const int *__restrict__ bcnt,
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
+                                                     const PairPb<T> *__restrict__ rec2,
                                                      const int *__restrict__ nreal, const int *__restrict__ nlist,
                                                      double *__restrict__ partial, int chunk0, int nch, int group)
 {
@@ -930,6 +1056,7 @@ void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
         const int la = sh.ls[sa];
         const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
         const typename RecOf<T>::type *recs = rec + g * K * K;
+        const PairPb<T> *recs2 = rec2 + g * K * K;
         const int nra = nreal[sgA * ns + sa], nla = nlist[sgA * ns + sa];
         const T *gsl = mine + (sh.off[sa] - ch.row0);
         double *aA = accA + 2 * (sa - ch.sa0) * K;
@@ -942,16 +1069,16 @@ void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
                 _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
                     _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
                         gr.v[i + ShellC<A>::nbf * j] = active ? src[i + static_cast<int64_t>(N) * j] : T(0); \
-                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
+                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
             } else if constexpr (std::is_same<T, float>::value &&                                  \
                                  ShellC<A>::nbf * ShellC<B_>::nbf >= BWD_GREG_MIN) {                \
                 GRegs<T, ShellC<A>::nbf, ShellC<B_>::nbf> gr;                                      \
                 _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
                     _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
                         gr.v[i + ShellC<A>::nbf * j] = gsl[i + nrows * j];                         \
-                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
+                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
             } else {                                                                               \
-                pair_bwd<T, A, B_>(recs, nra, nla, nrb, nlb, K, R2, ab, GStripe<T>{gsl, nrows}, aA, aB, \
+                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, GStripe<T>{gsl, nrows}, aA, aB, \
                                    lac, lane);                                                     \
             }                                                                                      \
             continue;                                                                              \
@@ -1082,8 +1209,13 @@ __global__ __launch_bounds__(256) void ovl2c_dx_reduce(int64_t B, int nchunk, co
 // index), dcoef first, then dalpha at + 2 * NZ * npar.  Stage 2 (the sum over the
 // 2 * NZ parts) runs in the output stage.
 // ---------------------------------------------------------------------------------
+// The exponent rows of the partials hold the exponent cotangent at fixed primitive norm,
+// da' = da + n coef dcoef with n = (2l + 3)/(4 alpha) (pair_contract); the slot's block
+// subtracts n coef dcoef in FP64 after its sum (real slots; padded ones have coef = 0).
 __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const int *__restrict__ toff,
-                                                        const int *__restrict__ pcls,
+                                                        const int *__restrict__ pcls, const int *__restrict__ nreal,
+                                                        const double *__restrict__ coef64,
+                                                        const double *__restrict__ alpha64,
                                                         const double *__restrict__ partial, double *__restrict__ part)
 {
     __shared__ int rel_ci[OVL_MAXCH], rel_col[OVL_MAXCH];
@@ -1136,6 +1268,8 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
     block_sum256x2(c, a, red);
     if (threadIdx.x == 0) {
         const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+        const double al = alpha64[dst];
+        if (li < nreal[sg * ns + s] && al > 0.0) a -= (2 * sh.ls[s] + 3) / (4.0 * al) * coef64[dst] * c;
         part[blockIdx.y * npar + dst] = c;
         part[(2 * NZ + blockIdx.y) * npar + dst] = a;
     }
@@ -1449,7 +1583,8 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
                        reinterpret_cast<int *>(ws + w.perm), reinterpret_cast<int *>(ws + w.tile_key), w.maxtiles,
                        reinterpret_cast<const double *>(ws + w.coef64), reinterpret_cast<const double *>(ws + w.alpha64),
                        reinterpret_cast<const int *>(ws + w.nlist), reinterpret_cast<const int *>(ws + w.lidx),
-                       reinterpret_cast<R *>(ws + w.rec));
+                       reinterpret_cast<R *>(ws + w.rec),
+                       w.rec2 ? reinterpret_cast<PairPb<T> *>(ws + w.rec2) : nullptr);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
@@ -1514,6 +1649,7 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
                        reinterpret_cast<const int *>(ws + w.perm), reinterpret_cast<const int *>(ws + w.tile_key),
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
+                       reinterpret_cast<const PairPb<T> *>(ws + w.rec2),
                        reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const int *>(ws + w.nlist),
                        reinterpret_cast<double *>(ws + w.partial), chunk0, nch, grp);
     AOK_HIP(ctx, hipGetLastError());
@@ -1525,6 +1661,8 @@ static aok_status bwd_reduce_launch(aok_ctx ctx, const OvlShape &sh, const Ovl2c
     const int nslot = sh.NZ * sh.ns * sh.K;
     hipLaunchKernelGGL(ovl2c_bwd_reduce, dim3(nslot, 2 * sh.NZ), dim3(256), 0, ctx->stream, sh,
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const int *>(ws + w.pcls),
+                       reinterpret_cast<const int *>(ws + w.nreal), reinterpret_cast<const double *>(ws + w.coef64),
+                       reinterpret_cast<const double *>(ws + w.alpha64),
                        reinterpret_cast<const double *>(ws + w.partial), reinterpret_cast<double *>(ws + w.part2));
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
@@ -1749,7 +1887,8 @@ aok_status ovl2c_run(aok_ctx ctx, const Ovl2cCall &c, bool pullback)
     aok_status st = shape_of(ctx, c, sh, pullback);
     if (st != AOK_OK) return st;
     if (c.flags & AOK_PLAIN) return plain_run<T>(ctx, c, sh, pullback);
-    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), pullback);
+    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type),
+                                   sizeof(PairPb<T>), pullback);
     st = ensure_ws(ctx, w.total);
     if (st != AOK_OK) return st;
     char *ws = static_cast<char *>(ctx->ws);
@@ -1792,7 +1931,8 @@ aok_status ovl2c_run_noalloc(aok_ctx ctx, const Ovl2cCall &c)
     OvlShape sh;
     aok_status st = shape_of(ctx, c, sh, false);
     if (st != AOK_OK) return st;
-    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), false);
+    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type),
+                                   sizeof(PairPb<T>), false);
     if (w.total > AOK_NOALLOC_SCRATCH_BYTES)
         return fail(ctx, AOK_ERR_ARG,
                     "aok_ovl2c_noalloc: this batch needs " + std::to_string(w.total) +
@@ -1823,7 +1963,8 @@ aok_status ovl2c_plan_build(aok_ctx ctx, const Ovl2cCall &c, Ovl2cPlanState &p)
     if (st != AOK_OK) return st;
     p.call = c;
     if (c.flags & AOK_PLAIN) return AOK_OK;
-    p.w = ovl2c_ws_map(c.B, p.sh.ns, p.sh.K, p.sh.NZ, p.sh.nchunk, sizeof(typename RecOf<T>::type), true);
+    p.w = ovl2c_ws_map(c.B, p.sh.ns, p.sh.K, p.sh.NZ, p.sh.nchunk, sizeof(typename RecOf<T>::type),
+                          sizeof(PairPb<T>), true);
     if (p.w.total > p.bytes) {
         if (in_capture(ctx))
             return fail(ctx, AOK_ERR_ARG,
@@ -1920,7 +2061,8 @@ aok_status ovl2c_dx_run(aok_ctx ctx, const Ovl2cCall &c)
     aok_status st = shape_of(ctx, c, sh, true);
     if (st != AOK_OK) return st;
     if (c.flags & AOK_PLAIN) return fail(ctx, AOK_ERR_ARG, "position cotangents are computed by the fast path only");
-    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type), false);
+    const Ovl2cWs w = ovl2c_ws_map(c.B, sh.ns, sh.K, sh.NZ, sh.nchunk, sizeof(typename RecOf<T>::type),
+                                   sizeof(PairPb<T>), false);
     st = ensure_ws(ctx, w.total + dx_partial_bytes(sh, c.B));
     if (st != AOK_OK) return st;
     char *ws = static_cast<char *>(ctx->ws);

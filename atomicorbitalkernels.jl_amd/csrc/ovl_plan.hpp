@@ -90,6 +90,16 @@ struct RecOf<double> {
     using type = PairRecD;
 };
 
+// Pullback-only companion of a record: the coefficients of the pullback's exponent
+// tables (pair_contract), c_i = (alpha - (2i+1) beta)/(4 alpha p) = ca - i sa and
+// d_j = (beta - (2j+1) alpha)/(4 beta p) = cb - j sb, from ca = (alpha - beta)/(4 alpha p),
+// cb = -(alpha - beta)/(4 beta p), sa = beta/(2 alpha p), sb = alpha/(2 beta p) (FP64,
+// rounded once).
+template <typename T>
+struct alignas(4 * sizeof(T)) PairPb {
+    T ca, cb, sa, sb;
+};
+
 __host__ __device__ inline int nbf_of(int l) { return (l + 1) * (l + 2) / 2; }
 
 // Stripe plan: for each ket shell sb, consecutive bra shells grouped while
@@ -156,7 +166,7 @@ inline std::string build_shape(int64_t ns, int64_t K, int64_t NZ, const int64_t 
 
 // Workspace map of one fast 2c call (byte offsets, 256-aligned).
 struct Ovl2cWs {
-    size_t coef64, alpha64, nreal, nlist, lidx, pcls, rec;
+    size_t coef64, alpha64, nreal, nlist, lidx, pcls, rec, rec2;  // rec2: pullback maps only, else 0
     size_t blk_hist, rank, blk_off, bcnt, boff, toff, perm, tile_key;
     size_t partial, part2;
     size_t total;
@@ -164,7 +174,8 @@ struct Ovl2cWs {
 };
 
 
-inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, int nchunk, size_t rec_bytes, bool pullback)
+inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, int nchunk, size_t rec_bytes, size_t pb_bytes,
+                            bool pullback)
 {
     Ovl2cWs w{};
     const size_t npar = static_cast<size_t>(ns) * K * NZ;
@@ -193,6 +204,7 @@ inline Ovl2cWs ovl2c_ws_map(int64_t B, int ns, int K, int NZ, int nchunk, size_t
     w.perm = take(sizeof(int) * static_cast<size_t>(B > 0 ? B : 1));
     w.tile_key = take(sizeof(int) * static_cast<size_t>(w.maxtiles));
     if (pullback) {
+        w.rec2 = take(pb_bytes * static_cast<size_t>(w.NK) * ns * ns * K * K);
         w.partial = take(sizeof(double) * static_cast<size_t>(w.maxtiles) * nchunk * (ns + 1) * K * 2);
         w.part2 = take(sizeof(double) * 4 * static_cast<size_t>(NZ) * npar);
     }
