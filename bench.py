@@ -30,14 +30,14 @@ FP32_UNPACKED_TFLOPS = 78.6       # the same without packed FP32 (the kernels ar
 # runs of this bench's C1 step (--no-extra; KiB per dispatch, mean over the launches), each
 # scaled by what the same counter reads for the bench's 2 GiB device copy (FETCH_SIZE reads
 # 1048589 KiB for 2097152 KiB moved: x2.0; WRITE_SIZE reads 2097152 KiB: x1.0).
-PMC_KIB = {"ovl2c_bwd_fast": (121814.0, 10326.0), "ovl2c_fwd_fast": (9147.0, 114265.0)}
+PMC_KIB = {"ovl2c_bwd_fast": (128534.1, 10322.4), "ovl2c_fwd_fast": (9133.0, 115223.8)}
 PMC_SCALE = (2.0, 1.0)
 # FLOPs per pair the C1 kernels execute: rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP32 and
 # SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes), per launch / 65536 pairs;
 # VALU busy: SQ_ACTIVE_INST_VALU * 4 / (1024 SIMDs * GRBM_GUI_ACTIVE / 8 XCDs).  Reported
 # beside the algorithmic rate, never priced.
-EXEC_FLOP_PER_PAIR = {"ovl2c_bwd_fast": (3.291e9 + 9.99e7) / 65536, "ovl2c_fwd_fast": (4.032e8 + 1.287e8) / 65536}
-VALU_BUSY = {"ovl2c_bwd_fast": 0.68, "ovl2c_fwd_fast": 0.42}
+EXEC_FLOP_PER_PAIR = {"ovl2c_bwd_fast": (3.7e+09 + 9.99e+07) / 65536, "ovl2c_fwd_fast": (4.032e+08 + 1.287e+08) / 65536}
+VALU_BUSY = {"ovl2c_bwd_fast": 0.68, "ovl2c_fwd_fast": 0.41}
 METRIC = "2-ctr overlap pairs/s + (ζ,D)-grad pairs/s, FP32@1e-6 vs plain FP64; AO GB/s"
 
 
@@ -310,9 +310,14 @@ def main():
         for k in ("zeta", "D"):
             g32, g64 = dps32["Rnl"][k].double(), dps64["Rnl"][k]
             fp32[f"c1_d{k}"] = float((g32 - g64).abs().max() / g64.abs().max())
+        # position cotangents (dA; dB = -dA)
+        dA32 = pbx(dS32)[2].double()
+        _, pbx64 = aok.rrule_batch_overlap(orb64, XA64r, XB64r, ps64, ctx=ctx, wrt_positions=True)
+        dA64 = pbx64(aok.as_fortran(dS32.double()))[2]
+        fp32["c1_dA"] = float((dA32 - dA64).abs().max() / dA64.abs().max())
         out["fp32_blockwise_max_rel_err"] = fp32["c1_S_blockwise"]
         out["fp32_error"] = fp32
-        del S64, pb64
+        del S64, pb64, pbx64
         # plain FP64 GPU form (reference decomposition) -- the >= 4x denominator
         log("C1 plain FP64 forward/pullback")
         dS64 = dS32.double()
@@ -495,6 +500,17 @@ def main():
         err2 = float((num / den).max())
         fp32["c2_S_blockwise"] = err2
         del S2d
+        # C2 (zeta, D) cotangents against the FP64 pullback of the same rounded inputs
+        dq32 = pb2(dS2)[4]
+        _, pb2d = aok.rrule_batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]),
+                                          (X2B[0].double().contiguous(), X2B[1]),
+                                          {"Rnl": {"zeta": aok.as_fortran(o2.Rnl.zeta.double()),
+                                                   "D": aok.as_fortran(o2.Rnl.D.double())}, "Ylm": {}}, ctx=ctx)
+        dq64 = pb2d(aok.as_fortran(dS2.double()))[4]
+        for k in ("zeta", "D"):
+            fp32[f"c2_d{k}"] = float((dq32["Rnl"][k].double() - dq64["Rnl"][k]).abs().max() /
+                                     dq64["Rnl"][k].abs().max())
+        del pb2d, dq64
         aok.batch_overlap(o2d, (X2A[0].double().contiguous(), X2A[1]), (X2B[0].double().contiguous(), X2B[1]),
                           o2d.static_params(), mode="plain", ctx=ctx)
         plain2 = ctx.last_kernel_ms()
@@ -566,6 +582,15 @@ def main():
                                      "wall_ms": wall3b, "triples_per_s": B3 / (wall3b * 1e-3),
                                      "fp32_max_rel_err": err3}
         fp32["c4a_dcoef_dalpha"] = err3
+        # (zeta, D) cotangents through the normalisation VJP, against FP64 on the same inputs
+        _, pb3 = aok.rrule_batch_overlap_3c(orb32, *X3, orb32.static_params(), ctx=ctx)
+        dq3 = pb3(dT)[5]
+        _, pb3d = aok.rrule_batch_overlap_3c(orb64, *X3d, ps64, ctx=ctx)
+        dq3d = pb3d(aok.as_fortran(dT.double()))[5]
+        for k in ("zeta", "D"):
+            fp32[f"c4a_d{k}"] = float((dq3["Rnl"][k].double() - dq3d["Rnl"][k]).abs().max() /
+                                      dq3d["Rnl"][k].abs().max())
+        del pb3, pb3d
         # the first 256 triples through the FP32 kernels, kept for the CPU leg's oracle check
         n3 = 256
         X3s = [(x[0][:n3].contiguous(), x[1][:n3].contiguous()) for x in X3]
