@@ -911,12 +911,14 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
                                                      const int *__restrict__ toff,
                                                      const typename RecOf<T>::type *__restrict__ rec,
                                                      const int *__restrict__ nreal, T *__restrict__ out, int chunk0,
-                                                     int nch, int group)
+                                                     int nch, int group, int narrow)
 {
     extern __shared__ unsigned char smem_raw[];
     // LDS: the lanes' block offsets b * N * N, the column-segment offsets of the chunk's
-    // stripe elements, then the stripe itself (fwd_lds_bytes)
+    // stripe elements, then the stripe itself (fwd_lds_bytes).  With `narrow` (the output
+    // spans less than 4 GiB) both offsets are 32-bit byte offsets.
     int64_t *bl_off = reinterpret_cast<int64_t *>(smem_raw);
+    uint32_t *bl_off32 = reinterpret_cast<uint32_t *>(smem_raw);
     int *q_off = reinterpret_cast<int *>(smem_raw + 8 * OVL_TILE);
     T *stripe = reinterpret_cast<T *>(smem_raw + 8 * OVL_TILE + 4 * ((sh.str + 3) & ~3));
     const int ns = sh.ns, K = sh.K, NZ = sh.NZ, N = sh.N;
@@ -927,7 +929,10 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
     const int lane = threadIdx.x;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
     const int64_t b = perm[ti.first + (lane < ti.nact ? lane : 0)];
-    bl_off[lane] = static_cast<int64_t>(b) * N * N;
+    if (narrow)
+        bl_off32[lane] = static_cast<uint32_t>(b * N * N * static_cast<int64_t>(sizeof(T)));
+    else
+        bl_off[lane] = static_cast<int64_t>(b) * N * N;
     const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
     const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
     const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);
@@ -942,7 +947,8 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
         const int nrb = nreal[sgB * ns + sb];
         const int nbb = nbf_of(lb), nrows = ch.nrows;
         const int len = nrows * nbb;
-        for (int q = lane; q < len; q += OVL_TILE) q_off[q] = N * (q / nrows) + q % nrows;
+        const int qscale = narrow ? static_cast<int>(sizeof(T)) : 1;
+        for (int q = lane; q < len; q += OVL_TILE) q_off[q] = (N * (q / nrows) + q % nrows) * qscale;
         for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
             const int la = sh.ls[sa];
             const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
@@ -965,13 +971,48 @@ __global__ __launch_bounds__(64) void ovl2c_fwd_fast(const OvlShape sh, const T 
         T *obase = out + static_cast<int64_t>(N) * sh.off[sb] + ch.row0;
         const int dbl = OVL_TILE / len, dq = OVL_TILE - dbl * len;
         int bl = lane / len, q = lane - bl * len;
-        for (int e = lane; e < total; e += OVL_TILE) {
-            obase[bl_off[bl] + q_off[q]] = stripe[bl * sh.str + q];
-            q += dq;
-            bl += dbl;
-            if (q >= len) {
-                q -= len;
-                ++bl;
+        if (narrow) {
+            // 32-bit byte offsets from the uniform base (one address add per element) and
+            // four elements per pass (C1 forward 53.6 -> 52.6 us)
+            char *ob = reinterpret_cast<char *>(obase);
+            int si = bl * sh.str + q;
+            const int dsi = dbl * sh.str + dq, wrap = sh.str - len;
+            auto next = [&]() {
+                q += dq;
+                bl += dbl;
+                si += dsi;
+                if (q >= len) {
+                    q -= len;
+                    ++bl;
+                    si += wrap;
+                }
+            };
+            int e = lane;
+            for (; e + 3 * OVL_TILE < total; e += 4 * OVL_TILE) {
+                uint32_t o[4];
+                T v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    o[u] = bl_off32[bl] + static_cast<uint32_t>(q_off[q]);
+                    v[u] = stripe[si];
+                    next();
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) *reinterpret_cast<T *>(ob + o[u]) = v[u];
+            }
+            for (; e < total; e += OVL_TILE) {
+                *reinterpret_cast<T *>(ob + (bl_off32[bl] + static_cast<uint32_t>(q_off[q]))) = stripe[si];
+                next();
+            }
+        } else {
+            for (int e = lane; e < total; e += OVL_TILE) {
+                obase[bl_off[bl] + q_off[q]] = stripe[bl * sh.str + q];
+                q += dq;
+                bl += dbl;
+                if (q >= len) {
+                    q -= len;
+                    ++bl;
+                }
             }
         }
     }
@@ -1625,7 +1666,7 @@ static aok_status fast_fwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
                        reinterpret_cast<const int *>(ws + w.boff), reinterpret_cast<const int *>(ws + w.bcnt),
                        reinterpret_cast<const int *>(ws + w.toff), reinterpret_cast<const R *>(ws + w.rec),
                        reinterpret_cast<const int *>(ws + w.nreal), static_cast<T *>(c.out), chunk0, nch,
-                       grp);
+                       grp, c.B * sh.N * sh.N * static_cast<int64_t>(sizeof(T)) < (int64_t(1) << 32) ? 1 : 0);
     AOK_HIP(ctx, hipGetLastError());
     return AOK_OK;
 }
