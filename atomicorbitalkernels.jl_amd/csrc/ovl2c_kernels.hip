@@ -223,11 +223,23 @@ __device__ __forceinline__ void make_record(double a, double b, double ca, doubl
     r.w2 = static_cast<T>(ca * pref);
 }
 template <typename T>
-__device__ __forceinline__ void make_pb_record(double a, double b, PairPb<T> &r)
+__device__ __forceinline__ void make_pb_record(double a, double b, double ca, double cb, bool ss, PairPb<T> &r)
 {
     const double p = a + b, d = a - b;
-    r.ca = static_cast<T>(0.25 * d / (a * p));
-    r.cb = static_cast<T>(-0.25 * d / (b * p));
+    const double c0a = 0.25 * d / (a * p), c0b = -0.25 * d / (b * p);
+    if (ss) {
+        // s-s pairs close their exponent sums over the axes (pair_contract): the companion
+        // holds Kab (beta/p)^2, 3 Kab c0a, Kab (alpha/p)^2, 3 Kab c0b, so that
+        // Kab Ha = g (ca r2 + cb) and Kab Hb = g (sa r2 + sb) cost one FMA each
+        const double kab = ca * cb * 5.568327996831707845 / (p * sqrt(p));
+        r.ca = static_cast<T>(kab * (b / p) * (b / p));
+        r.cb = static_cast<T>(3.0 * kab * c0a);
+        r.sa = static_cast<T>(kab * (a / p) * (a / p));
+        r.sb = static_cast<T>(3.0 * kab * c0b);
+        return;
+    }
+    r.ca = static_cast<T>(c0a);
+    r.cb = static_cast<T>(c0b);
     r.sa = static_cast<T>(0.5 * b / (a * p));
     r.sb = static_cast<T>(0.5 * a / (b * p));
 }
@@ -288,7 +300,7 @@ __global__ __launch_bounds__(256) void ovl_fill_kernel(const OvlShape sh, const 
     rec[gid] = r;
     if (rec2) {
         PairPb<T> r2;
-        make_pb_record<T>(alpha64[qa], alpha64[qb], r2);
+        make_pb_record<T>(alpha64[qa], alpha64[qb], coef64[qa], coef64[qb], sh.ls[sa] == 0 && sh.ls[sb] == 0, r2);
         rec2[gid] = r2;
     }
 }
@@ -501,10 +513,11 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
         // s-s: s_x s_y s_z = 1 and the exponent sums close over the axes,
         // sum_ax u_ax(0, 0) = sum_ax (PA_ax^2 + (alpha - beta)/(4 alpha p))
         //                   = (beta/p)^2 |AB|^2 + 3 (alpha - beta)/(4 alpha p), likewise the ket
+        // (with the companion's s-s constants, Ha and Hb come out multiplied by Kab)
         const T gv = g(0, 0);
         H = gv;
-        Ha = RAISED ? gv * fma(r.bop * r.bop, r2, T(3) * q.ca) : T(0);
-        Hb = RAISED ? gv * fma(r.aop * r.aop, r2, T(3) * q.cb) : T(0);
+        Ha = RAISED ? gv * fma(q.ca, r2, q.cb) : T(0);
+        Hb = RAISED ? gv * fma(q.sa, r2, q.sb) : T(0);
         return;
     }
     constexpr int NBA = ShellC<LA>::nbf, N1 = 2 * LA + 1;
@@ -641,10 +654,14 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
                                          double *accB, T *lac, int lane)
 {
     using R = typename RecOf<T>::type;
+    constexpr bool SS = LA + LB == 0;  // s-s exponent sums arrive multiplied by Kab
     const T r2f = static_cast<T>(R2);
     const int nrow = 2 * (nla + nlb);
     T *ket = lac + 2 * nla * LROW;
-    for (int v = 0; v < nrow; ++v) lac[v * LROW + lane] = T(0);
+    // bra rows are stored whole below; only the ket rows (accumulated over i) and the
+    // exponent rows of padded bra slots (never written) start at zero
+    for (int v = 2 * nla; v < nrow; ++v) lac[v * LROW + lane] = T(0);
+    for (int i = nra; i < nla; ++i) lac[(2 * i + 1) * LROW + lane] = T(0);
     // real x real: all four cotangents
     int i0 = 0;
     // s-s and s-p: two bra entries per pass, one read-modify-write of each ket row per two
@@ -666,9 +683,9 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
                     pair_contract<T, LA, LB, true>(r, recs2[(i0 + u) * K + j], ab, r2f, g, H, Ha, Hb);
                     const T eH = e * H;
                     cA[u] += r.w1 * eH;
-                    aA[u] -= r.Kab * e * Ha;
+                    aA[u] -= (SS ? e : r.Kab * e) * Ha;
                     kc += r.w2 * eH;
-                    ka += r.Kab * e * Hb;
+                    ka += (SS ? e : r.Kab * e) * Hb;
                 }
                 ket[2 * j * LROW + lane] += kc;
                 ket[(2 * j + 1) * LROW + lane] -= ka;
@@ -690,9 +707,9 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             pair_contract<T, LA, LB, true>(r, recs2[i * K + j], ab, r2f, g, H, Ha, Hb);
             const T eH = e * H;
             cA += r.w1 * eH;
-            aA -= r.Kab * e * Ha;
+            aA -= (SS ? e : r.Kab * e) * Ha;
             ket[2 * j * LROW + lane] += r.w2 * eH;
-            ket[(2 * j + 1) * LROW + lane] -= r.Kab * e * Hb;
+            ket[(2 * j + 1) * LROW + lane] -= (SS ? e : r.Kab * e) * Hb;
         }
         lac[2 * i * LROW + lane] = cA;
         lac[(2 * i + 1) * LROW + lane] = aA;
