@@ -225,8 +225,12 @@ __device__ __forceinline__ void make_record(double a, double b, double ca, doubl
 template <typename T>
 __device__ __forceinline__ void make_pb_record(double a, double b, double ca, double cb, bool ss, PairPb<T> &r)
 {
+    // A non-positive exponent has no L2 norm to take out (n = (2l + 3)/(4 alpha) is
+    // undefined): its side keeps the plain raised table, c_i = (i + 1)/(2p) = c0 - i s with
+    // c0 = 1/(2p), s = -1/(2p), and ovl2c_bwd_reduce adds nothing back for that slot.
     const double p = a + b, d = a - b;
-    const double c0a = 0.25 * d / (a * p), c0b = -0.25 * d / (b * p);
+    const double c0a = a > 0.0 ? 0.25 * d / (a * p) : 0.5 / p;
+    const double c0b = b > 0.0 ? -0.25 * d / (b * p) : 0.5 / p;
     if (ss) {
         // s-s pairs close their exponent sums over the axes (pair_contract): the companion
         // holds Kab (beta/p)^2, 3 Kab c0a, Kab (alpha/p)^2, 3 Kab c0b, so that
@@ -240,8 +244,8 @@ __device__ __forceinline__ void make_pb_record(double a, double b, double ca, do
     }
     r.ca = static_cast<T>(c0a);
     r.cb = static_cast<T>(c0b);
-    r.sa = static_cast<T>(0.5 * b / (a * p));
-    r.sb = static_cast<T>(0.5 * a / (b * p));
+    r.sa = static_cast<T>(a > 0.0 ? 0.5 * b / (a * p) : -0.5 / p);
+    r.sb = static_cast<T>(b > 0.0 ? 0.5 * a / (b * p) : -0.5 / p);
 }
 
 template <typename T>

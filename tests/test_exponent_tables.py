@@ -64,10 +64,14 @@ def exp_tables(PA, PB, i2p, ca, sa, cb, sb, s, dt=np.float64):
 
 
 def record(a, b, dt=np.float64):
-    """The pair constants the kernel reads (ovl_fill_kernel: make_record / make_pb_record)."""
+    """The pair constants the kernel reads (ovl_fill_kernel: make_record / make_pb_record);
+    a side with a non-positive exponent keeps the plain raised table (c0 = 1/(2p), s = -1/(2p))."""
     p = a + b
-    return dict(aop=dt(a / p), bop=dt(b / p), i2p=dt(0.5 / p), ca=dt(0.25 * (a - b) / (a * p)),
-                cb=dt(-0.25 * (a - b) / (b * p)), sa=dt(0.5 * b / (a * p)), sb=dt(0.5 * a / (b * p)))
+    ca = 0.25 * (a - b) / (a * p) if a > 0 else 0.5 / p
+    cb = -0.25 * (a - b) / (b * p) if b > 0 else 0.5 / p
+    sa = 0.5 * b / (a * p) if a > 0 else -0.5 / p
+    sb = 0.5 * a / (b * p) if b > 0 else -0.5 / p
+    return dict(aop=dt(a / p), bop=dt(b / p), i2p=dt(0.5 / p), ca=dt(ca), cb=dt(cb), sa=dt(sa), sb=dt(sb))
 
 
 @pytest.mark.parametrize("LA,LB", [(la, lb) for la in range(5) for lb in range(5)])
@@ -151,3 +155,20 @@ def test_fp32_recursion_keeps_the_cancelling_digits():
     err_diff = abs(float(diff) - exact) / exact
     assert err_rec < 1e-6
     assert err_diff > 100 * err_rec
+
+
+@pytest.mark.parametrize("LA,LB", [(0, 0), (1, 2), (2, 1), (3, 3)])
+def test_zero_exponent_keeps_the_raised_table(LA, LB):
+    """alpha = 0 (no L2 norm to take out): the bra table is the plain raised table s(i+2, j)."""
+    b = 1.7
+    AB = 0.9
+    r = record(0.0, b)
+    PA, PB = -r["bop"] * AB, r["aop"] * AB
+    big = os1d(PA, PB, r["i2p"], LA + 2, LB + 2)
+    s = big[:LA + 1, :LB + 1]
+    u, v = exp_tables(PA, PB, r["i2p"], r["ca"], r["sa"], r["cb"], r["sb"], s)
+    for i in range(LA + 1):
+        for j in range(LB + 1):
+            assert abs(u[i, j] - big[i + 2, j]) <= 1e-12 * (abs(big[i + 2, j]) + 1e-300)
+            rv = big[i, j + 2] - (2 * j + 1) / (4 * b) * big[i, j]
+            assert abs(v[i, j] - rv) <= 1e-12 * (abs(big[i, j + 2]) + abs((2 * j + 1) / (4 * b) * big[i, j]))

@@ -170,6 +170,35 @@ def test_c1_small_batch_zeta_gradient_seeds(dev):
     assert amplified > 5.0  # the cancelling case is among the seeds
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_zero_exponent_real_slot_pullback(dev, dtype):
+    """A real primitive with exponent 0 on the bra species (its partners all have positive
+    exponents): no L2 norm to take out of its exponent cotangent, the kernels keep the plain
+    raised table for that side.  (dcoef, dalpha) vs the oracle; FP64 1e-10, FP32 1e-5."""
+    ls = [0, 1, 2]
+    K, NZ, B = 3, 2, 130
+    rng = np.random.default_rng(21)
+    zeta = np.exp(rng.uniform(np.log(0.3), np.log(30.0), size=(len(ls), K, NZ)))
+    zeta[0, 1, 0] = 0.0  # species 1, s shell, second primitive
+    zeta[1, 0, 0] = 0.0  # species 1, p shell, first primitive
+    coef = rng.standard_normal((len(ls), K, NZ))
+    sA, sB = np.ones(B, dtype=np.int64), np.full(B, 2, dtype=np.int64)
+    A = rng.normal(0.0, 1.0, size=(B, 3))
+    Bp = A + rng.normal(0.0, 1.5, size=(B, 3))
+    log(f"2c zero-exponent slot pullback B={B} dtype={dtype}")
+    basis = _cart_basis(ls, zeta, coef, dtype, dev)
+    XA = (dev_rows(A, dtype, dev), torch.as_tensor(sA, device=dev))
+    XB = (dev_rows(Bp, dtype, dev), torch.as_tensor(sB, device=dev))
+    dS = rng.standard_normal((basis.nbf_total, basis.nbf_total, B))
+    dc, da = aok.overlap_pb_coef_alpha(dev_f(dS, dtype, dev), basis, XA, XB)
+    dcr, dar = O.ovl2c_pb(rounded(dS, dtype), rounded(A, dtype), rounded(Bp, dtype), sA, sB, ls,
+                          rounded(coef, dtype), rounded(zeta, dtype))
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    assert np.all(np.isfinite(host(da)))
+    assert rel_max(host(dc), dcr) < tol
+    assert rel_max(host(da), dar) < tol
+
+
 def test_pstates_species_labels_and_errors(dev):
     d = _c1_small(130)
     cb = d["basis"]
