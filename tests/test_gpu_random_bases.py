@@ -1,0 +1,107 @@
+"""Randomised parity sweep of the 2-centre fast path: bases of random shape (1-7 shells,
+l <= 4, 1-8 primitives per shell with absent and padded slots, 1-4 species), random batch
+sizes (ragged tiles, a single pair) and stripe capacities, through the forward and the
+(zeta, D) pullback of the rrule, FP64 and FP32, against the oracle.
+
+Tolerances: FP64 1e-12 blockwise (values) / 1e-10 (cotangents).  FP32 is held to the
+conditioning of each case.  A block's values are checked to 3e-6 * max(1, kappa_b) of its
+largest entry, kappa_b = max|S_b| contracted with |coefficients| / max|S_b| (mixed-sign
+contractions of near-equal diffuse exponents cancel: kappa reaches 99 in this sweep, and the
+FP32 error stays below 1.2e-6 * kappa_b over 40 seeds); blocks whose FP64 values lie below
+FP32's normal range (1e-30) are checked to stay there instead.  Cotangents are checked to
+1e-5 * max(1, kappa / 5) of the largest entry, kappa the largest kappa_b of the case.  The
+D-cotangent is measured against the larger of its own scale and that of the coefficient
+cotangent before the normalisation projection: a species whose shells have one primitive each
+has dD = 0 exactly (D is normalised away), so the result is rounding of the projection.
+"""
+import numpy as np
+import pytest
+import torch
+
+import aok
+import oracle as O
+from aok import configs
+from helpers import dev_f, dev_rows, host, log, rel_max, rounded
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_case(seed):
+    rng = np.random.default_rng(1000 + seed)
+    ns = int(rng.integers(1, 8))
+    ls = sorted(int(x) for x in rng.integers(0, 5, size=ns))
+    K = int(rng.integers(1, 9))
+    NZ = int(rng.integers(1, 5))
+    species = []
+    for _ in range(NZ):
+        shells = []
+        for l in ls:
+            if rng.random() < 0.15:
+                shells.append(None)  # absent shell: all slots padded
+                continue
+            depth = int(rng.integers(1, K + 1))
+            z = np.sort(np.exp(rng.uniform(np.log(0.1), np.log(3000.0), depth)))[::-1]
+            shells.append((z, rng.standard_normal(depth)))
+        species.append(shells)
+    cb = configs._pad_basis(ls, species, K)
+    B = int(rng.choice([1, 63, 64, 65, int(rng.integers(100, 700))]))
+    A = rng.normal(0.0, 0.6, size=(B, 3))
+    d = rng.normal(size=(B, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    Bp = A + d * rng.uniform(0.3, 6.0, size=(B, 1))
+    sA = rng.integers(1, NZ + 1, size=B)
+    sB = rng.integers(1, NZ + 1, size=B)
+    cap = int(rng.choice([0, 16, 24, 112]))
+    return cb, A, Bp, sA, sB, cap
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("seed", list(range(16)))
+def test_random_basis_forward_and_pullback(dev, seed, dtype):
+    cb, A, Bp, sA, sB, cap = _random_case(seed)
+    B = len(sA)
+    log(f"2c random basis seed={seed} ls={cb.ls} K={cb.K} NZ={cb.NZ} B={B} cap={cap} dtype={dtype}")
+    ctx = aok.context(0)
+    ctx.tune("ovl_cap", cap)
+    try:
+        spec, _, _ = cb.ao_maps()
+        R = aok.GaussianTypeRadials(dev_f(cb.zeta, dtype, dev), dev_f(cb.D, dtype, dev), spec, None,
+                                    tuple(range(1, cb.NZ + 1)))
+        orb = aok.AtomicOrbitals(R, max(cb.ls), length_unit="bohr")
+        XA = (dev_rows(A, dtype, dev), torch.as_tensor(sA, device=dev))
+        XB = (dev_rows(Bp, dtype, dev), torch.as_tensor(sB, device=dev))
+        S, pb = aok.rrule_batch_overlap(orb, XA, XB, orb.static_params(), None, ctx=ctx)
+        dS = np.random.default_rng(seed).standard_normal((cb.N, cb.N, B))
+        dps = pb(dev_f(dS, dtype, dev))[4]
+    finally:
+        ctx.tune("ovl_cap", 0)
+    z, D = rounded(cb.zeta, dtype), rounded(cb.D, dtype)
+    rA, rB = rounded(A, dtype), rounded(Bp, dtype)
+    coef = O.compile_coef(z, D, cb.ls)
+    Sr = O.ovl2c(rA, rB, sA, sB, cb.ls, coef, z)
+    Sabs = O.ovl2c(rA, rB, sA, sB, cb.ls, np.abs(coef), z)
+    dc, da = O.ovl2c_pb(rounded(dS, dtype), rA, rB, sA, sB, cb.ls, coef, z)
+    dDr, dzn = O.compile_coef_pb(dc, z, D, cb.ls)
+    dzr = da + dzn
+    f64 = dtype == torch.float64
+    Sg, dz, dD = host(S), host(dps["Rnl"]["zeta"]), host(dps["Rnl"]["D"])
+    assert np.all(np.isfinite(Sg)) and np.all(np.isfinite(dz)) and np.all(np.isfinite(dD))
+    bmax = np.max(np.abs(Sr), axis=(0, 1))
+    kappa_b = np.max(np.abs(Sabs), axis=(0, 1)) / np.maximum(bmax, 1e-300)
+    normal = bmax >= (1e-300 if f64 else 1e-30)
+    err_b = np.max(np.abs(Sg - Sr), axis=(0, 1)) / np.maximum(bmax, 1e-300)
+    kappa = float(kappa_b[normal].max()) if normal.any() else 1.0
+    es = float((err_b[normal] / np.maximum(kappa_b[normal], 1.0)).max()) if normal.any() else 0.0
+    tiny = float(np.abs(Sg[:, :, ~normal]).max()) if (~normal).any() else 0.0
+    live = D != 0
+    raw_D = np.abs(dc[live] * coef[live] / D[live]).max() if live.any() else 0.0
+    scale_D = max(np.abs(dDr).max(), raw_D, 1e-300)
+    ez = rel_max(dz, dzr)
+    eD = float(np.abs(dD - dDr).max() / scale_D)
+    print(f"seed {seed} {dtype}: S {es:.2e} (per kappa; kappa {kappa:.1f}, {(~normal).sum()} blocks"
+          f" below FP32 range, max {tiny:.1e}) dzeta {ez:.2e} dD {eD:.2e}")
+    assert es < (1e-12 if f64 else 3e-6), es
+    assert tiny < 1e-29, tiny
+    tol = 1e-10 if f64 else 1e-5 * max(1.0, kappa / 5)
+    assert ez < tol, (ez, tol)
+    assert eD < tol, (eD, tol)
