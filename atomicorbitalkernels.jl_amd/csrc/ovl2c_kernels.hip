@@ -648,7 +648,9 @@ __device__ __forceinline__ void pair_contract(const typename RecOf<T>::type &r, 
 // rows 2*nla + 2*j + c.  Bra sums run over j in registers, ket sums over i in the rows.
 // After the shell pair, lane v sums row v over the 64 lanes in FP64 (a transposed
 // reduction: one pass for all rows instead of one cross-lane reduction per row) and adds
-// it to the wave's FP64 accumulators accA[2*i + c] / accB[2*j + c].
+// it to the chunk's FP64 sums: the bra rows are stored to the shell's slice of the
+// workgroup's partial, accA[2*i + c] (global; each bra shell of a chunk is visited once),
+// the ket rows added to accB[2*j + c] (LDS, summed over the chunk's bra shells).
 constexpr int LROW = OVL_TILE + 1;
 
 template <typename T, int LA, int LB, typename G>
@@ -755,7 +757,7 @@ __device__ __forceinline__ void pair_bwd(const typename RecOf<T>::type *__restri
             for (int l = 0; l < OVL_TILE; ++l) sum += static_cast<double>(row[l]);
         }
         if (v < 2 * nla)
-            accA[v] += sum;
+            accA[v] = sum;
         else
             accB[v - 2 * nla] += sum;
     }
@@ -1079,18 +1081,21 @@ void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
     // into registers (no LDS stripe, so more waves fit per CU); larger blocks, and FP64
     // (twice the registers), stage it through the stripe.
     constexpr bool DIRECT = bwd_direct<T, LMAXB>();
-    double *accA = reinterpret_cast<double *>(smem_raw + 256);  // [sa - sa0][K][2], then the ket side at ns*K*2
-    T *stripe = reinterpret_cast<T *>(accA + pstride);
+    // LDS: [pair index per lane (stripe staging only)][ket sums K x 2 FP64][stripe][lane rows];
+    // the bra sums go straight to the partial, which keeps the FP32 C1 instance at 9.5 KB
+    // (16 workgroups per CU, the VGPR limit; with the bra sums in LDS, 10.6 KB: 15)
+    double *accB = reinterpret_cast<double *>(smem_raw + (DIRECT ? 0 : 256));
+    T *stripe = reinterpret_cast<T *>(accB + 2 * K);
     T *lac = DIRECT ? stripe : stripe + OVL_TILE * sh.str;  // per-lane sums [4 * K][LROW]
     TileInfo ti;
     int tile, chunk;
     if (!tile_chunk(chunk0, nch, group, tile, chunk)) return;
     if (!tile_info(tile, NZ * NZ, tile_key, boff, bcnt, toff, ti)) return;
     const int lane = threadIdx.x;
-    for (int q = lane; q < pstride; q += OVL_TILE) accA[q] = 0.0;
+    for (int q = lane; q < 2 * K; q += OVL_TILE) accB[q] = 0.0;
     const int sgA = ti.key / NZ, sgB = ti.key - sgA * NZ;
     const int64_t b = perm[ti.first + (lane < ti.nact ? lane : 0)];
-    bl_b[lane] = static_cast<int>(b);  // b < 2^30 (the batch cap)
+    if constexpr (!DIRECT) bl_b[lane] = static_cast<int>(b);  // b < 2^30 (the batch cap)
     const T Ax = scale * posA[3 * b], Ay = scale * posA[3 * b + 1], Az = scale * posA[3 * b + 2];
     const T Bx = scale * posB[3 * b], By = scale * posB[3 * b + 1], Bz = scale * posB[3 * b + 2];
     const double dx = static_cast<double>(Ax) - static_cast<double>(Bx);
@@ -1113,7 +1118,7 @@ void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
     const bool active = lane < ti.nact;
     const T *gblk = dS + static_cast<int64_t>(b) * NN + colbase - ch.row0;  // + off[sa] + i + N * j
     const int nrb = nreal[sgB * ns + sb], nlb = nlist[sgB * ns + sb];
-    double *aB = accA + ns * K * 2;
+    double *pt = partial + (static_cast<int64_t>(tile) * sh.nchunk + chunk) * pstride;
     for (int sa = ch.sa0; sa < ch.sa1; ++sa) {
         const int la = sh.ls[sa];
         const int64_t g = (static_cast<int64_t>(ti.key) * ns + sb) * ns + sa;
@@ -1121,7 +1126,7 @@ void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
         const PairPb<T> *recs2 = rec2 + g * K * K;
         const int nra = nreal[sgA * ns + sa], nla = nlist[sgA * ns + sa];
         const T *gsl = mine + (sh.off[sa] - ch.row0);
-        double *aA = accA + 2 * (sa - ch.sa0) * K;
+        double *aA = pt + 2 * (sa - ch.sa0) * K;
 #define AOK_BWD_CASE(A, B_)                                                                        \
     if constexpr (A <= LMAXB && B_ <= LMAXB)                                                       \
         if (la == A && lb == B_) {                                                                 \
@@ -1131,16 +1136,16 @@ void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
                 _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
                     _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
                         gr.v[i + ShellC<A>::nbf * j] = active ? src[i + static_cast<int64_t>(N) * j] : T(0); \
-                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
+                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, gr, aA, accB, lac, lane);    \
             } else if constexpr (std::is_same<T, float>::value &&                                  \
                                  ShellC<A>::nbf * ShellC<B_>::nbf >= BWD_GREG_MIN) {                \
                 GRegs<T, ShellC<A>::nbf, ShellC<B_>::nbf> gr;                                      \
                 _Pragma("unroll") for (int j = 0; j < ShellC<B_>::nbf; ++j)                        \
                     _Pragma("unroll") for (int i = 0; i < ShellC<A>::nbf; ++i)                     \
                         gr.v[i + ShellC<A>::nbf * j] = gsl[i + nrows * j];                         \
-                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, gr, aA, aB, lac, lane);    \
+                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, gr, aA, accB, lac, lane);    \
             } else {                                                                               \
-                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, GStripe<T>{gsl, nrows}, aA, aB, \
+                pair_bwd<T, A, B_>(recs, recs2, nra, nla, nrb, nlb, K, R2, ab, GStripe<T>{gsl, nrows}, aA, accB, \
                                    lac, lane);                                                     \
             }                                                                                      \
             continue;                                                                              \
@@ -1149,8 +1154,7 @@ void ovl2c_bwd_fast(const OvlShape sh, const T *__restrict__ dS,
 #undef AOK_BWD_CASE
     }
     __syncthreads();
-    double *pt = partial + (static_cast<int64_t>(tile) * sh.nchunk + chunk) * pstride;
-    for (int q = lane; q < pstride; q += OVL_TILE) pt[q] = accA[q];
+    for (int q = lane; q < 2 * K; q += OVL_TILE) pt[ns * K * 2 + q] = accB[q];
 }
 
 // ---------------------------------------------------------------------------------
@@ -1699,8 +1703,9 @@ static aok_status fast_bwd_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShap
     if (nch <= 0) return AOK_OK;
     using R = typename RecOf<T>::type;
     const int grp = grid_group(ctx, sh, w, nch);
-    const size_t shm = 256 + sizeof(double) * (sh.ns + 1) * sh.K * 2 +
-                       sizeof(T) * ((bwd_direct<T, LMAXB>() ? 0 : OVL_TILE * sh.str) + 4 * sh.K * LROW);
+    constexpr bool DIRECT = bwd_direct<T, LMAXB>();
+    const size_t shm = (DIRECT ? 0 : 256) + sizeof(double) * sh.K * 2 +
+                       sizeof(T) * ((DIRECT ? 0 : OVL_TILE * sh.str) + 4 * sh.K * LROW);
     auto kern = &ovl2c_bwd_fast<T, LMAXB>;
     if (shm > 64 * 1024)
         AOK_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
