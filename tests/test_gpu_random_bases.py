@@ -1,7 +1,8 @@
 """Randomised parity sweep of the 2-centre fast path: bases of random shape (1-7 shells,
 l <= 4, 1-8 primitives per shell with absent and padded slots, 1-4 species), random batch
-sizes (ragged tiles, a single pair) and stripe capacities, through the forward and the
-(zeta, D) pullback of the rrule, FP64 and FP32, against the oracle.
+sizes (ragged tiles, a single pair) and stripe capacities, through the forward, the
+(zeta, D) pullback and the position cotangents of the rrule, FP64 and FP32, against the
+oracle; then the 3-centre and AO paths likewise.
 
 Tolerances: FP64 1e-12 blockwise (values) / 1e-10 (cotangents).  FP32 is held to the
 conditioning of each case.  A block's values are checked to 3e-6 * max(1, kappa_b) of its
@@ -70,9 +71,9 @@ def test_random_basis_forward_and_pullback(dev, seed, dtype):
         orb = aok.AtomicOrbitals(R, max(cb.ls), length_unit="bohr")
         XA = (dev_rows(A, dtype, dev), torch.as_tensor(sA, device=dev))
         XB = (dev_rows(Bp, dtype, dev), torch.as_tensor(sB, device=dev))
-        S, pb = aok.rrule_batch_overlap(orb, XA, XB, orb.static_params(), None, ctx=ctx)
+        S, pb = aok.rrule_batch_overlap(orb, XA, XB, orb.static_params(), None, ctx=ctx, wrt_positions=True)
         dS = np.random.default_rng(seed).standard_normal((cb.N, cb.N, B))
-        dps = pb(dev_f(dS, dtype, dev))[4]
+        _, _, dXA, dXB, dps, _ = pb(dev_f(dS, dtype, dev))
     finally:
         ctx.tune("ovl_cap", 0)
     z, D = rounded(cb.zeta, dtype), rounded(cb.D, dtype)
@@ -98,13 +99,17 @@ def test_random_basis_forward_and_pullback(dev, seed, dtype):
     scale_D = max(np.abs(dDr).max(), raw_D, 1e-300)
     ez = rel_max(dz, dzr)
     eD = float(np.abs(dD - dDr).max() / scale_D)
+    rXA, _ = O.ovl2c_dx(rounded(dS, dtype), rA, rB, sA, sB, cb.ls, coef, z)
+    ex = rel_max(host(dXA), rXA)
     print(f"seed {seed} {dtype}: S {es:.2e} (per kappa; kappa {kappa:.1f}, {(~normal).sum()} blocks"
-          f" below FP32 range, max {tiny:.1e}) dzeta {ez:.2e} dD {eD:.2e}")
+          f" below FP32 range, max {tiny:.1e}) dzeta {ez:.2e} dD {eD:.2e} dA {ex:.2e}")
+    assert torch.equal(dXB, -dXA)
     assert es < (1e-12 if f64 else 3e-6), es
     assert tiny < 1e-29, tiny
     tol = 1e-10 if f64 else 1e-5 * max(1.0, kappa / 5)
     assert ez < tol, (ez, tol)
     assert eD < tol, (eD, tol)
+    assert ex < tol, (ex, tol)
 
 
 def _random_species(rng, ls, K, NZ, zlo, zhi):
