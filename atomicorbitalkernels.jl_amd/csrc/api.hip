@@ -717,7 +717,8 @@ aok_status aok_ovl3c(aok_ctx ctx, aok_dtype dtype, unsigned flags, const void *p
     AOK_REQUIRE(ctx, ls && coef && alpha, "NULL basis argument");
     AOK_REQUIRE(ctx, B >= 0, "B must be >= 0");
     AOK_REQUIRE(ctx, B == 0 || (posA && posB && posC && out), "positions/out are NULL");
-    Ovl3cCall c{flags, posA, posB, posC, sidxA, sidxB, sidxC, itype, B, ls, ns, K, NZ, coef, alpha, lengthscale, out};
+    Ovl3cCall c{flags, posA, posB, posC, sidxA, sidxB, sidxC, itype, B, ls, ns, K, NZ, coef, alpha, lengthscale, out,
+                nullptr, nullptr, nullptr};
     AOK_HIP(ctx, hipSetDevice(ctx->device));
     aok_status st = clear_err(ctx);
     if (st != AOK_OK) return st;
