@@ -1620,7 +1620,13 @@ static int lmax_of(const int64_t *ls, int64_t ns)
     return m;
 }
 
+// Default stripe capacities (floats per lane) when the ovl_cap knob is 0: the forward's
+// chunks, and the pullback's and position cotangents' (the FP32 l <= 2 instances of those
+// hold no stripe, so there the capacity only sets how many bra shells a workgroup walks).
+// C1 (B = 65536): pullback 99.6 -> 93.9 us at 28 against the forward's 40; the forward at
+// 28: 46.7 -> 53.9 us.
 constexpr int OVL_STRIPE_CAP = 40;
+constexpr int OVL_STRIPE_CAP_PB = 28;
 
 template <typename T>
 static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh, const Ovl2cWs &w, char *ws)
@@ -1816,7 +1822,7 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
 // SIMD); the forward keeps the fewer, longer chunks of the unsplit plan in one instance.
 static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh, bool split_light)
 {
-    const int cap = ctx->tune.ovl_cap > 0 ? ctx->tune.ovl_cap : OVL_STRIPE_CAP;
+    const int cap = ctx->tune.ovl_cap > 0 ? ctx->tune.ovl_cap : (split_light ? OVL_STRIPE_CAP_PB : OVL_STRIPE_CAP);
     const std::string msg =
         build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, cap, sh, split_light && ctx->tune.ovl_split != 2);
     if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);
