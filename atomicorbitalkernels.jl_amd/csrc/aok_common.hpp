@@ -67,6 +67,7 @@ struct aok_context {
         int ao_ppl = 0;   // AO values-only: consecutive points per lane (1, 2, 4)
         int ao_grid = 0;  // AO: persistent grid size (blocks)
         int ovl_cap = 0;  // 2c: LDS stripe capacity (floats per lane)
+        int ovl_cap_pb = 0;  // 2c pullback / position cotangents only (0: ovl_cap)
         int ovl_split = 0;  // 2c pullback: light chunks in the l <= 2 instance (0/1 yes, 2 no)
         int ovl_order = 0;  // 2c workgroup order: 0 auto, 1 tile-major, 2 XCD-grouped
         int ovl3_parts = 0;  // 3c pullback parts: 0 auto, 1 none, 2 by C shell, 3 by (C, B) shells

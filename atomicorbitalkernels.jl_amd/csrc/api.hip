@@ -279,6 +279,9 @@ aok_status aok_tune(aok_ctx ctx, const char *key, int64_t value)
     } else if (k == "ovl_cap") {
         AOK_REQUIRE(ctx, value == 0 || (value >= 16 && value <= 1024), "ovl_cap must be 0 or in 16..1024");
         ctx->tune.ovl_cap = static_cast<int>(value);
+    } else if (k == "ovl_cap_pb") {
+        AOK_REQUIRE(ctx, value == 0 || (value >= 16 && value <= 1024), "ovl_cap_pb must be 0 or in 16..1024");
+        ctx->tune.ovl_cap_pb = static_cast<int>(value);
     } else if (k == "ovl_order") {
         AOK_REQUIRE(ctx, value >= 0 && value <= OVL_MAXCH + 1, "ovl_order must be in 0..161");
         ctx->tune.ovl_order = static_cast<int>(value);
@@ -302,6 +305,7 @@ aok_status aok_tune_get(aok_ctx ctx, const char *key, int64_t *value)
     if (k == "ao_ppl") *value = ctx->tune.ao_ppl;
     else if (k == "ao_grid") *value = ctx->tune.ao_grid;
     else if (k == "ovl_cap") *value = ctx->tune.ovl_cap;
+    else if (k == "ovl_cap_pb") *value = ctx->tune.ovl_cap_pb;
     else if (k == "ovl_split") *value = ctx->tune.ovl_split;
     else if (k == "ovl_order") *value = ctx->tune.ovl_order;
     else if (k == "ovl3_parts") *value = ctx->tune.ovl3_parts;

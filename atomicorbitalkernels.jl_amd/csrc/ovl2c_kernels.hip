@@ -1822,7 +1822,8 @@ static aok_status plain_launch(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &
 // SIMD); the forward keeps the fewer, longer chunks of the unsplit plan in one instance.
 static aok_status shape_of(aok_ctx ctx, const Ovl2cCall &c, OvlShape &sh, bool split_light)
 {
-    const int cap = ctx->tune.ovl_cap > 0 ? ctx->tune.ovl_cap : (split_light ? OVL_STRIPE_CAP_PB : OVL_STRIPE_CAP);
+    const int knob = split_light && ctx->tune.ovl_cap_pb > 0 ? ctx->tune.ovl_cap_pb : ctx->tune.ovl_cap;
+    const int cap = knob > 0 ? knob : (split_light ? OVL_STRIPE_CAP_PB : OVL_STRIPE_CAP);
     const std::string msg =
         build_shape(c.ns, c.K, c.NZ, c.ls, AOK_OVL_LMAX, cap, sh, split_light && ctx->tune.ovl_split != 2);
     if (!msg.empty()) return fail(ctx, msg.find("shell l") != std::string::npos ? AOK_ERR_LMAX : AOK_ERR_ARG, msg);

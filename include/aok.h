@@ -295,7 +295,9 @@ aok_status aok_graph_destroy(aok_graph graph);
  * launch-shape knob used by the kernels and by the autotuner.  Keys:
  *   "ao_ppl"   AO values-only evaluation: consecutive points per lane (1, 2 or 4; 0 = default)
  *   "ao_grid"  AO kernels: persistent grid size in workgroups (0 = default)
- *   "ovl_cap"  2-centre kernels: LDS stripe capacity in elements per lane (0 = default)
+ *   "ovl_cap"  2-centre kernels: LDS stripe capacity in elements per lane (0 = default: 40
+ *              for the forward, 28 for the pullback and position cotangents)
+ *   "ovl_cap_pb" the same for the pullback and position cotangents only (0 = ovl_cap)
  *   "ovl_split" 2-centre pullback / position cotangents of bases with f or g shells: run
  *              the chunks of s, p, d shells in the kernel instance compiled for l <= 2
  *              (0 = default = 1; 2 = every chunk in the instance for the largest l)

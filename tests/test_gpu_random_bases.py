@@ -1,6 +1,6 @@
 """Randomised parity sweep of the 2-centre fast path: bases of random shape (1-7 shells,
 l <= 4, 1-8 primitives per shell with absent and padded slots, 1-4 species), random batch
-sizes (ragged tiles, a single pair) and stripe capacities, through the forward, the
+sizes (ragged tiles, a single pair) and stripe capacities (shared and pullback-only), through the forward, the
 (zeta, D) pullback and the position cotangents of the rrule, FP64 and FP32, against the
 oracle; then the 3-centre and AO paths likewise.
 
@@ -53,17 +53,20 @@ def _random_case(seed):
     sA = rng.integers(1, NZ + 1, size=B)
     sB = rng.integers(1, NZ + 1, size=B)
     cap = int(rng.choice([0, 16, 24, 112]))
-    return cb, A, Bp, sA, sB, cap
+    cap_pb = int(rng.choice([0, 16, 40]))  # the pullback's own capacity (0: cap)
+    return cb, A, Bp, sA, sB, cap, cap_pb
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
 @pytest.mark.parametrize("seed", list(range(16)))
 def test_random_basis_forward_and_pullback(dev, seed, dtype):
-    cb, A, Bp, sA, sB, cap = _random_case(seed)
+    cb, A, Bp, sA, sB, cap, cap_pb = _random_case(seed)
     B = len(sA)
-    log(f"2c random basis seed={seed} ls={cb.ls} K={cb.K} NZ={cb.NZ} B={B} cap={cap} dtype={dtype}")
+    log(f"2c random basis seed={seed} ls={cb.ls} K={cb.K} NZ={cb.NZ} B={B} cap={cap}/{cap_pb} dtype={dtype}")
     ctx = aok.context(0)
     ctx.tune("ovl_cap", cap)
+    ctx.tune("ovl_cap_pb", cap_pb)
+    assert ctx.tune_get("ovl_cap") == cap and ctx.tune_get("ovl_cap_pb") == cap_pb
     try:
         spec, _, _ = cb.ao_maps()
         R = aok.GaussianTypeRadials(dev_f(cb.zeta, dtype, dev), dev_f(cb.D, dtype, dev), spec, None,
@@ -76,6 +79,7 @@ def test_random_basis_forward_and_pullback(dev, seed, dtype):
         _, _, dXA, dXB, dps, _ = pb(dev_f(dS, dtype, dev))
     finally:
         ctx.tune("ovl_cap", 0)
+        ctx.tune("ovl_cap_pb", 0)
     z, D = rounded(cb.zeta, dtype), rounded(cb.D, dtype)
     rA, rB = rounded(A, dtype), rounded(Bp, dtype)
     coef = O.compile_coef(z, D, cb.ls)
