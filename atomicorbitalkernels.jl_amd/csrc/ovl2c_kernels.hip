@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void ovl_hist_kernel(const void *__restrict__ 
 // cotangents, so the pullback computes each once.
 // ---------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void ovl_plan_kernel(const OvlShape sh, int mode, const T *__restrict__ coefT,
+__global__ __launch_bounds__(1024) void ovl_plan_kernel(const OvlShape sh, int mode, const T *__restrict__ coefT,
                                                        const T *__restrict__ zetaT, const T *__restrict__ DT,
                                                        double *__restrict__ coef64, double *__restrict__ alpha64,
                                                        int *__restrict__ nreal, int *__restrict__ nlist,
@@ -1637,7 +1637,9 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
                        sh.NZ, reinterpret_cast<int *>(ws + w.rank), reinterpret_cast<int *>(ws + w.blk_hist),
                        ctx->d_err);
     AOK_HIP(ctx, hipGetLastError());
-    hipLaunchKernelGGL((ovl_plan_kernel<T>), dim3(1 + sh.NZ * sh.ns), dim3(256), 0, ctx->stream, sh, c.mode,
+    // 1024 threads: block 0's bucket offsets give each key's block histogram column to one of
+    // 16 waves (C2, 36 keys: 3 rounds of dependent loads instead of 9)
+    hipLaunchKernelGGL((ovl_plan_kernel<T>), dim3(1 + sh.NZ * sh.ns), dim3(1024), 0, ctx->stream, sh, c.mode,
                        static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha), static_cast<const T *>(c.D),
                        reinterpret_cast<double *>(ws + w.coef64), reinterpret_cast<double *>(ws + w.alpha64),
                        reinterpret_cast<int *>(ws + w.nreal), reinterpret_cast<int *>(ws + w.nlist),

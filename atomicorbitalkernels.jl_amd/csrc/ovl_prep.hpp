@@ -71,25 +71,52 @@ __device__ inline int64_t block_excl_scan64(int64_t v, int64_t *wtot, int64_t &t
 
 constexpr int OVL_MAXKEYS = 512;  // species pairs (16^2) or triples (8^3)
 
-// Bucket offsets from per-block histograms, inside one block of whole waves.  Every wave
-// takes keys w, w + nwave, ...; lane i owns a contiguous run of histogram blocks, and a
-// wave prefix sum gives each block's offset within the key.  The scan over keys then
-// gives boff (first element of each key's bucket), toff (first tile of each key, 64
-// elements per tile) and blk_off[b][key] = first output slot of block b's elements with
-// that key (stable order).
+// Bucket offsets from per-block histograms, inside one block of whole waves: each wave
+// sums its run of histogram blocks per key, the waves' exclusive prefix per key and the
+// scan over keys give boff (first element of each key's bucket), toff (first tile of each
+// key, 64 elements per tile) and blk_off[b][key] = first output slot of block b's elements
+// with that key (stable order).
 __device__ inline void bucket_offsets(int NK, int nblk_h, const int *__restrict__ blk_hist, int *__restrict__ blk_off,
                                       int *__restrict__ bcnt, int *__restrict__ boff, int *__restrict__ toff)
 {
+    // Wave w takes the histogram rows of blocks [w * per, (w + 1) * per), its lanes 64
+    // consecutive keys, so every load and store of a row segment is coalesced (one lane per
+    // block instead read one 4-byte word per cache line: C2's 1024 blocks x 36 keys took
+    // 75-85 us on the one CU).  Rows are read PF at a time, all in flight together.
+    constexpr int PF = 16;
+    constexpr int MAXW = 16;
     __shared__ int s_cnt[OVL_MAXKEYS];
     __shared__ int s_base[OVL_MAXKEYS];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int per = (nblk_h + 63) / 64;
-    const int b0 = lane * per, b1 = min(nblk_h, b0 + per);
-    for (int key = wave; key < NK; key += nwave) {
-        int s = 0;
-        for (int b = b0; b < b1; ++b) s += blk_hist[static_cast<int64_t>(b) * NK + key];
-        const int incl = wave_incl_scan(s);
-        if (lane == 63) s_cnt[key] = incl;
+    __shared__ int wpre[MAXW * OVL_MAXKEYS];  // per wave and key: its blocks' sum, then their offset
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwave = min(static_cast<int>(blockDim.x >> 6), MAXW);
+    const int per = (nblk_h + nwave - 1) / nwave;
+    const int b0 = wave * per, b1 = min(nblk_h, b0 + per);
+    if (wave < nwave) {
+        for (int k0 = 0; k0 < NK; k0 += 64) {
+            const int key = k0 + lane;
+            int s = 0;
+            if (key < NK)
+                for (int b = b0; b < b1; b += PF) {
+                    int v[PF];
+#pragma unroll
+                    for (int u = 0; u < PF; ++u) v[u] = b + u < b1 ? blk_hist[static_cast<int64_t>(b + u) * NK + key] : 0;
+#pragma unroll
+                    for (int u = 0; u < PF; ++u) s += v[u];
+                }
+            if (key < NK) wpre[wave * OVL_MAXKEYS + key] = s;
+        }
+    }
+    __syncthreads();
+    // per key: the waves' exclusive offsets and the key's total
+    for (int key = threadIdx.x; key < NK; key += blockDim.x) {
+        int c = 0;
+        for (int w = 0; w < nwave; ++w) {
+            const int x = wpre[w * OVL_MAXKEYS + key];
+            wpre[w * OVL_MAXKEYS + key] = c;
+            c += x;
+        }
+        s_cnt[key] = c;
     }
     __syncthreads();
     if (wave == 0) {
@@ -112,13 +139,21 @@ __device__ inline void bucket_offsets(int NK, int nblk_h, const int *__restrict_
         if (lane == 0) toff[NK] = carry_t;
     }
     __syncthreads();
-    for (int key = wave; key < NK; key += nwave) {
-        int s = 0;
-        for (int b = b0; b < b1; ++b) s += blk_hist[static_cast<int64_t>(b) * NK + key];
-        int run = s_base[key] + wave_incl_scan(s) - s;
-        for (int b = b0; b < b1; ++b) {
-            blk_off[static_cast<int64_t>(b) * NK + key] = run;
-            run += blk_hist[static_cast<int64_t>(b) * NK + key];
+    if (wave < nwave) {
+        for (int k0 = 0; k0 < NK; k0 += 64) {
+            const int key = k0 + lane;
+            if (key >= NK) continue;
+            int run = s_base[key] + wpre[wave * OVL_MAXKEYS + key];
+            for (int b = b0; b < b1; b += PF) {
+                int v[PF];
+#pragma unroll
+                for (int u = 0; u < PF; ++u) v[u] = b + u < b1 ? blk_hist[static_cast<int64_t>(b + u) * NK + key] : 0;
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    if (b + u < b1) blk_off[static_cast<int64_t>(b + u) * NK + key] = run;
+                    run += v[u];
+                }
+            }
         }
     }
 }
