@@ -1344,12 +1344,20 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
 // ---------------------------------------------------------------------------------
 // Output stages: cast, or normalisation VJP (overlap_2c_grad.jl:251-304) in FP64.
 // ---------------------------------------------------------------------------------
-// Parameter cotangent q summed over its nparts partial sums (stride pstride), in order.
+// Parameter cotangent q summed over its nparts partial sums (stride pstride), in order;
+// eight loads in flight at a time (one at a time, each part cost a memory latency).
 __device__ __forceinline__ double sum_parts(const double *__restrict__ p, int nparts, int64_t pstride, int64_t q)
 {
     double v = 0.0;
     if (p)
-        for (int i = 0; i < nparts; ++i) v += p[i * pstride + q];
+        for (int i0 = 0; i0 < nparts; i0 += 8) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = i0 + u < nparts ? p[(i0 + u) * pstride + q] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (i0 + u < nparts) v += x[u];
+        }
     return v;
 }
 
