@@ -1295,6 +1295,12 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
     const int li = pcls[(sg * ns + s) * K + k];
     const int side = blockIdx.y / NZ, partner = blockIdx.y - side * NZ;
     const int key = side == 0 ? sg * NZ + partner : partner * NZ + sg;
+    // loaded before the barriers below (the compiler keeps global loads on their side), so
+    // their latencies overlap the chunk compaction
+    const int t0 = toff[key], t1 = toff[key + 1];
+    const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
+    const double al = alpha64[dst], cf = coef64[dst];
+    const int nr = nreal[sg * ns + s];
     // relevant chunks, compacted in chunk order with ballots
     const int ci = threadIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     bool use = false;
@@ -1323,7 +1329,7 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
         rel_col[pos] = col;
     }
     __syncthreads();
-    const int t0 = toff[key], nw = (toff[key + 1] - t0) * nrel;
+    const int nw = (t1 - t0) * nrel;
     double c = 0.0, a = 0.0;
     for (int w = threadIdx.x; w < nw; w += 256) {
         const int t = w / nrel, r = w - t * nrel;
@@ -1333,9 +1339,7 @@ __global__ __launch_bounds__(256) void ovl2c_bwd_reduce(const OvlShape sh, const
     }
     block_sum256x2(c, a, red);
     if (threadIdx.x == 0) {
-        const int64_t dst = s + static_cast<int64_t>(ns) * (k + static_cast<int64_t>(K) * sg);
-        const double al = alpha64[dst];
-        if (li < nreal[sg * ns + s] && al > 0.0) a -= (2 * sh.ls[s] + 3) / (4.0 * al) * coef64[dst] * c;
+        if (li < nr && al > 0.0) a -= (2 * sh.ls[s] + 3) / (4.0 * al) * cf * c;
         part[blockIdx.y * npar + dst] = c;
         part[(2 * NZ + blockIdx.y) * npar + dst] = a;
     }
