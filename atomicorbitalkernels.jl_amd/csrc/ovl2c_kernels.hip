@@ -1650,7 +1650,7 @@ static aok_status fast_prep(aok_ctx ctx, const Ovl2cCall &c, const OvlShape &sh,
                        ctx->d_err);
     AOK_HIP(ctx, hipGetLastError());
     // 1024 threads: block 0's bucket offsets give each key's block histogram column to one of
-    // 16 waves (C2, 36 keys: 3 rounds of dependent loads instead of 9)
+    // 16 waves (36 keys: 3 passes of dependent loads per wave instead of 9)
     hipLaunchKernelGGL((ovl_plan_kernel<T>), dim3(1 + sh.NZ * sh.ns), dim3(1024), 0, ctx->stream, sh, c.mode,
                        static_cast<const T *>(c.coef), static_cast<const T *>(c.alpha), static_cast<const T *>(c.D),
                        reinterpret_cast<double *>(ws + w.coef64), reinterpret_cast<double *>(ws + w.alpha64),
