@@ -25,6 +25,17 @@ from aok import configs
 from helpers import dev_f, dev_rows, host, log, rel_max, rounded
 
 pytestmark = pytest.mark.gpu
+# Results near the bottom of a format's range carry no relative precision (their terms and
+# intermediate products underflow first): a case whose values all lie below the floor (far-apart
+# centres, tight exponents) is judged against the floor instead of its own tiny maximum.
+F32_FLOOR = 1e-25
+F64_FLOOR = 1e-290
+
+
+def _rel(got, ref, f64, scale=0.0):
+    """max|got - ref| over the largest of max|ref|, `scale` and the format's floor."""
+    d = np.max(np.abs(got - ref)) if got.size else 0.0
+    return d / max(np.max(np.abs(ref)) if ref.size else 0.0, scale, F64_FLOOR if f64 else F32_FLOOR)
 
 
 def _random_case(seed):
@@ -93,18 +104,18 @@ def test_random_basis_forward_and_pullback(dev, seed, dtype):
     assert np.all(np.isfinite(Sg)) and np.all(np.isfinite(dz)) and np.all(np.isfinite(dD))
     bmax = np.max(np.abs(Sr), axis=(0, 1))
     kappa_b = np.max(np.abs(Sabs), axis=(0, 1)) / np.maximum(bmax, 1e-300)
-    normal = bmax >= (1e-300 if f64 else 1e-30)
+    normal = bmax >= (F64_FLOOR if f64 else 1e-30)
     err_b = np.max(np.abs(Sg - Sr), axis=(0, 1)) / np.maximum(bmax, 1e-300)
     kappa = float(kappa_b[normal].max()) if normal.any() else 1.0
     es = float((err_b[normal] / np.maximum(kappa_b[normal], 1.0)).max()) if normal.any() else 0.0
     tiny = float(np.abs(Sg[:, :, ~normal]).max()) if (~normal).any() else 0.0
     live = D != 0
     raw_D = np.abs(dc[live] * coef[live] / D[live]).max() if live.any() else 0.0
-    scale_D = max(np.abs(dDr).max(), raw_D, 1e-300)
-    ez = rel_max(dz, dzr)
+    scale_D = max(np.abs(dDr).max(), raw_D, F64_FLOOR if f64 else F32_FLOOR)
+    ez = _rel(dz, dzr, f64)
     eD = float(np.abs(dD - dDr).max() / scale_D)
     rXA, _ = O.ovl2c_dx(rounded(dS, dtype), rA, rB, sA, sB, cb.ls, coef, z)
-    ex = rel_max(host(dXA), rXA)
+    ex = _rel(host(dXA), rXA, f64)
     print(f"seed {seed} {dtype}: S {es:.2e} (per kappa; kappa {kappa:.1f}, {(~normal).sum()} blocks"
           f" below FP32 range, max {tiny:.1e}) dzeta {ez:.2e} dD {eD:.2e} dA {ex:.2e}")
     assert torch.equal(dXB, -dXA)
@@ -145,7 +156,9 @@ def test_random_basis_3c_forward_and_pullback(dev, seed, dtype):
     1-3 species, batches of 1 to 130 triples (ragged tiles); the rrule forward and its
     (zeta, D) pullback.  FP32 bounds and metrics as in the 2-centre sweep: 3e-6 x max(1,
     kappa_b) for values, 1e-5 x max(1, kappa / 5) for cotangents, dD against the larger of its
-    own scale and the coefficient cotangent's before the normalisation projection."""
+    own scale and the coefficient cotangent's before the normalisation projection; dzeta
+    against the larger of its own scale and a tenth of its terms' magnitudes summed (so at
+    most 1e-6 of those: about 17 FP32 ulp)."""
     rng = np.random.default_rng(2000 + seed)
     ns = int(rng.integers(1, 6))
     ls = sorted(int(x) for x in rng.integers(0, 3, size=ns))
@@ -168,20 +181,24 @@ def test_random_basis_3c_forward_and_pullback(dev, seed, dtype):
     dc, da = O.ovl3c_pb(rounded(dT, dtype), *pos, *S3, cb.ls, coef, z)
     dDr, dzn = O.compile_coef_pb(dc, z, D, cb.ls)
     dzr = da + dzn
+    # the exponent cotangent's own conditioning: its terms' magnitudes summed (|dT|, |coef|)
+    # against the result, which cancels over the elements of a block and then against the
+    # normalisation VJP (up to ~500x for a single triple or a 2-primitive d shell)
+    _, da_abs = O.ovl3c_pb(np.abs(rounded(dT, dtype)), *pos, *S3, cb.ls, np.abs(coef), z)
     f64 = dtype == torch.float64
     Tg, dz, dD = host(T), host(dps["Rnl"]["zeta"]), host(dps["Rnl"]["D"])
     assert np.all(np.isfinite(Tg)) and np.all(np.isfinite(dz)) and np.all(np.isfinite(dD))
     bmax = np.max(np.abs(Tr), axis=(0, 1, 2))
     kappa_b = np.max(np.abs(Tabs), axis=(0, 1, 2)) / np.maximum(bmax, 1e-300)
-    normal = bmax >= (1e-300 if f64 else 1e-30)
+    normal = bmax >= (F64_FLOOR if f64 else 1e-30)
     err_b = np.max(np.abs(Tg - Tr), axis=(0, 1, 2)) / np.maximum(bmax, 1e-300)
     kappa = float(kappa_b[normal].max()) if normal.any() else 1.0
     es = float((err_b[normal] / np.maximum(kappa_b[normal], 1.0)).max()) if normal.any() else 0.0
     tiny = float(np.abs(Tg[..., ~normal]).max()) if (~normal).any() else 0.0
     live = D != 0
     raw_D = np.abs(dc[live] * coef[live] / D[live]).max() if live.any() else 0.0
-    scale_D = max(np.abs(dDr).max(), raw_D, 1e-300)
-    ez = rel_max(dz, dzr)
+    scale_D = max(np.abs(dDr).max(), raw_D, F64_FLOOR if f64 else F32_FLOOR)
+    ez = _rel(dz, dzr, f64, scale=0.1 * np.abs(da_abs).max())
     eD = float(np.abs(dD - dDr).max() / scale_D)
     print(f"3c seed {seed} {dtype}: T {es:.2e} (per kappa; kappa {kappa:.1f}) dzeta {ez:.2e} dD {eD:.2e}"
           f" (|dD| {np.abs(dDr).max():.2e} of the unprojected {raw_D:.2e})")
@@ -199,7 +216,8 @@ def test_random_basis_ao(dev, seed, dtype):
     ragged point counts (1 to 20000); evaluate, evaluate_ed and the rrule pullback
     against the oracle.  Tolerances as the fixed-basis AO tests (relative to each output's
     largest entry): FP64 1e-12 values and gradients, 1e-11 pullbacks; FP32 2e-6, 1e-5, 2e-5,
-    each times max(1, kappa), kappa = max|P| with |D| / max|P| (mixed-sign radials)."""
+    each times max(1, kappa, kappa_e / 20), kappa = max|P| with |D| / max|P| (mixed-sign
+    radials), kappa_e = the relative change of P per relative change of the exponents."""
     rng = np.random.default_rng(3000 + seed)
     ns = int(rng.integers(1, 7))
     ls = sorted(int(x) for x in rng.integers(0, 5, size=ns))
@@ -224,12 +242,18 @@ def test_random_basis_ao(dev, seed, dtype):
     Pa = O.ao_eval(*args, np.abs(rounded(cb.D, dtype)), *maps)
     dXr, dzr, dDr = O.ao_pullback(*args, rounded(cb.D, dtype), *maps, rounded(dPin, dtype))
     kappa = max(1.0, float(np.abs(Pa).max() / max(np.abs(Pr).max(), 1e-300)))
+    # exponent conditioning: e^{-zeta r^2} moves by zeta r^2 times a relative change of its
+    # argument, which FP32 rounds (a point far from a tight primitive: zeta r^2 ~ 50 gives
+    # 3e-6 alone); measured as the relative change of P under zeta (1 + 1e-6)
+    Pz = O.ao_eval(args[0], args[1], args[2] * (1 + 1e-6), rounded(cb.D, dtype), *maps)
+    kappa_e = rel_max(Pz, Pr) / 1e-6
+    kappa = max(kappa, kappa_e / 20)
     tv, tg, tp = (1e-12, 1e-12, 1e-11) if dtype == torch.float64 else (2e-6 * kappa, 1e-5 * kappa, 2e-5 * kappa)
     e = {"P": rel_max(P, Pr), "P_ed": rel_max(P2, Pr),
          "dP": max(rel_max(dP[:, :, c], dPr[:, :, c]) for c in range(3)),
          "dX": rel_max(host(dX), dXr), "dzeta": rel_max(host(dps["Rnl"]["zeta"]), dzr),
          "dD": rel_max(host(dps["Rnl"]["D"]), dDr)}
-    print(f"ao seed {seed} {dtype}: kappa {kappa:.1f} " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    print(f"ao seed {seed} {dtype}: kappa {kappa:.1f} (exponent {kappa_e:.1f}) " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
     assert e["P"] < tv and e["P_ed"] < tv, e
     assert e["dP"] < tg, e
     assert e["dX"] < tp and e["dzeta"] < tp and e["dD"] < tp, e
