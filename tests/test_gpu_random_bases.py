@@ -69,7 +69,7 @@ def _random_case(seed):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
-@pytest.mark.parametrize("seed", list(range(16)))
+@pytest.mark.parametrize("seed", list(range(32)))
 def test_random_basis_forward_and_pullback(dev, seed, dtype):
     cb, A, Bp, sA, sB, cap, cap_pb = _random_case(seed)
     B = len(sA)
@@ -150,7 +150,7 @@ def _orb(cb, dtype, dev):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
-@pytest.mark.parametrize("seed", list(range(16)))
+@pytest.mark.parametrize("seed", list(range(32)))
 def test_random_basis_3c_forward_and_pullback(dev, seed, dtype):
     """3-centre fast path (shells up to d): random bases of 1-5 shells, 1-6 primitive slots,
     1-3 species, batches of 1 to 130 triples (ragged tiles); the rrule forward and its
@@ -210,7 +210,7 @@ def test_random_basis_3c_forward_and_pullback(dev, seed, dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
-@pytest.mark.parametrize("seed", list(range(16)))
+@pytest.mark.parametrize("seed", list(range(32)))
 def test_random_basis_ao(dev, seed, dtype):
     """AO path: random bases of 1-6 shells up to l = 4, 1-6 primitive slots, 1-4 species,
     ragged point counts (1 to 20000); evaluate, evaluate_ed and the rrule pullback
