@@ -421,3 +421,61 @@ def test_batch_cap_rejected_before_launch(dev):
         ctx.call("aok_ovl3c", aok._lib.AOK_F32, 0, p, p, p, None, None, None, aok._lib.AOK_I64, (1 << 28) + 1, cb.ns,
                  cb.K, cb.NZ, ls, p, p, 1.0, p)
     ctx.close()
+
+
+@pytest.mark.parametrize("dtype,B", [(torch.float64, 2_400_000), (torch.float32, 4_800_000)], ids=["f64", "f32"])
+def test_forward_output_beyond_4gib(dev, dtype, B):
+    """An output of more than 4 GiB (C1 basis, N = 15: 4.32 GB) takes the forward's 64-bit
+    write-out addressing instead of the 32-bit byte offsets; the first, last (ragged tile) and
+    300 scattered pairs' blocks against the oracle."""
+    d = configs.c1(1, B=B)
+    cb = d["basis"]
+    assert B * cb.N * cb.N * torch.empty((), dtype=dtype).element_size() >= 1 << 32
+    log(f"2c forward beyond 4 GiB B={B} dtype={dtype}")
+    orb = _orb(cb, dtype, dev)
+    pA, pB = dev_rows(d["posA"], dtype, dev), dev_rows(d["posB"], dtype, dev)
+    sA = torch.as_tensor(d["sidxA"], device=dev)
+    sB = torch.as_tensor(d["sidxB"], device=dev)
+    S = aok.batch_overlap(orb, (pA, sA), (pB, sB), orb.static_params(), None)
+    idx = np.concatenate([np.arange(300), np.arange(B - 300, B),
+                          np.sort(np.random.default_rng(5).choice(np.arange(300, B - 300), 300, replace=False))])
+    got = host(S[:, :, torch.as_tensor(idx, device=dev)])
+    del S
+    torch.cuda.empty_cache()
+    coef = O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls)
+    ref = O.ovl2c(rounded(d["posA"][idx], dtype), rounded(d["posB"][idx], dtype), d["sidxA"][idx], d["sidxB"][idx],
+                  cb.ls, coef, rounded(cb.zeta, dtype))
+    err = blockwise_rel(got, ref)
+    print(f"beyond 4 GiB {dtype}: blockwise {err:.3e} over {len(idx)} pairs")
+    assert err < (1e-12 if dtype == torch.float64 else 1e-6), err
+
+
+def test_pullback_cotangent_beyond_4gib(dev):
+    """A 4.32 GB FP32 cotangent (C1 basis, B = 4.8M): only the last 300 pairs and 300 scattered
+    pairs past the 4 GiB mark carry a nonzero block, so the (coef, alpha) cotangents are those
+    pairs' alone and the oracle checks them on that subset (64-bit addressing of dS)."""
+    dtype, B = torch.float32, 4_800_000
+    d = configs.c1(1, B=B)
+    cb = d["basis"]
+    N = cb.N
+    first_high = -(-(1 << 32) // (N * N * 4))  # first pair whose block starts past 4 GiB
+    rng = np.random.default_rng(9)
+    idx = np.concatenate([np.sort(rng.choice(np.arange(first_high, B - 300), 300, replace=False)),
+                          np.arange(B - 300, B)])
+    log(f"2c pullback cotangent beyond 4 GiB B={B} first_high={first_high}")
+    coef = rng.standard_normal((cb.ns, cb.K, cb.NZ)) * (cb.D != 0)
+    basis = _cart_basis(cb.ls, cb.zeta, coef, dtype, dev)
+    XA = (dev_rows(d["posA"], dtype, dev), torch.as_tensor(d["sidxA"], device=dev))
+    XB = (dev_rows(d["posB"], dtype, dev), torch.as_tensor(d["sidxB"], device=dev))
+    vals = rng.standard_normal((N, N, len(idx)))
+    dS = torch.zeros((B, N, N), dtype=dtype, device=dev).permute(2, 1, 0)
+    dS[:, :, torch.as_tensor(idx, device=dev)] = torch.as_tensor(vals, dtype=dtype, device=dev)
+    dc, da = aok.overlap_pb_coef_alpha(dS, basis, XA, XB)
+    dc, da = host(dc), host(da)
+    del dS
+    torch.cuda.empty_cache()
+    dcr, dar = O.ovl2c_pb(rounded(vals, dtype), rounded(d["posA"][idx], dtype), rounded(d["posB"][idx], dtype),
+                          d["sidxA"][idx], d["sidxB"][idx], cb.ls, rounded(coef, dtype), rounded(cb.zeta, dtype))
+    print(f"pullback beyond 4 GiB: dcoef {rel_max(dc, dcr):.3e} dalpha {rel_max(da, dar):.3e}")
+    assert rel_max(dc, dcr) < 1e-5
+    assert rel_max(da, dar) < 1e-5
