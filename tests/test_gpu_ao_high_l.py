@@ -170,3 +170,48 @@ def test_reference_multi_species_fp32(dev):
     for got, ref in ((host(dps["Rnl"]["zeta"]), dzc), (host(dps["Rnl"]["D"]), dDc)):
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-2
         assert rel_max(got, ref) < 2e-5
+
+
+def test_outputs_beyond_4gib(dev):
+    """P of 4.32 GB and dP of 13 GB (the 4s3p2d1f basis, 30 orbitals, nX = 36M, FP32): values and
+    gradients of the first, last and 300 scattered points against the oracle, and the rrule
+    pullback of a 4.32 GB cotangent that is nonzero only at those points (64-bit column
+    addressing of P, dP and the cotangent)."""
+    cb = configs.c4b_basis(4)
+    nX = 36_000_000
+    assert nX * cb.nOrb * 4 >= 1 << 32
+    rng = np.random.default_rng(61)
+    X = rng.normal(0.0, 3.0, size=(nX, 3)).astype(np.float32)
+    sp = rng.integers(1, cb.NZ + 1, nX).astype(np.int32)
+    idx = np.concatenate([np.arange(300), np.arange(nX - 300, nX),
+                          np.sort(rng.choice(np.arange(300, nX - 300), 300, replace=False))])
+    log(f"ao beyond 4 GiB nX={nX}")
+    orb = _orb(cb, torch.float32, dev)
+    Xd = torch.as_tensor(X, device=dev)
+    sidx = torch.as_tensor(sp, device=dev)
+    ti = torch.as_tensor(idx, device=dev)
+    P = aok.evaluate(orb, Xd, sidx=sidx)
+    Pg = host(P[ti])
+    del P
+    P2, dP = aok.evaluate_ed(orb, Xd, sidx=sidx)
+    P2g, dPg = host(P2[ti]), host(dP[ti])
+    del P2, dP
+    dPin = torch.zeros((nX, cb.nOrb), dtype=torch.float32, device=dev)
+    vals = rng.standard_normal((len(idx), cb.nOrb))
+    dPin[ti] = torch.as_tensor(vals, dtype=torch.float32, device=dev)
+    _, pb = aok.rrule_evaluate(orb, Xd, sidx=sidx)
+    _, _, dX, dps, _ = pb(dPin)
+    dXg, dzg, dDg = host(dX[ti]), host(dps["Rnl"]["zeta"]), host(dps["Rnl"]["D"])
+    del dPin, dX, Xd
+    torch.cuda.empty_cache()
+    Pr, dPr = _oracle(cb, X[idx].astype(np.float64), sp[idx], torch.float32, grad=True)
+    _, iR, iY = cb.ao_maps()
+    dXr, dzr, dDr = O.ao_pullback(X[idx].astype(np.float64), sp[idx], rounded(cb.zeta, torch.float32),
+                                  rounded(cb.D, torch.float32), np.zeros(cb.ns, np.int64), np.array(iR),
+                                  np.array(iY), max(cb.ls), rounded(vals, torch.float32))
+    tv, tg, tp = TOL[torch.float32]
+    assert rel_max(Pg, Pr) < tv and rel_max(P2g, Pr) < tv
+    for c in range(3):
+        assert rel_max(dPg[:, :, c], dPr[:, :, c]) < tg
+    assert rel_max(dXg, dXr) < tp
+    assert rel_max(dzg, dzr) < tp and rel_max(dDg, dDr) < tp

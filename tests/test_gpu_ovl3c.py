@@ -88,3 +88,28 @@ def test_single_species_reduces_to_reference_form(dev):
     assert np.array_equal(out1, out3)
     ref = O.ovl3c(*(d[f"pos{c}"] for c in "ABC"), None, None, None, cb.ls, coef[:, :, :1], cb.zeta[:, :, :1])
     assert _blockwise(out1, ref) < 1e-12
+
+
+def test_forward_output_beyond_4gib(dev):
+    """A 4.32 GB FP32 output (C4a basis, N = 15, B = 320,000 triples): the first, last (ragged
+    tile) and 100 scattered triples' blocks against the oracle (64-bit block addressing)."""
+    B = 320_000
+    d = configs.c4a(B=B)
+    cb = d["basis"]
+    assert B * cb.N ** 3 * 4 >= 1 << 32
+    log(f"3c forward beyond 4 GiB B={B}")
+    dtype = torch.float32
+    coef = O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls)
+    basis = _basis(cb.ls, cb.zeta, coef, dtype, dev)
+    X = [(dev_rows(d[f"pos{c}"], dtype, dev), torch.as_tensor(d[f"sidx{c}"], device=dev)) for c in "ABC"]
+    out = aok.batch_overlap_3c(basis, *X)
+    idx = np.concatenate([np.arange(100), np.arange(B - 100, B),
+                          np.sort(np.random.default_rng(3).choice(np.arange(100, B - 100), 100, replace=False))])
+    got = host(out[:, :, :, torch.as_tensor(idx, device=dev)])
+    del out
+    torch.cuda.empty_cache()
+    ref = O.ovl3c(*(rounded(d[f"pos{c}"][idx], dtype) for c in "ABC"), d["sidxA"][idx], d["sidxB"][idx],
+                  d["sidxC"][idx], cb.ls, rounded(coef, dtype), rounded(cb.zeta, dtype))
+    err = _blockwise(got, ref)
+    print(f"3c beyond 4 GiB: blockwise {err:.3e} over {len(idx)} triples")
+    assert err < 2e-6, err
