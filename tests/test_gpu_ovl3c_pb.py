@@ -178,3 +178,32 @@ def test_edges(dev):
     P = (dev_rows(np.zeros((1, 3)), torch.float64, dev), None)
     with pytest.raises(aok.AOKLmaxError):
         aok.overlap_3c_pb_coef_alpha(torch.zeros((N, N, N, 1), dtype=torch.float64, device=dev), b3, P, P, P)
+
+
+def test_pullback_cotangent_beyond_4gib(dev):
+    """A 4.32 GB FP32 cotangent (C4a basis, B = 320,000 triples) nonzero only on the last 40
+    triples and 40 scattered ones past the 4 GiB mark (the oracle takes ~0.1 s per triple): the (coef, alpha) cotangents are those
+    triples' alone, checked against the oracle on that subset (64-bit addressing of dT)."""
+    dtype, B = torch.float32, 320_000
+    d = configs.c4a(B=B)
+    cb = d["basis"]
+    N = cb.N
+    first_high = -(-(1 << 32) // (N ** 3 * 4))
+    rng = np.random.default_rng(13)
+    idx = np.concatenate([np.sort(rng.choice(np.arange(first_high, B - 40), 40, replace=False)),
+                          np.arange(B - 40, B)])
+    log(f"3c pullback cotangent beyond 4 GiB B={B} first_high={first_high}")
+    coef = rounded(O.compile_coef(rounded(cb.zeta, dtype), rounded(cb.D, dtype), cb.ls), dtype)
+    basis = _basis(cb.ls, cb.zeta, coef, dtype, dev)
+    vals = rng.standard_normal((N, N, N, len(idx)))
+    dT = torch.zeros((B, N, N, N), dtype=dtype, device=dev).permute(3, 2, 1, 0)
+    dT[:, :, :, torch.as_tensor(idx, device=dev)] = torch.as_tensor(vals, dtype=dtype, device=dev)
+    dc, da = aok.overlap_3c_pb_coef_alpha(dT, basis, *_inputs(d, dtype, dev))
+    dc, da = host(dc), host(da)
+    del dT
+    torch.cuda.empty_cache()
+    rc, ra = O.ovl3c_pb(rounded(vals, dtype), *(rounded(d[f"pos{c}"][idx], dtype) for c in "ABC"), d["sidxA"][idx],
+                        d["sidxB"][idx], d["sidxC"][idx], cb.ls, coef, rounded(cb.zeta, dtype))
+    print(f"3c pullback beyond 4 GiB: dcoef {rel_max(dc, rc):.3e} dalpha {rel_max(da, ra):.3e}")
+    assert rel_max(dc, rc) < 1e-5
+    assert rel_max(da, ra) < 1e-5
